@@ -1,0 +1,114 @@
+/*
+ * pittpack_hip.h -- C ABI of the MI355X (gfx950) PittPack solve path.
+ *
+ * The entry points mirror the public set-up/solve interface of the reference's
+ * PencilDcmp (src/include/pencilDcmp.hpp:441-466) as main.cpp drives it
+ * (src/main.cpp:73-118): construct -> assignBoundary -> setBox -> setCoords ->
+ * assignRhs -> pittPack -> read the solution -> destroy.  Plain pointers and sizes only.
+ *
+ * Return values are PittPack's error codes (src/include/params.h:86-114, same numbers and
+ * meanings: 0 SUCCESS, 15 ALLOCATION_FAIL, 19 INPUT_ARGS, ...) plus the codes >= 24 below.
+ * No entry point calls exit(); every failure is returned.
+ *
+ * Threading: a handle owns its device buffers and one HIP stream; different handles may be
+ * used from different threads; one handle must not be used from two threads at once.
+ */
+#ifndef PITTPACK_HIP_H
+#define PITTPACK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pp_solver *pp_handle;
+
+/* codes beyond the reference's 0..23 */
+enum {
+    PP_SUCCESS = 0,
+    PP_ALLOCATION_FAIL = 15,    /* reference ALLOCATION_FAIL */
+    PP_INPUT_ARGS = 19,         /* reference INPUT_ARGS */
+    PP_HIP_ERROR = 24,          /* a HIP runtime call failed */
+    PP_UNSUPPORTED_BC = 25,     /* boundary-condition combination not available */
+    PP_UNSUPPORTED_SIZE = 26,   /* no compiled transform / z-solve for this size */
+    PP_INVALID_HANDLE = 27,
+    PP_CALL_ORDER = 28,         /* e.g. pittPack before assignBoundary / setCoords / RHS */
+    PP_NO_DEVICE = 29           /* no HIP device visible */
+};
+
+/* option keys for pp_set_option */
+enum {
+    /* 0: reference row-0 rule for the singular (0,0) column whenever |eig+2| < 1e-10
+     *    (src/triDiag.cpp:1753-1776);
+     * 1 (default): apply it only when the z operator is singular (z Neumann-Neumann); with a
+     *    Dirichlet z face the true system is solved. */
+    PP_OPT_ZERO_MODE = 1,
+    /* 1: record per-stage HIP events during pp_pittpack (pp_get_stage_times) */
+    PP_OPT_TIMING = 2
+};
+
+/* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
+ * Global grid sizes (one GPU holds the whole grid).  Allocates the device field. */
+int pp_create(int nx, int ny, int nz, pp_handle *out);
+
+/* ~PencilDcmp (src/pencilDcmp.cpp:528-658) */
+int pp_destroy(pp_handle h);
+
+/* PencilDcmp::assignBoundary(char bc[6]) (src/pencilDcmp.cpp:3578-3634).
+ * bc = faces xmin,xmax,ymin,ymax,zmin,zmax, each 'P', 'N' or 'D'.  x and y take PP, NN, DD,
+ * and (extension) ND / DN; z takes DD, NN, DN, ND. */
+int pp_assign_boundary(pp_handle h, const char *bc);
+
+/* PencilDcmp::setBox(double X[6]) (src/pencilDcmp.cpp:470-480) */
+int pp_set_box(pp_handle h, const double *X);
+
+/* PencilDcmp::setCoords(int dir) (src/pencilDcmp.cpp:748-840); dir = 2 as main.cpp uses */
+int pp_set_coords(pp_handle h, int dir);
+
+/* PencilDcmp::assignRhs(double *rhs) (src/pencilDcmp.cpp:5591-5647): host array, real,
+ * nx*ny*nz, x fastest, unscaled.  Copied to the device field. */
+int pp_assign_rhs(pp_handle h, const double *rhs);
+
+/* Same, from a device pointer (nx*ny*nz doubles, x fastest). */
+int pp_assign_rhs_device(pp_handle h, const double *d_rhs);
+
+/* PencilDcmp::pittPack() (src/pencilDcmp.cpp:5801-6430): solve in place on the device field.
+ * Blocks until the solve has finished on the handle's stream. */
+int pp_pittpack(pp_handle h);
+
+/* No reference counterpart (P is protected): copy the solution to a host array. */
+int pp_get_solution(pp_handle h, double *u);
+
+/* No reference counterpart: copy the solution to a device array. */
+int pp_get_solution_device(pp_handle h, double *d_u);
+
+/* Flow-solver form (no reference counterpart): solve in place on a caller-owned device array
+ * of nx*ny*nz doubles holding the RHS on entry and the solution on exit, on the handle's
+ * stream, without a host synchronisation. */
+int pp_solve_device(pp_handle h, double *d_field);
+
+/* Use the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
+int pp_set_stream(pp_handle h, void *stream);
+
+int pp_set_option(pp_handle h, int key, long long value);
+
+/* Durations (ms) of the five stages of the last timed pp_pittpack / pp_solve_device:
+ * x forward, y forward, z solve, y inverse, x inverse.  Requires PP_OPT_TIMING. */
+int pp_get_stage_times(pp_handle h, float *ms5);
+
+/* Device pointer of the handle's field (nx*ny*nz doubles). */
+int pp_get_device_field(pp_handle h, double **d_field);
+
+/* PittPackGetErrorEnum (src/pencilDcmp.cpp:4869-4913) extended with the codes above. */
+const char *pp_get_error_enum(int code);
+
+/* 1 if the transform length n is compiled in, else 0 */
+int pp_size_supported(int n);
+
+/* GPU self test of the 16-lane rotation the z solve relies on; 0 when it behaves as assumed */
+int pp_selftest(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PITTPACK_HIP_H */

@@ -1,0 +1,805 @@
+/*
+ * pp_oracle.c -- CPU oracle for the PittPack solve path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This file restates, in plain C and FP64, what PittPack's single-rank solve computes:
+ * PencilDcmp::pittPack() (src/pencilDcmp.cpp:5801-6430) and what it calls.  It is the
+ * checker for the HIP implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path (pittpack_amd/) never links or calls it.
+ *
+ * Storage follows the reference: P holds 2*nx*ny*nz doubles, complex interleaved, x fastest
+ * (src/chunkedArray.cpp:13-59,163-238); the imaginary slot is scratch for the DCT/DST steps.
+ * Each function cites the reference lines it restates.  Where the reference is undefined
+ * (per-axis N-D / D-N faces: transform 'U' with uninitialised num/denum,
+ * src/pencilDcmp.cpp:340-341,3533-3576) the oracle implements the quarter-wave extension
+ * (DCT-IV / DST-IV, eigenvalues -2+2cos((k+1/2)pi/N)); that part has no reference
+ * counterpart and is pinned by closed forms (scipy's DCT-IV, dense solves) in tests/.
+ *
+ * Pinning (tests/test_oracle.py): FFT against numpy; each signalProc step against scipy's
+ * DCT/DST definitions; Thomas / Sherman-Morrison against dense LAPACK solves; the whole
+ * solve against a dense 7-point ghost-cell operator on tiny grids, the discrete residual
+ * and second-order convergence on manufactured solutions.  Parity unpinned (no reference
+ * run is possible here, see the project notes): the exact (0,0)-mode constant of the
+ * row-0 rule under z-NN beyond "rows 1..n-1 of the original system hold and u0=r0", and
+ * the singular periodic-z averaging of shermanMorrisonThomasV1.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PPO_PI 3.1415926535897932384 /* src/include/params.h:116 */
+
+/* ------------------------------------------------------------------------------------ */
+/* Complex FFT of any length (stand-in for FFTW's fftw_plan_dft_1d, unnormalised,        */
+/* FORWARD = exp(-2 pi i jk/n), BACKWARD = exp(+...), as called at                       */
+/* src/poissonCPU.cpp:56,87,162,211).  Mixed-radix Stockham, radices 4,2,3,5 and a       */
+/* direct DFT for any other prime factor.                                                */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    int nf;
+    int fac[64];
+    double *w; /* w[e] = exp(-2 pi i e / n), e = 0..n-1, interleaved */
+} ppo_fft_plan;
+
+static void ppo_fft_plan_init(ppo_fft_plan *pl, int n)
+{
+    pl->n = n;
+    pl->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { pl->fac[pl->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { pl->fac[pl->nf++] = 2; m /= 2; }
+    while (m % 3 == 0) { pl->fac[pl->nf++] = 3; m /= 3; }
+    while (m % 5 == 0) { pl->fac[pl->nf++] = 5; m /= 5; }
+    for (int p = 7; m > 1; p += 2) {
+        while (m % p == 0) { pl->fac[pl->nf++] = p; m /= p; }
+    }
+    pl->w = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    for (int e = 0; e < n; e++) {
+        double a = -2.0 * PPO_PI * (double)e / (double)n;
+        pl->w[2 * e] = cos(a);
+        pl->w[2 * e + 1] = sin(a);
+    }
+}
+
+static void ppo_fft_plan_free(ppo_fft_plan *pl) { free(pl->w); pl->w = NULL; }
+
+/* in-place transform of one complex line x[2n]; work[2n] scratch; sign -1 forward, +1 backward */
+static void ppo_fft_exec(const ppo_fft_plan *pl, double *x, double *work, int sign)
+{
+    const int n = pl->n;
+    if (n <= 1) return;
+    double *src = x, *dst = work;
+    int p = 1;
+    double u[2 * 64], v[2 * 64];
+    double *ub = u, *vb = v, *big = NULL;
+    for (int f = 0; f < pl->nf; f++) {
+        const int R = pl->fac[f];
+        if (R > 64 && big == NULL) {
+            big = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+        }
+        double *uu = (R > 64) ? big : ub;
+        double *vv = (R > 64) ? big + 2 * n : vb;
+        const int nr = n / R;
+        for (int i = 0; i < nr; i++) {
+            const int k = i % p;
+            for (int r = 0; r < R; r++) {
+                double re = src[2 * (i + r * nr)], im = src[2 * (i + r * nr) + 1];
+                if (p > 1 && r > 0) {
+                    long e = ((long)r * k * (n / (p * R))) % n;
+                    double c = pl->w[2 * e], s = -sign * pl->w[2 * e + 1];
+                    double t = re * c - im * s;
+                    im = re * s + im * c;
+                    re = t;
+                }
+                uu[2 * r] = re;
+                uu[2 * r + 1] = im;
+            }
+            if (R == 2) {
+                vv[0] = uu[0] + uu[2]; vv[1] = uu[1] + uu[3];
+                vv[2] = uu[0] - uu[2]; vv[3] = uu[1] - uu[3];
+            } else if (R == 4) {
+                double a0r = uu[0] + uu[4], a0i = uu[1] + uu[5];
+                double a1r = uu[0] - uu[4], a1i = uu[1] - uu[5];
+                double a2r = uu[2] + uu[6], a2i = uu[3] + uu[7];
+                double a3r = uu[2] - uu[6], a3i = uu[3] - uu[7];
+                /* multiply a3 by sign*i:  forward (sign -1) -> -i */
+                double b3r = -sign * a3i, b3i = sign * a3r;
+                vv[0] = a0r + a2r; vv[1] = a0i + a2i;
+                vv[4] = a0r - a2r; vv[5] = a0i - a2i;
+                vv[2] = a1r + b3r; vv[3] = a1i + b3i;
+                vv[6] = a1r - b3r; vv[7] = a1i - b3i;
+            } else {
+                for (int t = 0; t < R; t++) {
+                    double sr = 0.0, si = 0.0;
+                    for (int r = 0; r < R; r++) {
+                        long e = ((long)r * t % R) * (n / R);
+                        double c = pl->w[2 * e], s = -sign * pl->w[2 * e + 1];
+                        sr += uu[2 * r] * c - uu[2 * r + 1] * s;
+                        si += uu[2 * r] * s + uu[2 * r + 1] * c;
+                    }
+                    vv[2 * t] = sr;
+                    vv[2 * t + 1] = si;
+                }
+            }
+            const int j = (i / p) * p * R + k;
+            for (int r = 0; r < R; r++) {
+                dst[2 * (j + r * p)] = vv[2 * r];
+                dst[2 * (j + r * p) + 1] = vv[2 * r + 1];
+            }
+        }
+        p *= R;
+        double *t = src; src = dst; dst = t;
+    }
+    if (src != x) memcpy(x, src, sizeof(double) * 2 * (size_t)n);
+    free(big);
+}
+
+/* exported for tests: one complex line, sign -1 (FORWARD) / +1 (BACKWARD) */
+void ppo_fft(int n, double *x, int sign)
+{
+    ppo_fft_plan pl;
+    ppo_fft_plan_init(&pl, n);
+    double *work = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    ppo_fft_exec(&pl, x, work, sign);
+    free(work);
+    ppo_fft_plan_free(&pl);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* SignalProc restatement: one complex-interleaved line L of length `size`.              */
+/* ------------------------------------------------------------------------------------ */
+
+/* src/signalProc.cpp:32-71 */
+static void sp_post_dct10(double *L, int size)
+{
+    int idx = size % 2;
+    idx = (idx == 1) ? 0 : 1;
+    for (int j = 0; j < size / 2 - idx; j++) {
+        L[2 * (j + size / 2 + 1)] = L[2 * (size / 2 - j - idx)];
+        L[2 * (j + size / 2 + 1) + 1] = -L[2 * (size / 2 - j - idx) + 1];
+    }
+    for (int j = 0; j < size; j++) {
+        double theta = (PPO_PI / 2. / size) * j;
+        L[2 * j] = 2. * (L[2 * j] * cos(theta) + L[2 * j + 1] * sin(theta));
+        L[2 * j + 1] = 0.0;
+    }
+}
+
+/* src/signalProc.cpp:76-108 */
+static void sp_post_dct01(double *L, int size)
+{
+    int upper = (size - 1) / 2 + 1;
+    for (int i = 0; i < upper; i++) L[2 * (2 * i) + 1] = L[2 * i];
+    for (int i = 0; i < size / 2; i++) L[2 * (2 * i + 1) + 1] = L[2 * (size - i - 1)];
+    for (int i = 0; i < size; i++) {
+        L[2 * i] = L[2 * i + 1];
+        L[2 * i + 1] = 0.0;
+    }
+}
+
+/* src/signalProc.cpp:112-154 */
+static void sp_pre_dct01(double *L, int size)
+{
+    double theta0 = PPO_PI / 2. / size;
+    double theta = 0.0;
+    L[0] = 0.5 * cos(theta) * L[0];
+    L[1] = 0.0;
+    for (int j = 1; j < size / 2 + 1; j++) {
+        theta = theta0 * j;
+        L[2 * j + 1] = 0.5 * (sin(theta) * L[2 * j] - cos(theta) * L[2 * (size - j)]);
+        L[2 * j] = 0.5 * (cos(theta) * L[2 * j] + sin(theta) * L[2 * (size - j)]);
+    }
+    for (int j = size / 2 + 1; j < size; j++) {
+        L[2 * j] = L[2 * (size - j)];
+        L[2 * j + 1] = -L[2 * (size - j) + 1];
+    }
+}
+
+/* src/signalProc.cpp:160-195 */
+static void sp_pre_dct10(double *L, int size)
+{
+    int upper = (size - 1) / 2 + 1;
+    int istart = upper;
+    for (int j = 0; j < upper; j++) L[2 * j + 1] = L[2 * (2 * j)];
+    int idx = size % 2;
+    for (int j = istart; j < size; j++) L[2 * j + 1] = L[2 * (size - 2 * (j - istart) - 1 - idx)];
+    for (int j = 0; j < size; j++) {
+        double tmp = L[2 * j + 1];
+        L[2 * j] = tmp;
+        L[2 * j + 1] = 0.0;
+    }
+}
+
+/* src/signalProc.cpp:242-265: y_j = (-1)^j x_{size-1-j}, imaginary part cleared */
+static void sp_swap(double *L, int size)
+{
+    for (int j = 0; j < size; j++) {
+        L[2 * j + 1] = L[2 * (size - j - 1)];
+        if (j % 2 == 1) L[2 * j + 1] = -L[2 * j + 1];
+    }
+    for (int j = 0; j < size; j++) {
+        L[2 * j] = L[2 * j + 1];
+        L[2 * j + 1] = 0.0;
+    }
+}
+
+/* Quarter-wave extension (no reference counterpart).  DCT-IV of the real part of a line,
+ *   X_k = f * sum_n x_n cos(pi (2n+1)(2k+1) / (4N)),  f = 2 forward, 1 inverse,
+ * so that DCT4(f=1) o DCT4(f=2) = N * identity, matching the reference's per-axis round
+ * trip of N (scale = Nx*Ny, src/pencilDcmp.cpp:3670-3683).  Computed from the definition as
+ *   X_k = f * Re( exp(-i pi (2k+1)/(4N)) * DFT_2N[ x_n exp(-i pi n/(2N)), zero padded ]_k ).
+ * DST-IV = (-1)^k DCT-IV(reversed input). */
+static void ext_dct4(double *L, int size, double f, const ppo_fft_plan *pl2, double *Z, double *W)
+{
+    for (int n = 0; n < 2 * size; n++) {
+        if (n < size) {
+            double a = -PPO_PI * (double)n / (2.0 * size);
+            Z[2 * n] = L[2 * n] * cos(a);
+            Z[2 * n + 1] = L[2 * n] * sin(a);
+        } else {
+            Z[2 * n] = 0.0;
+            Z[2 * n + 1] = 0.0;
+        }
+    }
+    ppo_fft_exec(pl2, Z, W, -1);
+    for (int k = 0; k < size; k++) {
+        double a = -PPO_PI * (2.0 * k + 1.0) / (4.0 * size);
+        L[2 * k] = f * (Z[2 * k] * cos(a) - Z[2 * k + 1] * sin(a));
+        L[2 * k + 1] = 0.0;
+    }
+}
+
+static void ext_reverse(double *L, int size)
+{
+    for (int j = 0; j < size / 2; j++) {
+        double t = L[2 * j];
+        L[2 * j] = L[2 * (size - 1 - j)];
+        L[2 * (size - 1 - j)] = t;
+    }
+    for (int j = 0; j < size; j++) L[2 * j + 1] = 0.0;
+}
+
+static void ext_altsign(double *L, int size)
+{
+    for (int j = 1; j < size; j += 2) L[2 * j] = -L[2 * j];
+}
+
+/* exported for tests: apply one of the line transforms to a REAL line x[size] (in/out).
+ * kind: 0 DST-II(fwd, reference DST10), 1 DCT-II(fwd, DCT10), 2 DST-III/2 (DST01),
+ * 3 DCT-III/2 (DCT01), 4 DCT-IV fwd, 5 DCT-IV inv (/2), 6 DST-IV fwd, 7 DST-IV inv. */
+void ppo_line_transform(int size, int kind, double *x)
+{
+    double *L = (double *)calloc(2 * (size_t)size, sizeof(double));
+    double *W = (double *)calloc(2 * (size_t)size, sizeof(double));
+    double *Z = (double *)calloc(4 * (size_t)size, sizeof(double));
+    double *W2 = (double *)calloc(4 * (size_t)size, sizeof(double));
+    ppo_fft_plan pl, pl2;
+    ppo_fft_plan_init(&pl, size);
+    ppo_fft_plan_init(&pl2, 2 * size);
+    for (int j = 0; j < size; j++) L[2 * j] = x[j];
+    switch (kind) {
+    case 0: sp_swap(L, size); sp_pre_dct10(L, size); ppo_fft_exec(&pl, L, W, -1); sp_post_dct10(L, size); sp_swap(L, size); break;
+    case 1: sp_pre_dct10(L, size); ppo_fft_exec(&pl, L, W, -1); sp_post_dct10(L, size); break;
+    case 2: sp_swap(L, size); sp_pre_dct01(L, size); ppo_fft_exec(&pl, L, W, +1); sp_post_dct01(L, size); sp_swap(L, size); break;
+    case 3: sp_pre_dct01(L, size); ppo_fft_exec(&pl, L, W, +1); sp_post_dct01(L, size); break;
+    case 4: ext_dct4(L, size, 2.0, &pl2, Z, W2); break;
+    case 5: ext_dct4(L, size, 1.0, &pl2, Z, W2); break;
+    case 6: ext_reverse(L, size); ext_dct4(L, size, 2.0, &pl2, Z, W2); ext_altsign(L, size); break;
+    case 7: ext_reverse(L, size); ext_dct4(L, size, 1.0, &pl2, Z, W2); ext_altsign(L, size); break;
+    default: break;
+    }
+    for (int j = 0; j < size; j++) x[j] = L[2 * j];
+    ppo_fft_plan_free(&pl);
+    ppo_fft_plan_free(&pl2);
+    free(Z);
+    free(W2);
+    free(L);
+    free(W);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* TriDiag restatement.  sub = {0,1,1}, sup = {1,1,0} (src/pencilDcmp.cpp:343-349).       */
+/* ------------------------------------------------------------------------------------ */
+static const double SUBD[3] = {0.0, 1.0, 1.0};
+static const double SUPD[3] = {1.0, 1.0, 0.0};
+
+/* src/triDiag.cpp:1629-1687 (thomasLowMemNoBCV1): row 0 decoupled (tmp[1] = 0) */
+static void td_thomas_nobc_v1(double *tmp, double *rh, const double *b, int N)
+{
+    double bet;
+    rh[0] = rh[0] / (bet = b[0]);
+    int j = 1;
+    tmp[j] = 0.0;
+    bet = b[1] - SUBD[1] * tmp[j];
+    rh[1] = (rh[1] - SUBD[1] * rh[j - 1]) / bet;
+    for (j = 2; j < N - 1; j++) {
+        tmp[j] = SUPD[1] / bet;
+        bet = b[1] - SUBD[1] * tmp[j];
+        rh[j] = (rh[j] - SUBD[1] * rh[j - 1]) / bet;
+    }
+    j = N - 1;
+    tmp[j] = SUPD[1] / bet;
+    bet = b[2] - SUBD[2] * tmp[j];
+    rh[j] = (rh[j] - SUBD[2] * rh[j - 1]) / bet;
+    for (j = N - 2; j >= 0; j--) rh[j] -= tmp[j + 1] * rh[j + 1];
+}
+
+/* src/triDiag.cpp:1828-1886 (thomasLowMemNoBC) */
+static void td_thomas_nobc(double *tmp, double *rh, const double *b, int N)
+{
+    double bet;
+    rh[0] = rh[0] / (bet = b[0]);
+    int j = 1;
+    tmp[j] = SUPD[j - 1] / bet;
+    bet = b[1] - SUBD[1] * tmp[j];
+    rh[1] = (rh[1] - SUBD[1] * rh[j - 1]) / bet;
+    for (j = 2; j < N - 1; j++) {
+        tmp[j] = SUPD[1] / bet;
+        bet = b[1] - SUBD[1] * tmp[j];
+        rh[j] = (rh[j] - SUBD[1] * rh[j - 1]) / bet;
+    }
+    j = N - 1;
+    tmp[j] = SUPD[1] / bet;
+    bet = b[2] - SUBD[2] * tmp[j];
+    rh[j] = (rh[j] - SUBD[2] * rh[j - 1]) / bet;
+    for (j = N - 2; j >= 0; j--) rh[j] -= tmp[j + 1] * rh[j + 1];
+}
+
+/* src/triDiag.cpp:1693-1822 (thomasLowMem).  zero_mode: 0 = reference rule (row 0 pinned to
+ * u0 = r0 whenever |eig+2| < 1e-10), 1 = pin only when the z operator is singular (z NN);
+ * with a Dirichlet z face the true system is solved instead. */
+static void td_thomas_lowmem(double *tmp, double *rh, double diag, char bc0, char bc1, int N, int zero_mode)
+{
+    double b[3] = {diag, diag, diag};
+    if (bc0 == 'D') b[0] = b[1] - 1.;
+    if (bc1 == 'D') b[2] = b[1] - 1.;
+    if (bc0 == 'N') b[0] = b[1] + 1.;
+    if (bc1 == 'N') b[2] = b[1] + 1.0;
+    int pin = fabs(diag + 2.) < 1.e-10;
+    if (pin && zero_mode == 1 && !(bc0 == 'N' && bc1 == 'N')) pin = 0;
+    if (pin) {
+        b[0] = 1.0;
+        td_thomas_nobc_v1(tmp, rh, b, N);
+        return;
+    }
+    double bet;
+    rh[0] = rh[0] / (bet = b[0]);
+    int j = 1;
+    tmp[j] = SUPD[j - 1] / bet;
+    bet = b[1] - SUBD[1] * tmp[j];
+    rh[1] = (rh[1] - SUBD[1] * rh[j - 1]) / bet;
+    for (j = 2; j < N - 1; j++) {
+        tmp[j] = SUPD[1] / bet;
+        bet = b[1] - SUBD[1] * tmp[j];
+        rh[j] = (rh[j] - SUBD[1] * rh[j - 1]) / bet;
+    }
+    j = N - 1;
+    tmp[j] = SUPD[1] / bet;
+    bet = b[2] - SUBD[2] * tmp[j];
+    rh[j] = (rh[j] - SUBD[2] * rh[j - 1]) / bet;
+    for (j = N - 2; j >= 0; j--) rh[j] -= tmp[j + 1] * rh[j + 1];
+}
+
+/* src/triDiag.cpp:2112-2255 (shermanMorrisonThomasV1), the singular periodic-z branch */
+static void td_smt_v1(double *tmp, double *rh, double *rh1, double diag, double alpha, double beta, int N)
+{
+    double b[3] = {1.0, diag, diag}, bb[3];
+    double gamma = -b[0];
+    bb[0] = b[0] - gamma;
+    bb[1] = b[1];
+    bb[2] = b[2] - alpha * beta / gamma;
+    rh[0] = (rh[0] + rh[N - 1]) * 0.5;
+    td_thomas_nobc_v1(tmp, rh, bb, N);
+    rh1[0] = gamma;
+    rh1[N - 1] = alpha;
+    for (int i = 1; i < N - 1; i++) rh1[i] = 0.0;
+    td_thomas_nobc_v1(tmp, rh1, bb, N);
+    double part1 = (rh[0] + beta * rh[N - 1] / gamma);
+    double part2 = (1. + rh1[0] + beta * rh1[N - 1] / gamma);
+    double fact = part1 / part2;
+    for (int i = 0; i < N; i++) rh[i] -= fact * rh1[i];
+}
+
+/* src/triDiag.cpp:1961-2108 (shermanMorrisonThomas), periodic z */
+static void td_smt(double *tmp, double *rh, double *rh1, double diag, double alpha, double beta, int N)
+{
+    double b[3] = {diag, diag, diag}, bb[3];
+    double gamma = -b[0];
+    bb[0] = b[0] - gamma;
+    bb[1] = b[1];
+    bb[2] = b[2] - alpha * beta / gamma;
+    if (fabs(diag + 2.) < 1.e-10) {
+        td_smt_v1(tmp, rh, rh1, diag, 1.0, 1.0, N);
+        return;
+    }
+    td_thomas_nobc(tmp, rh, bb, N);
+    rh1[0] = gamma;
+    rh1[N - 1] = alpha;
+    for (int i = 1; i < N - 1; i++) rh1[i] = 0.0;
+    td_thomas_nobc(tmp, rh1, bb, N);
+    double part1 = (rh[0] + beta * rh[N - 1] / gamma);
+    double part2 = (1. + rh1[0] + beta * rh1[N - 1] / gamma);
+    double fact = part1 / part2;
+    for (int i = 0; i < N; i++) rh[i] -= fact * rh1[i];
+}
+
+/* exported for tests: one z-line solve, as solveThmBatch does per (kx,ky) */
+void ppo_zsolve_line(int N, double eig, char bc0, char bc1, int zero_mode, double *rh)
+{
+    double *tmp = (double *)calloc((size_t)N + 1, sizeof(double));
+    double *rh1 = (double *)calloc((size_t)N + 1, sizeof(double));
+    if (bc0 == 'P') td_smt(tmp, rh, rh1, eig, 1.0, 1.0, N);
+    else td_thomas_lowmem(tmp, rh, eig, bc0, bc1, N, zero_mode);
+    free(tmp);
+    free(rh1);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Boundary-condition set-up: assignBoundary / detectTransforms / extractTag /           */
+/* setEigenVal / setScale (src/pencilDcmp.cpp:3533-3732), extended with kinds 3/4.       */
+/* ------------------------------------------------------------------------------------ */
+/* transform kind per axis: 0 DST (DD), 1 DCT (NN), 2 Fourier (PP), 3 DCT-IV (N at min, D at
+ * max: extension), 4 DST-IV (D at min, N at max: extension).  `dtag` is the tag the reference
+ * would hold for the Dirichlet boundary term (exactValue's `tg`), i.e. -1 for 'U' axes. */
+typedef struct {
+    int nx, ny, nz;
+    char bc[6];
+    int kind[2];
+    int dtag[3];
+    double num[2], denum[2], freqs[2];
+    double scale;
+    double Xbox[6];
+    double dxyz[3];
+    int faceTag[6];
+} ppo_setup;
+
+static int axis_kind(char lo, char hi)
+{
+    if (lo == 'D' && hi == 'D') return 0;
+    if (lo == 'N' && hi == 'N') return 1;
+    if (lo == 'P' && hi == 'P') return 2;
+    if (lo == 'N' && hi == 'D') return 3;
+    if (lo == 'D' && hi == 'N') return 4;
+    return -1;
+}
+
+static int ppo_setup_init(ppo_setup *S, int nx, int ny, int nz, const char *bc, const double *X)
+{
+    S->nx = nx; S->ny = ny; S->nz = nz;
+    for (int i = 0; i < 6; i++) S->bc[i] = bc[i];
+    for (int i = 0; i < 6; i++) S->Xbox[i] = X[i];
+    for (int a = 0; a < 2; a++) {
+        S->kind[a] = axis_kind(bc[2 * a], bc[2 * a + 1]);
+        if (S->kind[a] < 0) return 25;
+    }
+    if (!((bc[4] == 'P' && bc[5] == 'P') || ((bc[4] == 'D' || bc[4] == 'N') && (bc[5] == 'D' || bc[5] == 'N'))))
+        return 25;
+    /* detectTransforms (3685-3732) + extractTag (3636-3668): reference tags for the
+     * Dirichlet term.  transform: DD->S(0) NN->C(1) PP->F(2) else U(-1); if no axis is U,
+     * transform[2] becomes U; tags[2] then overridden by z face type. */
+    char tr[3];
+    for (int i = 0; i < 3; i++) {
+        char lo = bc[2 * i], hi = bc[2 * i + 1];
+        tr[i] = 'U';
+        if (lo == 'D' && hi == 'D') tr[i] = 'S';
+        else if (lo == 'N' && hi == 'N') tr[i] = 'C';
+        else if (lo == 'P' && hi == 'P') tr[i] = 'F';
+    }
+    int count = 0;
+    for (int i = 0; i < 3; i++) count += (tr[i] == 'U');
+    if (count == 0) tr[2] = 'U';
+    for (int i = 0; i < 3; i++) {
+        S->dtag[i] = tr[i] == 'S' ? 0 : tr[i] == 'C' ? 1 : tr[i] == 'F' ? 2 : -1;
+    }
+    if (bc[4] == 'D') S->dtag[2] = -1;
+    else if (bc[4] == 'N') S->dtag[2] = -2;
+    /* setEigenVal (3533-3576), plus num = 1/2 for the quarter-wave extension */
+    for (int a = 0; a < 2; a++) {
+        S->freqs[a] = PPO_PI;
+        S->denum[a] = 0.0;
+        switch (S->kind[a]) {
+        case 0: S->num[a] = 1.0; break;
+        case 1: S->num[a] = 0.0; break;
+        case 2: S->num[a] = 0.0; S->freqs[a] = 2. * S->freqs[a]; break;
+        default: S->num[a] = 0.5; break;
+        }
+    }
+    /* setScale (3670-3683) */
+    S->scale = (double)nx * (double)ny;
+    /* setDxyz (826-840) */
+    S->dxyz[0] = -(X[0] - X[1]) / (nx);
+    S->dxyz[1] = -(X[2] - X[3]) / (ny);
+    S->dxyz[2] = -(X[4] - X[5]) / (nz);
+    /* faceTag at one rank (3578-3612) */
+    for (int i = 0; i < 6; i++) S->faceTag[i] = (bc[i] == 'D');
+    return 0;
+}
+
+/* getEigenVal (4393-4417), i = y-mode, j = x-mode, one rank (offsets 0) */
+static double ppo_eig(const ppo_setup *S, int i, int j)
+{
+    double Cx = (S->dxyz[2] / S->dxyz[0]) * (S->dxyz[2] / S->dxyz[0]);
+    double Cy = (S->dxyz[2] / S->dxyz[1]) * (S->dxyz[2] / S->dxyz[1]);
+    return (-2.0 + (-2.0 + 2. * cos((j + S->num[0]) * S->freqs[0] / (S->nx + S->denum[0]))) * Cx
+            + (-2.0 + 2. * cos((i + S->num[1]) * S->freqs[1] / (S->ny + S->denum[1]))) * Cy);
+}
+
+/* exactValue (1837-1866) */
+static double exact_value(double omega, double x, int tg)
+{
+    double ans = 0.0;
+    if (fabs(omega) < 1.0e-6) return 1.0;
+    if (tg == 0) ans = sin(omega * x);
+    else if (tg == 1) ans = cos(omega * x);
+    else if (tg == -1) ans = sin(omega * x);
+    else if (tg == -2) ans = sin(omega * x);
+    return ans;
+}
+
+/* modifyRhsDirichlet (4623-4835) at one rank, SHIFT = 1 (params.h:51) */
+static void ppo_dirichlet(const ppo_setup *S, double *P)
+{
+    const int nx = S->nx, ny = S->ny, nz = S->nz;
+    double pi = 4. * atan(1.0);
+    double Xa = S->Xbox[0], Ya = S->Xbox[2], Za = S->Xbox[4];
+    double c1 = S->dxyz[0], c2 = S->dxyz[1], c3 = S->dxyz[2];
+    double shift = 1.0;
+    double omega[3] = {1 * pi, 2 * pi, 3 * pi}; /* COEFF0..2 = 1,2,3 (params.h:58-60) */
+    const int *tg = S->dtag;
+#define PIDX(i, j, k) (2 * ((size_t)(i) + (size_t)nx * (j) + (size_t)nx * ny * (k)))
+    double x, y, z;
+    if (S->faceTag[4]) {
+        z = Za;
+        int k = 0;
+        for (int i = 0; i < nx; i++) {
+            x = Xa + i * c1 + shift * c1 * .5;
+            for (int j = 0; j < ny; j++) {
+                y = Ya + j * c2 + shift * c2 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2. * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+    if (S->faceTag[5]) {
+        z = Za + (nz - 1) * c3 + c3;
+        int k = nz - 1;
+        for (int i = 0; i < nx; i++) {
+            x = Xa + i * c1 + shift * c1 * .5;
+            for (int j = 0; j < ny; j++) {
+                y = Ya + j * c2 + shift * c2 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2 * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+    if (S->faceTag[0]) {
+        x = Xa;
+        int i = 0;
+        for (int k = 0; k < nz; k++) {
+            z = Za + k * c3 + shift * c3 * .5;
+            for (int j = 0; j < ny; j++) {
+                y = Ya + j * c2 + shift * c2 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2. * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+    if (S->faceTag[1]) {
+        int i = nx - 1;
+        x = Xa + (i + 1) * c1;
+        for (int k = 0; k < nz; k++) {
+            z = Za + k * c3 + shift * c3 * .5;
+            for (int j = 0; j < ny; j++) {
+                y = Ya + j * c2 + shift * c2 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2. * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+    if (S->faceTag[2]) {
+        y = Ya;
+        int j = 0;
+        for (int k = 0; k < nz; k++) {
+            z = Za + k * c3 + shift * c3 * .5;
+            for (int i = 0; i < nx; i++) {
+                x = Xa + i * c1 + shift * c1 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2. * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+    if (S->faceTag[3]) {
+        int j = ny - 1;
+        y = Ya + (j + 1) * c2;
+        for (int k = 0; k < nz; k++) {
+            z = Za + k * c3 + shift * c3 * .5;
+            for (int i = 0; i < nx; i++) {
+                x = Xa + i * c1 + shift * c1 * 0.5;
+                P[PIDX(i, j, k)] = P[PIDX(i, j, k)] - 2. * exact_value(omega[0], x, tg[0]) * exact_value(omega[1], y, tg[1]) * exact_value(omega[2], z, tg[2]);
+            }
+        }
+    }
+#undef PIDX
+}
+
+/* Transform every contiguous line of P (nlines lines of length size) forward or inverse,
+ * following pre/postprocessSignalAccordingly[Reverse] (4464-4618) around the FFT hooks
+ * (poissonCPU.cpp:40-234).  inverse = 0: pre, FFT FORWARD, post; inverse = 1: pre-rev,
+ * FFT BACKWARD, post-rev.  Quarter-wave kinds (3,4) apply ext_dct4 both ways. */
+static void ppo_lines(double *P, long nlines, int size, int kind, int inverse)
+{
+    ppo_fft_plan pl, pl2;
+    ppo_fft_plan_init(&pl, size);
+    ppo_fft_plan_init(&pl2, 2 * size);
+    double *W = (double *)malloc(sizeof(double) * 4 * (size_t)size);
+    double *Z = (double *)malloc(sizeof(double) * 4 * (size_t)size);
+    for (long l = 0; l < nlines; l++) {
+        double *L = P + 2 * (size_t)size * l;
+        if (kind == 3 || kind == 4) {
+            double f = inverse ? 1.0 : 2.0;
+            if (kind == 4) ext_reverse(L, size);
+            ext_dct4(L, size, f, &pl2, Z, W);
+            if (kind == 4) ext_altsign(L, size);
+            continue;
+        }
+        if (!inverse) {
+            if (kind == 0) { sp_swap(L, size); sp_pre_dct10(L, size); }
+            else if (kind == 1) sp_pre_dct10(L, size);
+            ppo_fft_exec(&pl, L, W, -1);
+            if (kind == 0) { sp_post_dct10(L, size); sp_swap(L, size); }
+            else if (kind == 1) sp_post_dct10(L, size);
+        } else {
+            if (kind == 0) { sp_swap(L, size); sp_pre_dct01(L, size); }
+            else if (kind == 1) sp_pre_dct01(L, size);
+            ppo_fft_exec(&pl, L, W, +1);
+            if (kind == 0) { sp_post_dct01(L, size); sp_swap(L, size); }
+            else if (kind == 1) sp_post_dct01(L, size);
+        }
+    }
+    free(W);
+    free(Z);
+    ppo_fft_plan_free(&pl);
+    ppo_fft_plan_free(&pl2);
+}
+
+/* rearrangeX2Y (2279-2380): per plane [k][j][i] -> [k][i][j] (complex), via scratch */
+static void ppo_x2y(double *P, double *R, int nx, int ny, int nz, int inverse)
+{
+    for (int k = 0; k < nz; k++) {
+        double *pl = P + 2 * (size_t)nx * ny * k;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t s = inverse ? (size_t)j + (size_t)ny * i : (size_t)i + (size_t)nx * j;
+                size_t d = inverse ? (size_t)i + (size_t)nx * j : (size_t)j + (size_t)ny * i;
+                R[2 * d] = pl[2 * s];
+                R[2 * d + 1] = pl[2 * s + 1];
+            }
+        memcpy(pl, R, sizeof(double) * 2 * (size_t)nx * ny);
+    }
+}
+
+/*
+ * Whole solve, one rank.  Inputs: sizes, bc[6] ('P','N','D' at xmin,xmax,ymin,ymax,zmin,zmax),
+ * box X[6], rhs (real, x fastest, unscaled, as assignRhs takes it).  Output u (real part of P
+ * after pittPack).  zero_mode as in td_thomas_lowmem.  Returns 0 or a PittPack-style code
+ * (25 = unsupported boundary combination).
+ */
+int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const double *rhs, double *u, int zero_mode)
+{
+    ppo_setup S;
+    int rc = ppo_setup_init(&S, nx, ny, nz, bc, X);
+    if (rc) return rc;
+    /* the reference's DCT/DST pre/post steps drop the imaginary part of a line: a periodic
+     * x followed by a non-periodic y is not representable in its algorithm */
+    if (S.kind[0] == 2 && S.kind[1] != 2) return 25;
+    const size_t N = (size_t)nx * ny * nz;
+    double *P = (double *)calloc(2 * N, sizeof(double));
+    double *R = (double *)malloc(sizeof(double) * 2 * (size_t)nx * ny);
+    if (!P || !R) { free(P); free(R); return 15; }
+    /* assignRhs (5591-5647): P = rhs * c3 * c3, Im = 0 */
+    double c3 = S.dxyz[2];
+    for (size_t q = 0; q < N; q++) { P[2 * q] = rhs[q] * c3 * c3; P[2 * q + 1] = 0.0; }
+    /* modifyRhsDirichlet (5890) */
+    ppo_dirichlet(&S, P);
+    /* forward x (5945-5964): x-lines contiguous, nx each, ny*nz of them */
+    ppo_lines(P, (long)ny * nz, nx, S.kind[0], 0);
+    /* x -> y major (6013) */
+    ppo_x2y(P, R, nx, ny, nz, 0);
+    /* forward y (6034-6048) */
+    ppo_lines(P, (long)nx * nz, ny, S.kind[1], 0);
+    /* z solve (6105-6115): per (i = y-mode, j = x-mode), y fastest layout */
+    {
+        double *line = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *tmp = (double *)calloc((size_t)nz + 1, sizeof(double));
+        double *rh1 = (double *)calloc((size_t)nz + 1, sizeof(double));
+        int nidx = (bc[0] == 'P' || bc[2] == 'P') ? 2 : 1;
+        for (int index = 0; index < nidx; index++) {
+            for (int j = 0; j < nx; j++)
+                for (int i = 0; i < ny; i++) {
+                    double eig = ppo_eig(&S, i, j);
+                    for (int k = 0; k < nz; k++)
+                        line[k] = P[2 * ((size_t)i + (size_t)ny * j + (size_t)nx * ny * k) + index];
+                    if (bc[4] != 'P') td_thomas_lowmem(tmp, line, eig, bc[4], bc[5], nz, zero_mode);
+                    else td_smt(tmp, line, rh1, eig, 1.0, 1.0, nz);
+                    for (int k = 0; k < nz; k++)
+                        P[2 * ((size_t)i + (size_t)ny * j + (size_t)nx * ny * k) + index] = line[k];
+                }
+        }
+        free(line);
+        free(tmp);
+        free(rh1);
+    }
+    /* inverse y (6208-6215) */
+    ppo_lines(P, (long)nx * nz, ny, S.kind[1], 1);
+    /* y -> x major (6229) */
+    ppo_x2y(P, R, nx, ny, nz, 1);
+    /* inverse x (6287-6294) */
+    ppo_lines(P, (long)ny * nz, nx, S.kind[0], 1);
+    /* rescale (3503-3531) */
+    for (size_t q = 0; q < N; q++) {
+        P[2 * q] = 1. / (S.scale) * P[2 * q];
+        P[2 * q + 1] = 1. / (S.scale) * P[2 * q + 1];
+    }
+    for (size_t q = 0; q < N; q++) u[q] = P[2 * q];
+    free(P);
+    free(R);
+    return 0;
+}
+
+/* exported for tests: the effective right-hand side the solve inverts,
+ * f_eff = rhs * dz^2 - (Dirichlet boundary term), real, x fastest */
+int ppo_effective_rhs(int nx, int ny, int nz, const char *bc, const double *X, const double *rhs, double *out)
+{
+    ppo_setup S;
+    int rc = ppo_setup_init(&S, nx, ny, nz, bc, X);
+    if (rc) return rc;
+    const size_t N = (size_t)nx * ny * nz;
+    double *P = (double *)calloc(2 * N, sizeof(double));
+    double c3 = S.dxyz[2];
+    for (size_t q = 0; q < N; q++) P[2 * q] = rhs[q] * c3 * c3;
+    ppo_dirichlet(&S, P);
+    for (size_t q = 0; q < N; q++) out[q] = P[2 * q];
+    free(P);
+    return 0;
+}
+
+/* exported for tests: the eigenvalue the z solve uses for column (x-mode j, y-mode i) */
+double ppo_eigenvalue(int nx, int ny, int nz, const char *bc, const double *X, int i, int j)
+{
+    ppo_setup S;
+    if (ppo_setup_init(&S, nx, ny, nz, bc, X)) return NAN;
+    return ppo_eig(&S, i, j);
+}
+
+/* exported for tests: the 7-point operator the solve inverts, scaled by dz^2 like the
+ * z tridiagonal (interior: (u[i+1]-2u[i]+u[i-1])*Cx + (..y..)*Cy + (..z..)).  Ghost cells:
+ * P wraps, N mirrors (u_g = u_b), D antisymmetric (u_g = -u_b). */
+int ppo_apply_operator(int nx, int ny, int nz, const char *bc, const double *X, const double *u, double *out)
+{
+    ppo_setup S;
+    int rc = ppo_setup_init(&S, nx, ny, nz, bc, X);
+    if (rc) return rc;
+    double Cx = (S.dxyz[2] / S.dxyz[0]) * (S.dxyz[2] / S.dxyz[0]);
+    double Cy = (S.dxyz[2] / S.dxyz[1]) * (S.dxyz[2] / S.dxyz[1]);
+    int n[3] = {nx, ny, nz};
+    size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+    double C[3] = {Cx, Cy, 1.0};
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                int id[3] = {i, j, k};
+                size_t q = i + st[1] * j + st[2] * k;
+                double acc = 0.0;
+                for (int a = 0; a < 3; a++) {
+                    double c = u[q];
+                    double lo, hi;
+                    if (id[a] > 0) lo = u[q - st[a]];
+                    else if (bc[2 * a] == 'P') lo = u[q + st[a] * (n[a] - 1)];
+                    else if (bc[2 * a] == 'N') lo = c;
+                    else lo = -c;
+                    if (id[a] < n[a] - 1) hi = u[q + st[a]];
+                    else if (bc[2 * a + 1] == 'P') hi = u[q - st[a] * (n[a] - 1)];
+                    else if (bc[2 * a + 1] == 'N') hi = c;
+                    else hi = -c;
+                    acc += (hi - 2.0 * c + lo) * C[a];
+                }
+                out[q] = acc;
+            }
+    return 0;
+}

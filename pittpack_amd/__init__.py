@@ -1,0 +1,206 @@
+"""pittpack_amd -- MI355X (gfx950) implementation of PittPack's solve path.
+
+The compute lives in ``pittpack_amd/lib/libpittpack_hip.so`` (hand-written HIP kernels behind
+the C ABI declared in ``include/pittpack_hip.h``).  This module is a thin ctypes binding plus a
+Python mirror of the reference's ``PencilDcmp`` set-up/solve interface
+(src/include/pencilDcmp.hpp:441-466), used by the tests, ``bench.py`` and Python callers.
+
+There is no CPU fallback: if the shared library is missing or no GPU is visible, every solve
+raises ``PittPackError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpittpack_hip.so")
+
+SUCCESS = 0
+ALLOCATION_FAIL = 15
+INPUT_ARGS = 19
+HIP_ERROR = 24
+UNSUPPORTED_BC = 25
+UNSUPPORTED_SIZE = 26
+INVALID_HANDLE = 27
+CALL_ORDER = 28
+NO_DEVICE = 29
+
+OPT_ZERO_MODE = 1
+OPT_TIMING = 2
+
+# every symbol include/pittpack_hip.h declares
+EXPORTED = (
+    "pp_create", "pp_destroy", "pp_assign_boundary", "pp_set_box", "pp_set_coords",
+    "pp_assign_rhs", "pp_assign_rhs_device", "pp_pittpack", "pp_get_solution",
+    "pp_get_solution_device", "pp_solve_device", "pp_set_stream", "pp_set_option",
+    "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
+    "pp_selftest",
+)
+
+
+class PittPackError(RuntimeError):
+    def __init__(self, code: int, where: str = ""):
+        self.code = code
+        name = error_name(code) if _lib is not None else str(code)
+        super().__init__(f"{where}: PittPack error {code} ({name})")
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load the HIP library (no GPU needed to load it)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PittPackError(HIP_ERROR, f"HIP library not built ({p}); run __graft_entry__.build()")
+    L = ctypes.CDLL(p)
+    vp, i, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.pp_create.argtypes = [i, i, i, ctypes.POINTER(vp)]
+    L.pp_destroy.argtypes = [vp]
+    L.pp_assign_boundary.argtypes = [vp, ctypes.c_char_p]
+    L.pp_set_box.argtypes = [vp, dp]
+    L.pp_set_coords.argtypes = [vp, i]
+    L.pp_assign_rhs.argtypes = [vp, dp]
+    L.pp_assign_rhs_device.argtypes = [vp, vp]
+    L.pp_pittpack.argtypes = [vp]
+    L.pp_get_solution.argtypes = [vp, dp]
+    L.pp_get_solution_device.argtypes = [vp, vp]
+    L.pp_solve_device.argtypes = [vp, vp]
+    L.pp_set_stream.argtypes = [vp, vp]
+    L.pp_set_option.argtypes = [vp, i, ctypes.c_longlong]
+    L.pp_get_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.pp_get_device_field.argtypes = [vp, ctypes.POINTER(vp)]
+    L.pp_get_error_enum.argtypes = [i]
+    L.pp_get_error_enum.restype = ctypes.c_char_p
+    L.pp_size_supported.argtypes = [i]
+    L.pp_selftest.argtypes = []
+    for name in EXPORTED:
+        if name != "pp_get_error_enum":
+            getattr(L, name).restype = ctypes.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def error_name(code: int) -> str:
+    return load_library().pp_get_error_enum(int(code)).decode()
+
+
+def _check(rc: int, where: str):
+    if rc != SUCCESS:
+        raise PittPackError(rc, where)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def size_supported(n: int) -> bool:
+    return bool(load_library().pp_size_supported(int(n)))
+
+
+def selftest() -> int:
+    return load_library().pp_selftest()
+
+
+class PoissonHIP:
+    """Python mirror of PencilDcmp's public solve interface on one MI355X.
+
+    Same call order as src/main.cpp:73-118::
+
+        M = PoissonHIP(nx, ny, nz)
+        M.assignBoundary("NNNNDD"); M.setBox([0, 1, 0, 1, 0, 1]); M.setCoords(2)
+        M.assignRhs(rhs); M.pittPack(); u = M.getSolution()
+
+    ``rhs`` and the solution are real arrays of shape (nz, ny, nx) (x fastest), like the
+    reference's ``assignRhs(double*)`` input.
+    """
+
+    def __init__(self, nx: int, ny: int, nz: int):
+        self._L = load_library()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        h = ctypes.c_void_p()
+        _check(self._L.pp_create(self.nx, self.ny, self.nz, ctypes.byref(h)), "pp_create")
+        self._h = h
+
+    # -- reference-named set-up -------------------------------------------------------
+    def assignBoundary(self, bc) -> None:
+        s = "".join(bc) if not isinstance(bc, str) else bc
+        if len(s) != 6:
+            raise PittPackError(INPUT_ARGS, "assignBoundary")
+        _check(self._L.pp_assign_boundary(self._h, s.encode()), "assignBoundary")
+
+    def setBox(self, X) -> None:
+        x = np.ascontiguousarray(X, dtype=np.float64)
+        _check(self._L.pp_set_box(self._h, _dp(x)), "setBox")
+
+    def setCoords(self, dir: int = 2) -> None:
+        _check(self._L.pp_set_coords(self._h, int(dir)), "setCoords")
+
+    def assignRhs(self, rhs) -> None:
+        f = np.ascontiguousarray(rhs, dtype=np.float64)
+        if f.size != self.nx * self.ny * self.nz:
+            raise PittPackError(INPUT_ARGS, "assignRhs")
+        _check(self._L.pp_assign_rhs(self._h, _dp(f)), "assignRhs")
+
+    def pittPack(self) -> None:
+        _check(self._L.pp_pittpack(self._h), "pittPack")
+
+    def getSolution(self) -> np.ndarray:
+        u = np.empty((self.nz, self.ny, self.nx), dtype=np.float64)
+        _check(self._L.pp_get_solution(self._h, _dp(u)), "getSolution")
+        return u
+
+    # -- device / flow-solver forms ------------------------------------------------------
+    def set_option(self, key: int, value: int) -> None:
+        _check(self._L.pp_set_option(self._h, int(key), int(value)), "set_option")
+
+    def set_stream(self, stream_ptr: int) -> None:
+        _check(self._L.pp_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "set_stream")
+
+    def solve_device(self, d_ptr: int) -> None:
+        """Solve in place on a device array of nx*ny*nz doubles (e.g. tensor.data_ptr())."""
+        _check(self._L.pp_solve_device(self._h, ctypes.c_void_p(d_ptr)), "solve_device")
+
+    def stage_times(self):
+        ms = (ctypes.c_float * 5)()
+        _check(self._L.pp_get_stage_times(self._h, ms), "stage_times")
+        return list(ms)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value is not None:
+            self._L.pp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def solve(rhs, bc: str, box=None, zero_mode: int | None = None) -> np.ndarray:
+    """One-shot FP64 solve of an (nz, ny, nx) RHS."""
+    f = np.ascontiguousarray(rhs, dtype=np.float64)
+    nz, ny, nx = f.shape
+    with PoissonHIP(nx, ny, nz) as M:
+        if zero_mode is not None:
+            M.set_option(OPT_ZERO_MODE, zero_mode)
+        M.assignBoundary(bc)
+        M.setBox(box if box is not None else [0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.assignRhs(f)
+        M.pittPack()
+        return M.getSolution()

@@ -1,0 +1,660 @@
+// pp_solver.hip -- host side of the MI355X PittPack solve path: handle state, boundary
+// condition set-up, device tables, kernel dispatch and the C ABI of include/pittpack_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pittpack_hip.h"
+#include "pp_kernels.h"
+
+using namespace pp;
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932384;  // src/include/params.h:116
+
+// ---------------------------------------------------------------------------------------
+// compiled transform sizes: N = 2M, (M, T) with E = M/T elements per thread
+// ---------------------------------------------------------------------------------------
+constexpr int threads_per_line(int M)
+{
+    switch (M) {
+    case 8: return 1;
+    case 12: return 1;
+    case 16: return 1;
+    case 24: return 2;
+    case 32: return 2;
+    case 48: return 4;
+    case 64: return 4;
+    case 96: return 4;
+    case 128: return 8;
+    case 192: return 8;
+    case 256: return 16;
+    case 384: return 16;
+    case 512: return 32;
+    default: return 0;
+    }
+}
+
+template <typename R, int M, int AX>
+constexpr int lines_per_block()
+{
+    constexpr int T = threads_per_line(M);
+    constexpr int bytes_per_line = (M + 1) * 2 * (int)sizeof(R);
+    int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
+    if (lines > 64) lines = 64;
+    while (lines > 1 && lines * bytes_per_line > 64 * 1024) lines /= 2;
+    while (lines > 1 && lines * T > 1024) lines /= 2;
+    return lines;
+}
+
+template <typename R, int M, int AX, int FWD>
+hipError_t launch_line_t(const LineArgs &a, long nlines, int nplanes, hipStream_t s)
+{
+    constexpr int T = threads_per_line(M);
+    constexpr int LINES = lines_per_block<R, M, AX>();
+    const size_t lds = (size_t)LINES * (M + 1) * sizeof(cpx<R>);
+    dim3 block(T * LINES);
+    dim3 grid;
+    if (AX == 0) grid = dim3((unsigned)((nlines + LINES - 1) / LINES));
+    else grid = dim3((unsigned)((nlines + LINES - 1) / LINES), (unsigned)nplanes);
+    hipLaunchKernelGGL((k_line<double, double, R, M, T, LINES, AX, FWD>), grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename R, int AX, int FWD>
+hipError_t launch_line(int N, const LineArgs &a, long nlines, int nplanes, hipStream_t s)
+{
+    switch (N / 2) {
+    case 8: return launch_line_t<R, 8, AX, FWD>(a, nlines, nplanes, s);
+    case 12: return launch_line_t<R, 12, AX, FWD>(a, nlines, nplanes, s);
+    case 16: return launch_line_t<R, 16, AX, FWD>(a, nlines, nplanes, s);
+    case 24: return launch_line_t<R, 24, AX, FWD>(a, nlines, nplanes, s);
+    case 32: return launch_line_t<R, 32, AX, FWD>(a, nlines, nplanes, s);
+    case 48: return launch_line_t<R, 48, AX, FWD>(a, nlines, nplanes, s);
+    case 64: return launch_line_t<R, 64, AX, FWD>(a, nlines, nplanes, s);
+    case 96: return launch_line_t<R, 96, AX, FWD>(a, nlines, nplanes, s);
+    case 128: return launch_line_t<R, 128, AX, FWD>(a, nlines, nplanes, s);
+    case 192: return launch_line_t<R, 192, AX, FWD>(a, nlines, nplanes, s);
+    case 256: return launch_line_t<R, 256, AX, FWD>(a, nlines, nplanes, s);
+    case 384: return launch_line_t<R, 384, AX, FWD>(a, nlines, nplanes, s);
+    case 512: return launch_line_t<R, 512, AX, FWD>(a, nlines, nplanes, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool size_supported(int n) { return n % 2 == 0 && threads_per_line(n / 2) > 0; }
+
+// z solve: J = rows per lane, rounded up to a compiled value
+constexpr int kJ[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64};
+
+int pick_J(int nz)
+{
+    int need = (nz + 15) / 16;
+    for (int j : kJ)
+        if (j >= need) return j;
+    return 0;
+}
+
+template <typename R, int J>
+hipError_t launch_z_t(const ZArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.SX + 15) / 16), (unsigned)a.SY);
+    hipLaunchKernelGGL((k_zsolve<R, J>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_z(int J, const ZArgs &a, hipStream_t s)
+{
+    switch (J) {
+#define PPJ(v) \
+    case v: return launch_z_t<R, v>(a, s);
+        PPJ(1) PPJ(2) PPJ(3) PPJ(4) PPJ(5) PPJ(6) PPJ(7) PPJ(8) PPJ(10) PPJ(12) PPJ(14) PPJ(16)
+        PPJ(20) PPJ(24) PPJ(28) PPJ(32) PPJ(40) PPJ(48) PPJ(56) PPJ(64)
+#undef PPJ
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// exactValue (src/pencilDcmp.cpp:1837-1866)
+double exact_value(double omega, double x, int tg)
+{
+    double ans = 0.0;
+    if (std::fabs(omega) < 1.0e-6) return 1.0;
+    if (tg == 0) ans = std::sin(omega * x);
+    else if (tg == 1) ans = std::cos(omega * x);
+    else if (tg == -1) ans = std::sin(omega * x);
+    else if (tg == -2) ans = std::sin(omega * x);
+    return ans;
+}
+
+int axis_kind(char lo, char hi)
+{
+    if (lo == 'D' && hi == 'D') return KIND_DD;
+    if (lo == 'N' && hi == 'N') return KIND_NN;
+    if (lo == 'P' && hi == 'P') return KIND_PP;
+    if (lo == 'N' && hi == 'D') return KIND_ND;
+    if (lo == 'D' && hi == 'N') return KIND_DN;
+    return -1;
+}
+
+// per-axis twiddle tables in FP64 (host) for a real line of length N = 2M
+struct AxisTables {
+    std::vector<double> tw, A, B, C, D;  // interleaved complex
+};
+
+void make_axis_tables(int N, AxisTables &t)
+{
+    const int M = N / 2;
+    auto put = [](std::vector<double> &v, int cnt, auto f) {
+        v.resize(2 * (size_t)cnt);
+        for (int k = 0; k < cnt; k++) {
+            double ang = f(k);
+            v[2 * k] = std::cos(ang);
+            v[2 * k + 1] = std::sin(ang);
+        }
+    };
+    put(t.tw, M, [&](int e) { return -2.0 * kPi * (double)e / (double)M; });
+    put(t.A, M + 1, [&](int k) { return -2.0 * kPi * (double)k / (double)N; });
+    put(t.B, M + 1, [&](int k) { return -kPi * (double)k / (2.0 * N); });
+    put(t.C, M, [&](int n) { return -kPi * (4.0 * n + 1.0) / (4.0 * N); });
+    put(t.D, M, [&](int k) { return -kPi * (double)k / (double)N; });
+}
+
+struct DevTables {
+    void *tw = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr;
+};
+
+}  // namespace
+
+struct pp_solver {
+    int nx = 0, ny = 0, nz = 0;
+    char bc[6] = {0, 0, 0, 0, 0, 0};
+    bool bc_set = false, box_set = false, coords_set = false, rhs_set = false;
+    double Xbox[6] = {0, 1, 0, 1, 0, 1};
+    int kind[2] = {-1, -1};
+    int zlo_d = 0, zhi_d = 0;
+    int dtag[3] = {0, 0, 0};
+    double num[2] = {0, 0}, denum[2] = {0, 0}, freqs[2] = {kPi, kPi};
+    double scale = 1.0;
+    double dxyz[3] = {0, 0, 0};
+    int zero_mode = 1;
+    bool timing = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double *d_field = nullptr;
+    double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
+    DevTables tab64[2];
+    int dmask = 0;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool tables_ready = false;
+};
+
+namespace {
+
+#define PP_HIP_CHECK(call)                       \
+    do {                                         \
+        hipError_t e_ = (call);                  \
+        if (e_ != hipSuccess) return PP_HIP_ERROR; \
+    } while (0)
+
+void free_tables(pp_solver *h)
+{
+    for (int a = 0; a < 2; a++) {
+        DevTables &t = h->tab64[a];
+        for (void **p : {&t.tw, &t.A, &t.B, &t.C, &t.D}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+    }
+    for (double **p : {&h->d_lamx, &h->d_lamy, &h->d_dtab}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    h->tables_ready = false;
+}
+
+int upload(void **dst, const std::vector<double> &v)
+{
+    if (hipMalloc(dst, v.size() * sizeof(double)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    if (hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return PP_HIP_ERROR;
+    return PP_SUCCESS;
+}
+
+// Build everything that depends on (sizes, bc, box): eigenvalue factors, twiddles and the
+// Dirichlet boundary term tables.  Mirrors assignBoundary/setEigenVal/setScale/setCoords.
+int build_tables(pp_solver *h)
+{
+    free_tables(h);
+    const int n[2] = {h->nx, h->ny};
+    int rc;
+    for (int a = 0; a < 2; a++) {
+        AxisTables t;
+        make_axis_tables(n[a], t);
+        DevTables &d = h->tab64[a];
+        if ((rc = upload(&d.tw, t.tw))) return rc;
+        if ((rc = upload(&d.A, t.A))) return rc;
+        if ((rc = upload(&d.B, t.B))) return rc;
+        if ((rc = upload(&d.C, t.C))) return rc;
+        if ((rc = upload(&d.D, t.D))) return rc;
+    }
+    // eigenvalue factors -2 + 2 cos((j + num) * freq / (n + denum)) (src/pencilDcmp.cpp:4410-4413)
+    std::vector<double> lx(h->nx), ly(h->ny);
+    for (int j = 0; j < h->nx; j++) lx[j] = (-2.0 + 2. * std::cos((j + h->num[0]) * h->freqs[0] / (h->nx + h->denum[0])));
+    for (int i = 0; i < h->ny; i++) ly[i] = (-2.0 + 2. * std::cos((i + h->num[1]) * h->freqs[1] / (h->ny + h->denum[1])));
+    if ((rc = upload((void **)&h->d_lamx, lx))) return rc;
+    if ((rc = upload((void **)&h->d_lamy, ly))) return rc;
+    // Dirichlet term tables (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835, SHIFT = 1)
+    const double pi = 4. * std::atan(1.0);
+    const double omega[3] = {1 * pi, 2 * pi, 3 * pi};
+    const double Xa = h->Xbox[0], Ya = h->Xbox[2], Za = h->Xbox[4];
+    const double c1 = h->dxyz[0], c2 = h->dxyz[1], c3 = h->dxyz[2];
+    const double shift = 1.0;
+    std::vector<double> dt((size_t)h->nx + h->ny + h->nz + 6);
+    size_t o = 0;
+    for (int i = 0; i < h->nx; i++) dt[o++] = exact_value(omega[0], Xa + i * c1 + shift * c1 * .5, h->dtag[0]);
+    for (int j = 0; j < h->ny; j++) dt[o++] = exact_value(omega[1], Ya + j * c2 + shift * c2 * 0.5, h->dtag[1]);
+    for (int k = 0; k < h->nz; k++) dt[o++] = exact_value(omega[2], Za + k * c3 + shift * c3 * .5, h->dtag[2]);
+    dt[o++] = exact_value(omega[0], Xa, h->dtag[0]);
+    dt[o++] = exact_value(omega[0], Xa + ((h->nx - 1) + 1) * c1, h->dtag[0]);
+    dt[o++] = exact_value(omega[1], Ya, h->dtag[1]);
+    dt[o++] = exact_value(omega[1], Ya + ((h->ny - 1) + 1) * c2, h->dtag[1]);
+    dt[o++] = exact_value(omega[2], Za, h->dtag[2]);
+    dt[o++] = exact_value(omega[2], Za + (h->nz - 1) * c3 + c3, h->dtag[2]);
+    if ((rc = upload((void **)&h->d_dtab, dt))) return rc;
+    h->dmask = 0;
+    for (int f = 0; f < 6; f++)
+        if (h->bc[f] == 'D') h->dmask |= (1 << f);
+    h->tables_ready = true;
+    return PP_SUCCESS;
+}
+
+int ensure_ready(pp_solver *h)
+{
+    if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
+    if (!h->tables_ready) return build_tables(h);
+    return PP_SUCCESS;
+}
+
+LineArgs line_args(pp_solver *h, int axis, double *field)
+{
+    LineArgs a{};
+    a.in = field;
+    a.out = field;
+    a.kind = h->kind[axis];
+    const DevTables &t = h->tab64[axis];
+    a.tw = t.tw;
+    a.A = t.A;
+    a.B = t.B;
+    a.C = t.C;
+    a.D = t.D;
+    a.in_scale = 1.0;
+    a.out_scale = 1.0;
+    a.dtab = h->d_dtab;
+    a.dmask = 0;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.nz = h->nz;
+    if (axis == 0) {
+        a.nlines = (long)h->ny * h->nz;
+    } else {
+        a.nlines = h->nx;
+        a.ncols = h->nx;
+        a.nplanes = h->nz;
+    }
+    return a;
+}
+
+// the five stages on `field` (in place), FP64
+int run_fp64(pp_solver *h, double *field)
+{
+    hipStream_t s = h->stream;
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[0], s));
+    // 1. forward x: rhs * dz^2, Dirichlet boundary term, x transform
+    {
+        LineArgs a = line_args(h, 0, field);
+        a.in_scale = h->dxyz[2];
+        a.dmask = h->dmask;
+        PP_HIP_CHECK((launch_line<double, 0, 1>(h->nx, a, a.nlines, 1, s)));
+    }
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[1], s));
+    // 2. forward y
+    {
+        LineArgs a = line_args(h, 1, field);
+        PP_HIP_CHECK((launch_line<double, 1, 1>(h->ny, a, a.nlines, h->nz, s)));
+    }
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[2], s));
+    // 3. z solve
+    {
+        ZArgs z{};
+        z.data = field;
+        z.SX = h->nx;
+        z.SY = h->ny;
+        z.nz = h->nz;
+        z.lamx = h->d_lamx;
+        z.lamy = h->d_lamy;
+        z.Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
+        z.Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+        z.zlo_dirichlet = h->zlo_d;
+        z.zhi_dirichlet = h->zhi_d;
+        z.zero_mode = h->zero_mode;
+        PP_HIP_CHECK(launch_z<double>(pick_J(h->nz), z, s));
+    }
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[3], s));
+    // 4. inverse y
+    {
+        LineArgs a = line_args(h, 1, field);
+        PP_HIP_CHECK((launch_line<double, 1, 0>(h->ny, a, a.nlines, h->nz, s)));
+    }
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[4], s));
+    // 5. inverse x with the 1/(Nx*Ny) rescale fused
+    {
+        LineArgs a = line_args(h, 0, field);
+        a.out_scale = 1. / (h->scale);
+        PP_HIP_CHECK((launch_line<double, 0, 0>(h->nx, a, a.nlines, 1, s)));
+    }
+    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[5], s));
+    return PP_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_create(int nx, int ny, int nz, pp_handle *out)
+{
+    if (!out) return PP_INPUT_ARGS;
+    *out = nullptr;
+    if (nx < 2 || ny < 2 || nz < 4) return PP_INPUT_ARGS;
+    if (!size_supported(nx) || !size_supported(ny)) return PP_UNSUPPORTED_SIZE;
+    if (pick_J(nz) == 0) return PP_UNSUPPORTED_SIZE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PP_NO_DEVICE;
+    pp_solver *h = new (std::nothrow) pp_solver();
+    if (!h) return PP_ALLOCATION_FAIL;
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    const size_t bytes = (size_t)nx * ny * nz * sizeof(double);
+    if (hipMalloc((void **)&h->d_field, bytes) != hipSuccess) {
+        delete h;
+        return PP_ALLOCATION_FAIL;
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(h->d_field);
+        delete h;
+        return PP_HIP_ERROR;
+    }
+    h->own_stream = true;
+    for (int i = 0; i < 6; i++) {
+        if (hipEventCreate(&h->ev[i]) != hipSuccess) {
+            pp_destroy(h);
+            return PP_HIP_ERROR;
+        }
+    }
+    *out = h;
+    return PP_SUCCESS;
+}
+
+int pp_destroy(pp_handle h)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_tables(h);
+    if (h->d_field) (void)hipFree(h->d_field);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PP_SUCCESS;
+}
+
+int pp_assign_boundary(pp_handle h, const char *bc)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!bc) return PP_INPUT_ARGS;
+    for (int i = 0; i < 6; i++)
+        if (bc[i] != 'P' && bc[i] != 'N' && bc[i] != 'D') return PP_UNSUPPORTED_BC;
+    int kx = axis_kind(bc[0], bc[1]), ky = axis_kind(bc[2], bc[3]);
+    if (kx < 0 || ky < 0) return PP_UNSUPPORTED_BC;
+    if (bc[4] == 'P' || bc[5] == 'P') return PP_UNSUPPORTED_BC;  // periodic z: not built yet
+    for (int i = 0; i < 6; i++) h->bc[i] = bc[i];
+    h->kind[0] = kx;
+    h->kind[1] = ky;
+    h->zlo_d = bc[4] == 'D';
+    h->zhi_d = bc[5] == 'D';
+    // detectTransforms + extractTag (src/pencilDcmp.cpp:3636-3732): tags used by the
+    // Dirichlet boundary term (exactValue's tg)
+    char tr[3];
+    for (int i = 0; i < 3; i++) {
+        char lo = bc[2 * i], hi = bc[2 * i + 1];
+        tr[i] = 'U';
+        if (lo == 'D' && hi == 'D') tr[i] = 'S';
+        else if (lo == 'N' && hi == 'N') tr[i] = 'C';
+        else if (lo == 'P' && hi == 'P') tr[i] = 'F';
+    }
+    int count = 0;
+    for (int i = 0; i < 3; i++) count += (tr[i] == 'U');
+    if (count == 0) tr[2] = 'U';
+    for (int i = 0; i < 3; i++) h->dtag[i] = tr[i] == 'S' ? 0 : tr[i] == 'C' ? 1 : tr[i] == 'F' ? 2 : -1;
+    if (bc[4] == 'D') h->dtag[2] = -1;
+    else if (bc[4] == 'N') h->dtag[2] = -2;
+    // setEigenVal (3533-3576) with num = 1/2 for the quarter-wave axes
+    int k2[2] = {kx, ky};
+    for (int a = 0; a < 2; a++) {
+        h->freqs[a] = kPi;
+        h->denum[a] = 0.0;
+        switch (k2[a]) {
+        case KIND_DD: h->num[a] = 1.0; break;
+        case KIND_NN: h->num[a] = 0.0; break;
+        case KIND_PP: h->num[a] = 0.0; h->freqs[a] = 2. * h->freqs[a]; break;
+        default: h->num[a] = 0.5; break;
+        }
+    }
+    // setScale (3670-3683)
+    h->scale = (double)h->nx * (double)h->ny;
+    h->bc_set = true;
+    h->tables_ready = false;
+    return PP_SUCCESS;
+}
+
+int pp_set_box(pp_handle h, const double *X)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!X) return PP_INPUT_ARGS;
+    for (int i = 0; i < 6; i++) h->Xbox[i] = X[i];
+    if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
+    h->box_set = true;
+    h->tables_ready = false;
+    if (h->coords_set) return pp_set_coords(h, 2);
+    return PP_SUCCESS;
+}
+
+int pp_set_coords(pp_handle h, int dir)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (dir < 0 || dir > 2) return PP_INPUT_ARGS;
+    // setDxyz (src/pencilDcmp.cpp:826-840); at one rank every orientation spans the whole box
+    const double *X = h->Xbox;
+    h->dxyz[0] = -(X[0] - X[1]) / (h->nx);
+    h->dxyz[1] = -(X[2] - X[3]) / (h->ny);
+    h->dxyz[2] = -(X[4] - X[5]) / (h->nz);
+    h->coords_set = true;
+    h->tables_ready = false;
+    return PP_SUCCESS;
+}
+
+int pp_assign_rhs(pp_handle h, const double *rhs)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!rhs) return PP_INPUT_ARGS;
+    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    PP_HIP_CHECK(hipMemcpyAsync(h->d_field, rhs, bytes, hipMemcpyHostToDevice, h->stream));
+    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->rhs_set = true;
+    return PP_SUCCESS;
+}
+
+int pp_assign_rhs_device(pp_handle h, const double *d_rhs)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!d_rhs) return PP_INPUT_ARGS;
+    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    PP_HIP_CHECK(hipMemcpyAsync(h->d_field, d_rhs, bytes, hipMemcpyDeviceToDevice, h->stream));
+    h->rhs_set = true;
+    return PP_SUCCESS;
+}
+
+int pp_pittpack(pp_handle h)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!h->rhs_set) return PP_CALL_ORDER;
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    rc = run_fp64(h, h->d_field);
+    if (rc) return rc;
+    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->rhs_set = false;  // the field now holds the solution
+    return PP_SUCCESS;
+}
+
+int pp_solve_device(pp_handle h, double *d_field)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!d_field) return PP_INPUT_ARGS;
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    return run_fp64(h, d_field);
+}
+
+int pp_get_solution(pp_handle h, double *u)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!u) return PP_INPUT_ARGS;
+    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    PP_HIP_CHECK(hipMemcpyAsync(u, h->d_field, bytes, hipMemcpyDeviceToHost, h->stream));
+    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return PP_SUCCESS;
+}
+
+int pp_get_solution_device(pp_handle h, double *d_u)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!d_u) return PP_INPUT_ARGS;
+    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    PP_HIP_CHECK(hipMemcpyAsync(d_u, h->d_field, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return PP_SUCCESS;
+}
+
+int pp_set_stream(pp_handle h, void *stream)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (h->own_stream) {
+        (void)hipStreamDestroy(h->stream);
+        h->own_stream = false;
+    }
+    h->stream = reinterpret_cast<hipStream_t>(stream);
+    return PP_SUCCESS;
+}
+
+int pp_set_option(pp_handle h, int key, long long value)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    switch (key) {
+    case PP_OPT_ZERO_MODE:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->zero_mode = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_TIMING:
+        h->timing = value != 0;
+        return PP_SUCCESS;
+    default: return PP_INPUT_ARGS;
+    }
+}
+
+int pp_get_stage_times(pp_handle h, float *ms5)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!ms5) return PP_INPUT_ARGS;
+    if (!h->timing) return PP_CALL_ORDER;
+    PP_HIP_CHECK(hipEventSynchronize(h->ev[5]));
+    for (int i = 0; i < 5; i++) PP_HIP_CHECK(hipEventElapsedTime(&ms5[i], h->ev[i], h->ev[i + 1]));
+    return PP_SUCCESS;
+}
+
+int pp_get_device_field(pp_handle h, double **d_field)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!d_field) return PP_INPUT_ARGS;
+    *d_field = h->d_field;
+    return PP_SUCCESS;
+}
+
+const char *pp_get_error_enum(int code)
+{
+    switch (code) {
+    case 0: return "SUCCESS";
+    case 1: return "GPU_INIT_FAILURE";
+    case 2: return "SOLVER_ARRAY_INCONSISTENT";
+    case 3: return "MPI_INIT_FAIL";
+    case 4: return "MPI_DUP_FAIL";
+    case 5: return "COMSIZE_FAIL";
+    case 6: return "MESH_DIVISIBLE";
+    case 7: return "MPI_GET_RANK_FAIL";
+    case 8: return "MPI_COMSIZE_FAIL";
+    case 9: return "MPI_DIST_GTAPH_FAIL_X";
+    case 10: return "MPI_DIST_GTAPH_FAIL_Y";
+    case 11: return "GRAPH_CREATE_FAIL";
+    case 12: return "MPI_INEIGHBOR_FAIL_ZX";
+    case 13: return "MPI_INEIGHBOR_FAIL_XY";
+    case 14: return "BLOCK_NUMBER_FAIL";
+    case 15: return "ALLOCATION_FAIL";
+    case 16: return "MPI_FINALIZE_FAIL";
+    case 17: return "CONNECTIVITY_CONSTRUCTION_FAIL";
+    case 18: return "THOMAS_FAIL";
+    case 19: return "INPUT_ARGS";
+    case 20: return "CUFFT_FAIL_X";
+    case 21: return "CUFFT_FAIL_INV_X";
+    case 22: return "CUFFT_FAIL_Y";
+    case 23: return "CUFFT_FAIL_INV_Y";
+    case PP_HIP_ERROR: return "HIP_ERROR";
+    case PP_UNSUPPORTED_BC: return "UNSUPPORTED_BC";
+    case PP_UNSUPPORTED_SIZE: return "UNSUPPORTED_SIZE";
+    case PP_INVALID_HANDLE: return "INVALID_HANDLE";
+    case PP_CALL_ORDER: return "CALL_ORDER";
+    case PP_NO_DEVICE: return "NO_DEVICE";
+    default: return "UNKNOWN_ERROR";
+    }
+}
+
+int pp_size_supported(int n) { return size_supported(n) ? 1 : 0; }
+
+int pp_selftest(void)
+{
+    int *d = nullptr;
+    int host[6 * 64];
+    if (hipMalloc((void **)&d, sizeof(host)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    hipLaunchKernelGGL(k_selftest_rot, dim3(1), dim3(64), 0, 0, d);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(host, d, sizeof(host), hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipFree(d);
+        return PP_HIP_ERROR;
+    }
+    (void)hipFree(d);
+    const int S[6] = {1, 2, 4, 8, -1, -8};
+    int bad = 0;
+    for (int r = 0; r < 6; r++)
+        for (int lane = 0; lane < 64; lane++) {
+            int row = lane & ~15, t = lane & 15;
+            int expect = row + ((t - S[r]) & 15);
+            if (host[r * 64 + lane] != expect) bad++;
+        }
+    return bad ? 1000 + bad : 0;
+}
+
+}  // extern "C"

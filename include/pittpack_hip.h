@@ -42,7 +42,8 @@ enum {
      * 1 (default): apply it only when the z operator is singular (z Neumann-Neumann); with a
      *    Dirichlet z face the true system is solved. */
     PP_OPT_ZERO_MODE = 1,
-    /* 1: record per-stage HIP events during pp_pittpack (pp_get_stage_times) */
+    /* value k > 0: record per-stage HIP events for the last k solves (pp_get_stage_times);
+     * 0 switches timing off */
     PP_OPT_TIMING = 2
 };
 
@@ -86,13 +87,19 @@ int pp_get_solution_device(pp_handle h, double *d_u);
  * stream, without a host synchronisation. */
 int pp_solve_device(pp_handle h, double *d_field);
 
+/* Out-of-place device form (no reference counterpart): reads the RHS from d_rhs (kept
+ * intact) and leaves the solution in d_u (both nx*ny*nz doubles, x fastest; d_u may equal
+ * d_rhs).  Asynchronous on the handle's stream. */
+int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u);
+
 /* Use the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
 int pp_set_stream(pp_handle h, void *stream);
 
 int pp_set_option(pp_handle h, int key, long long value);
 
-/* Durations (ms) of the five stages of the last timed pp_pittpack / pp_solve_device:
- * x forward, y forward, z solve, y inverse, x inverse.  Requires PP_OPT_TIMING. */
+/* Mean durations (ms) of the five stages over the solves recorded since the last call:
+ * x forward, y forward, z solve, y inverse, x inverse.  Requires PP_OPT_TIMING.
+ * Synchronises the handle's stream. */
 int pp_get_stage_times(pp_handle h, float *ms5);
 
 /* Device pointer of the handle's field (nx*ny*nz doubles). */

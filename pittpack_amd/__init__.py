@@ -35,7 +35,7 @@ OPT_TIMING = 2
 EXPORTED = (
     "pp_create", "pp_destroy", "pp_assign_boundary", "pp_set_box", "pp_set_coords",
     "pp_assign_rhs", "pp_assign_rhs_device", "pp_pittpack", "pp_get_solution",
-    "pp_get_solution_device", "pp_solve_device", "pp_set_stream", "pp_set_option",
+    "pp_get_solution_device", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
     "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
     "pp_selftest",
 )
@@ -72,6 +72,7 @@ def load_library(path: str | None = None):
     L.pp_get_solution.argtypes = [vp, dp]
     L.pp_get_solution_device.argtypes = [vp, vp]
     L.pp_solve_device.argtypes = [vp, vp]
+    L.pp_solve_device_oop.argtypes = [vp, vp, vp]
     L.pp_set_stream.argtypes = [vp, vp]
     L.pp_set_option.argtypes = [vp, i, ctypes.c_longlong]
     L.pp_get_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
@@ -167,6 +168,11 @@ class PoissonHIP:
     def solve_device(self, d_ptr: int) -> None:
         """Solve in place on a device array of nx*ny*nz doubles (e.g. tensor.data_ptr())."""
         _check(self._L.pp_solve_device(self._h, ctypes.c_void_p(d_ptr)), "solve_device")
+
+    def solve_device_oop(self, d_rhs: int, d_u: int) -> None:
+        """RHS device array kept; solution written to d_u (may equal d_rhs)."""
+        _check(self._L.pp_solve_device_oop(self._h, ctypes.c_void_p(d_rhs), ctypes.c_void_p(d_u)),
+               "solve_device_oop")
 
     def stage_times(self):
         ms = (ctypes.c_float * 5)()

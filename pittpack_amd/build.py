@@ -1,9 +1,12 @@
 """Build the HIP library pittpack_amd/lib/libpittpack_hip.so for gfx950 with hipcc.
 
+Each translation unit under csrc/ is compiled to an object in parallel, then linked.
 Usage: python -m pittpack_amd.build   (or pittpack_amd.build.build() from Python)
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
+import glob
 import os
 import shutil
 import subprocess
@@ -13,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
+OBJ_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(OUT_DIR, "libpittpack_hip.so")
 
 HIPCC_FLAGS = [
@@ -20,7 +24,6 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "--offload-arch=gfx950",
     "-fPIC",
-    "-shared",
     # packed FP32 instructions off: some of their forms are unreliable on these parts
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
@@ -34,20 +37,37 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
+def _compile(src: str, extra) -> str:
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-I" + os.path.join(ROOT, "include"), "-I" + SRC,
+           "-c", src, "-o", obj]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    with open(obj + ".log", "w") as fh:
+        fh.write(res.stdout + res.stderr)
+    if res.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed on {src}")
+    return obj
+
+
+def build(verbose: bool = False, extra=(), jobs: int | None = None) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(SRC, "*.hip")))
+    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, list(extra)), srcs))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I" + os.path.join(ROOT, "include"), "-I" + SRC,
-           os.path.join(SRC, "pp_solver.hip"), "-o", tmp]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("link failed")
     os.replace(tmp, OUT)
     return OUT
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, extra=sys.argv[1:]))

@@ -215,18 +215,75 @@ __device__ __forceinline__ void dft(cpx<R> *u)
 }
 
 // ---------------------------------------------------------------------------------------
-// One Stockham pass.  v[E]: this thread's elements t + T*m of the pass input.
-// p = product of the radices already applied.  tw[e] = exp(-2 pi i e / M).
-// LAST: results stay in v (same layout).  Otherwise results are written to `lds`
-// (the line's M-element complex slice) and read back in the t + T*m layout.
+// LDS layout: one padding slot every 16 complex elements, so the stride-R writes of a
+// radix-R pass spread over all banks.
 // ---------------------------------------------------------------------------------------
-template <typename R, int M, int T, int RAD, int P, bool LAST, int DIR>
-__device__ __forceinline__ void stockham_pass(cpx<R> *v, cpx<R> *lds, int t, const cpx<R> *__restrict__ tw)
+__device__ __forceinline__ constexpr int padc(int i) { return i + (i >> 4); }
+
+template <int M>
+constexpr int line_slots() { return M + M / 16 + 1; }
+
+// Twiddle seeds of the passes after the first: one base twiddle W_M^(k*S) per butterfly,
+// loaded from the FP64-computed table once at kernel start; powers follow by products.
+template <int M, int T, int IDX = 0, int P = 1>
+constexpr int count_tw()
+{
+    constexpr RadixPlan plan = make_plan(M, M / T);
+    if constexpr (IDX >= plan.n) return 0;
+    else {
+        constexpr int RAD = plan.r[IDX];
+        constexpr int NB = (M / T) / RAD;
+        return (P > 1 ? NB : 0) + count_tw<M, T, IDX + 1, P * RAD>();
+    }
+}
+
+template <typename R, int M, int T, int IDX, int P, int OFF>
+__device__ __forceinline__ void load_tw_from(cpx<R> *w, int t, const double *__restrict__ tw)
+{
+    constexpr RadixPlan plan = make_plan(M, M / T);
+    if constexpr (IDX < plan.n) {
+        constexpr int RAD = plan.r[IDX];
+        constexpr int NB = (M / T) / RAD;
+        if constexpr (P > 1) {
+            constexpr int S = M / (P * RAD);
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const int i = t + T * q;
+                const int e = ((i % P) * S) % M;
+                w[OFF + q] = {(R)tw[2 * e], (R)tw[2 * e + 1]};
+            }
+            load_tw_from<R, M, T, IDX + 1, P * RAD, OFF + NB>(w, t, tw);
+        } else {
+            load_tw_from<R, M, T, IDX + 1, P * RAD, OFF>(w, t, tw);
+        }
+    }
+}
+
+template <typename R, int M, int T>
+struct FFTSeeds {
+    static constexpr int NTW = count_tw<M, T>();
+    cpx<R> w[NTW > 0 ? NTW : 1];
+    // tw: FP64 table exp(-2 pi i e / M), interleaved
+    __device__ __forceinline__ void load(int t, const double *__restrict__ tw)
+    {
+        if constexpr (NTW > 0) load_tw_from<R, M, T, 0, 1, 0>(w, t, tw);
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// One Stockham pass.  v[E]: this thread's elements t + T*m of the pass input.
+// p = product of the radices already applied.  w1[q]: base twiddle of butterfly q.
+// LAST: results stay in v (same layout).  Otherwise results are written to `lds`
+// (the line's padded complex slice) and read back in the t + T*m layout.
+// SYNC: block barriers around the exchange (false when the block is a single wave).
+// ---------------------------------------------------------------------------------------
+template <typename R, int M, int T, int RAD, int P, bool LAST, int DIR, bool SYNC>
+__device__ __forceinline__ void stockham_pass(cpx<R> *v, cpx<R> *lds, int t, const cpx<R> *w1)
 {
     constexpr int E = M / T;
     constexpr int NB = E / RAD;  // butterflies per thread
     static_assert(E % RAD == 0, "radix must divide elements per thread");
-    cpx<R> out[E];
+    cpx<R> out[LAST ? E : 1];
 #pragma unroll
     for (int q = 0; q < NB; q++) {
         const int i = t + T * q;
@@ -235,13 +292,12 @@ __device__ __forceinline__ void stockham_pass(cpx<R> *v, cpx<R> *lds, int t, con
 #pragma unroll
         for (int r = 0; r < RAD; r++) u[r] = v[q + r * NB];
         if constexpr (P > 1) {
-            constexpr int S = M / (P * RAD);
+            cpx<R> wp[RAD];
+            wp[1] = (DIR < 0) ? w1[q] : c_conj(w1[q]);
 #pragma unroll
-            for (int r = 1; r < RAD; r++) {
-                const int e = (r * k * S) % M;
-                cpx<R> w = tw[e];
-                u[r] = (DIR < 0) ? c_mul(u[r], w) : c_mulc(u[r], w);
-            }
+            for (int r = 2; r < RAD; r++) wp[r] = (r % 2 == 0) ? c_mul(wp[r / 2], wp[r / 2]) : c_mul(wp[r - 1], wp[1]);
+#pragma unroll
+            for (int r = 1; r < RAD; r++) u[r] = c_mul(u[r], wp[r]);
         }
         dft<RAD, DIR>(u);
         if constexpr (LAST) {
@@ -250,38 +306,41 @@ __device__ __forceinline__ void stockham_pass(cpx<R> *v, cpx<R> *lds, int t, con
         } else {
             const int j = (i / P) * P * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; r++) lds[j + r * P] = u[r];
+            for (int r = 0; r < RAD; r++) lds[padc(j + r * P)] = u[r];
         }
     }
     if constexpr (LAST) {
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = out[m];
     } else {
-        __syncthreads();
+        if constexpr (SYNC) __syncthreads();
+        else __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int m = 0; m < E; m++) v[m] = lds[t + T * m];
-        __syncthreads();
+        for (int m = 0; m < E; m++) v[m] = lds[padc(t + T * m)];
+        if constexpr (SYNC) __syncthreads();
+        else __builtin_amdgcn_wave_barrier();
     }
 }
 
-template <typename R, int M, int T, int DIR, int IDX, int P>
-__device__ __forceinline__ void stockham_from(cpx<R> *v, cpx<R> *lds, int t, const cpx<R> *__restrict__ tw)
+template <typename R, int M, int T, int DIR, bool SYNC, int IDX, int P, int OFF>
+__device__ __forceinline__ void stockham_from(cpx<R> *v, cpx<R> *lds, int t, const cpx<R> *w)
 {
     constexpr RadixPlan plan = make_plan(M, M / T);
     if constexpr (IDX < plan.n) {
         constexpr int RAD = plan.r[IDX];
-        stockham_pass<R, M, T, RAD, P, IDX == plan.n - 1, DIR>(v, lds, t, tw);
-        stockham_from<R, M, T, DIR, IDX + 1, P * RAD>(v, lds, t, tw);
+        constexpr int NB = (M / T) / RAD;
+        stockham_pass<R, M, T, RAD, P, IDX == plan.n - 1, DIR, SYNC>(v, lds, t, w + OFF);
+        stockham_from<R, M, T, DIR, SYNC, IDX + 1, P * RAD, OFF + (P > 1 ? NB : 0)>(v, lds, t, w);
     }
 }
 
-// Full FFT of one line: v in/out in the t + T*m layout.  All threads of the block must
-// call this together (it contains block barriers when the plan has more than one pass).
-template <typename R, int M, int T, int DIR>
-__device__ __forceinline__ void line_fft(cpx<R> *v, cpx<R> *lds, int t, const cpx<R> *__restrict__ tw)
+// Full FFT of one line: v in/out in the t + T*m layout.  With SYNC all threads of the
+// block must call this together (block barriers around each exchange).
+template <typename R, int M, int T, int DIR, bool SYNC = true>
+__device__ __forceinline__ void line_fft(cpx<R> *v, cpx<R> *lds, int t, const FFTSeeds<R, M, T> &s)
 {
     static_assert(plan_ok(M, M / T), "no radix plan for this (M, T)");
-    stockham_from<R, M, T, DIR, 0, 1>(v, lds, t, tw);
+    stockham_from<R, M, T, DIR, SYNC, 0, 1, 0>(v, lds, t, s.w);
 }
 
 }  // namespace pp

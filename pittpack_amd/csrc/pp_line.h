@@ -1,0 +1,444 @@
+// pp_line.h -- batched real-to-real line transforms along x (contiguous lines) or y
+// (strided columns) for CDNA4 (gfx950).
+//
+// Replaces the reference's per-axis transform step: SignalProc's Makhoul pre/post
+// processing (src/signalProc.cpp:32-265) around cuFFT Z2Z (src/poissonGPU.cpp:5-98), the
+// x<->y plane transposes (src/pencilDcmp.cpp:2279-2515), assignRhs's dz^2 scaling
+// (5591-5647), modifyRhsDirichlet (4623-4835) and rescale (3503-3531).
+//
+// Every axis keeps a real line of length N = 2M real:
+//   PP : R2HC (half-complex)          / HC2R          (unnormalised, round trip N)
+//   NN : DCT-II  (2 sum x cos ...)    / DCT-III / 2   (round trip N)
+//   DD : DST-II                       / DST-III / 2
+//   ND : DCT-IV  (x2)                 / DCT-IV        (quarter-wave, round trip N)
+//   DN : DST-IV  (x2)                 / DST-IV
+// each through one complex FFT of length M on the packed line (z_m = v_{2m} + i v_{2m+1}).
+//
+// Pipeline inside one block: coalesced global load (+ fused input scaling / Dirichlet
+// term) -> scatter into the padded LDS slice of the line -> pre-processing into registers
+// -> Stockham passes -> post-processing -> coalesced global store (+ fused output scale).
+// Twiddles come from FP64 tables: a handful of seeds per thread are loaded at kernel
+// start and the per-element values follow by FP64 products, so no table load sits between
+// two barriers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pp_fft.h"
+
+namespace pp {
+
+enum AxisKind : int { KIND_DD = 0, KIND_NN = 1, KIND_PP = 2, KIND_ND = 3, KIND_DN = 4 };
+
+struct LineArgs {
+    const void *in;
+    void *out;
+    long nlines;  // x: number of lines (ny*nz); y: columns per plane (nx)
+    int ncols;    // y: element stride (= nx)
+    int nplanes;  // y: nz
+    // FP64 tables for this axis, interleaved complex:
+    //   tw[M] = exp(-2 pi i e/M), A[M+1] = exp(-2 pi i k/N), B[M+1] = exp(-i pi k/(2N)),
+    //   C[M] = exp(-i pi (4n+1)/(4N)), D[M] = exp(-i pi k/N)
+    const double *tw, *A, *B, *C, *D;
+    double in_scale;   // forward x: dz (applied twice: rhs*dz*dz); otherwise 1
+    double out_scale;  // inverse x: 1/(Nx*Ny); otherwise 1
+    // Dirichlet boundary term (forward x only): dtab = [ex(nx) ey(ny) ez(nz) exf0 exf1 eyf0
+    // eyf1 ezf0 ezf1], dmask bit f set when face f (xmin,xmax,ymin,ymax,zmin,zmax) is D.
+    const double *dtab;
+    int dmask;
+    int nx, ny, nz;
+};
+
+struct c64 {
+    double x, y;
+};
+__device__ __forceinline__ c64 m64(c64 a, c64 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ c64 ld64(const double *t, int k) { return {t[2 * k], t[2 * k + 1]}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> to_c(c64 a) { return {(R)a.x, (R)a.y}; }
+
+// position in the real view of a line slice where input element n is placed
+template <int N, int KIND, int FWD>
+__device__ __forceinline__ int load_pos(int n, bool &neg)
+{
+    neg = false;
+    if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+        const int a = (KIND == KIND_DN) ? (N - 1 - n) : n;
+        return (a & 1) ? (N - a) : a;
+    } else if constexpr (FWD && (KIND == KIND_NN || KIND == KIND_DD)) {
+        if constexpr (KIND == KIND_DD) neg = (n & 1);
+        return (n & 1) ? (N - 1 - (n >> 1)) : (n >> 1);
+    } else {
+        return n;
+    }
+}
+
+template <int N, int KIND, int FWD>
+__device__ __forceinline__ int store_pos(int n, bool &neg)
+{
+    neg = false;
+    if constexpr (!FWD && (KIND == KIND_NN || KIND == KIND_DD)) {
+        if constexpr (KIND == KIND_DD) neg = (n & 1);
+        return (n & 1) ? (N - 1 - (n >> 1)) : (n >> 1);
+    } else {
+        return n;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R &lr(cpx<R> *L, int p)
+{
+    return reinterpret_cast<R *>(&L[padc(p >> 1)])[p & 1];
+}
+
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+__global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
+{
+    constexpr int N = 2 * M;
+    constexpr int E = M / T;
+    constexpr int LS = line_slots<M>();
+    constexpr bool SYNC = true;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
+
+    const int tid = threadIdx.x;
+    int l, t;
+    if constexpr (AX == 0) {
+        t = tid % T;
+        l = tid / T;
+    } else {
+        l = tid % LINES;
+        t = tid / LINES;
+    }
+    cpx<R> *L = lds + l * LS;
+
+    const TIn *in = reinterpret_cast<const TIn *>(a.in);
+    TOut *out = reinterpret_cast<TOut *>(a.out);
+
+    long base, stride;
+    bool valid;
+    int ly = 0, lz = 0;
+    if constexpr (AX == 0) {
+        long line = (long)blockIdx.x * LINES + l;
+        valid = line < a.nlines;
+        if (!valid) line = 0;
+        base = line * N;
+        stride = 1;
+        ly = (int)(line % a.ny);
+        lz = (int)(line / a.ny);
+    } else {
+        long col = (long)blockIdx.x * LINES + l;
+        valid = col < a.ncols;
+        if (!valid) col = 0;
+        base = (long)blockIdx.y * N * a.ncols + col;
+        stride = a.ncols;
+    }
+
+    // ---- issue the coalesced global loads first ----
+    double xin[2 * E];
+#pragma unroll
+    for (int c = 0; c < 2 * E; c++) xin[c] = valid ? (double)in[base + (long)(t + T * c) * stride] : 0.0;
+
+    // ---- twiddle seeds (independent of the data, overlap the load latency) ----
+    FFTSeeds<R, M, T> fs;
+    fs.load(t, a.tw);
+    c64 s1 = {1, 0}, st1 = {1, 0}, s2 = {1, 0}, st2 = {1, 0};
+    if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+        s1 = ld64(a.C, t);   // C_k = C_t * D_T^m
+        s2 = ld64(a.D, t);   // D_k = D_t * D_T^m
+        st1 = ld64(a.D, T);
+        st2 = st1;
+    } else {
+        s1 = ld64(a.A, t);   // A_k = A_t * A_T^m
+        st1 = ld64(a.A, T);
+        if constexpr (KIND != KIND_PP) {
+            s2 = ld64(a.B, t);  // B_k = B_t * B_T^m
+            st2 = ld64(a.B, T);
+        }
+    }
+
+    // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
+#pragma unroll
+    for (int c = 0; c < 2 * E; c++) {
+        const int n = t + T * c;
+        double v = xin[c];
+        if constexpr (AX == 0 && FWD) {
+            v = v * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
+            v = v * a.in_scale;
+            if (a.dmask) {
+                const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
+                const double *ef = a.dtab + a.nx + a.ny + a.nz;
+                if ((a.dmask & 16) && lz == 0) v = v - 2. * ex[n] * ey[ly] * ef[4];
+                if ((a.dmask & 32) && lz == a.nz - 1) v = v - 2 * ex[n] * ey[ly] * ef[5];
+                if ((a.dmask & 1) && n == 0) v = v - 2. * ef[0] * ey[ly] * ez[lz];
+                if ((a.dmask & 2) && n == N - 1) v = v - 2. * ef[1] * ey[ly] * ez[lz];
+                if ((a.dmask & 4) && ly == 0) v = v - 2. * ex[n] * ef[2] * ez[lz];
+                if ((a.dmask & 8) && ly == a.ny - 1) v = v - 2. * ex[n] * ef[3] * ez[lz];
+            }
+        } else {
+            v = v * a.in_scale;
+        }
+        bool neg;
+        const int pos = load_pos<N, KIND, FWD>(n, neg);
+        lr(L, pos) = neg ? (R)(-v) : (R)v;
+    }
+    __syncthreads();
+
+    // ---- pre-processing: FFT input in registers, layout k = t + T*m ----
+    cpx<R> v[E];
+    if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+        c64 cw = s1;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            v[m] = c_mul(L[padc(t + T * m)], to_c<R>(cw));
+            cw = m64(cw, st1);
+        }
+    } else if constexpr (FWD) {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = L[padc(t + T * m)];
+    } else if constexpr (KIND == KIND_PP) {
+        // Z_k = (X_k + conj X_{M-k}) + i e^{2 pi i k/N} (X_k - conj X_{M-k})
+        c64 aw = s1;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + T * m;
+            const int k2 = M - k;
+            const c64 am = aw;
+            aw = m64(aw, st1);
+            cpx<R> X1 = {lr(L, k), (k == 0) ? (R)0 : lr(L, N - k)};
+            cpx<R> X2 = {lr(L, k2), (k2 == M) ? (R)0 : lr(L, N - k2)};
+            cpx<R> s = {X1.x + X2.x, X1.y - X2.y};
+            cpx<R> d = {X1.x - X2.x, X1.y + X2.y};
+            cpx<R> q = c_mulc(d, to_c<R>(am));
+            v[m] = {s.x - q.y, s.y + q.x};
+        }
+    } else {
+        // DCT-III/2 (NN) or DST-III/2 (DD, input reversed):
+        // V_k = 1/2 e^{i pi k/(2N)} (X_k - i X_{N-k}),  X_N = 0
+        // Z_k = (V_k + conj V_{M-k}) + i e^{2 pi i k/N} (V_k - conj V_{M-k})
+        c64 aw = s1, bw = s2;
+        const R h = (R)0.70710678118654752440084436210485;
+        auto X = [&](int q) -> R {
+            if (q >= N) return (R)0;
+            if constexpr (KIND == KIND_DD) return lr(L, N - 1 - q);
+            else return lr(L, q);
+        };
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + T * m;
+            const int k2 = M - k;
+            const c64 am = aw;
+            const cpx<R> b = to_c<R>(bw);
+            aw = m64(aw, st1);
+            bw = m64(bw, st2);
+            // conj(B_{M-k}) = e^{i pi/4} B_k
+            const cpx<R> bm = {h * (b.x - b.y), h * (b.x + b.y)};
+            cpx<R> V1 = c_scale(c_mulc(cpx<R>{X(k), -X(N - k)}, b), (R)0.5);
+            cpx<R> V2 = c_scale(c_mul(cpx<R>{X(k2), -X(N - k2)}, bm), (R)0.5);
+            cpx<R> s = {V1.x + V2.x, V1.y - V2.y};
+            cpx<R> d = {V1.x - V2.x, V1.y + V2.y};
+            cpx<R> q = c_mulc(d, to_c<R>(am));
+            v[m] = {s.x - q.y, s.y + q.x};
+        }
+    }
+    __syncthreads();
+
+    // ---- complex FFT of length M ----
+    constexpr bool INV_FFT = (!FWD) && (KIND == KIND_PP || KIND == KIND_NN || KIND == KIND_DD);
+    line_fft<R, M, T, INV_FFT ? +1 : -1, SYNC>(v, L, t, fs);
+
+    // ---- post-processing into the line's real LDS view ----
+    if constexpr (FWD && (KIND == KIND_PP || KIND == KIND_NN || KIND == KIND_DD)) {
+#pragma unroll
+        for (int m = 0; m < E; m++) L[padc(t + T * m)] = v[m];
+        __syncthreads();
+        c64 aw = s1;
+        R o1[E], o2[E];
+        int p1[E], p2[E];
+        if constexpr (KIND == KIND_PP) {
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                const int k = t + T * m;
+                cpx<R> Zk = L[padc(k)];
+                cpx<R> Zc = c_conj(L[padc(k == 0 ? 0 : M - k)]);
+                // X_k = 1/2 (Zk + Zc) - 1/2 i e^{-2 pi i k/N} (Zk - Zc)
+                cpx<R> s = c_scale(c_add(Zk, Zc), (R)0.5);
+                cpx<R> q = c_mul(to_c<R>(aw), c_scale(c_sub(Zk, Zc), (R)0.5));
+                aw = m64(aw, st1);
+                if (k == 0) {
+                    o1[m] = Zk.x + Zk.y;
+                    p1[m] = 0;
+                    o2[m] = Zk.x - Zk.y;
+                    p2[m] = M;
+                } else {
+                    o1[m] = s.x + q.y;
+                    p1[m] = k;
+                    o2[m] = s.y - q.x;
+                    p2[m] = N - k;
+                }
+            }
+        } else {
+            c64 bw = s2;
+            const R h = (R)0.70710678118654752440084436210485;
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                const int k = t + T * m;
+                cpx<R> Zk = L[padc(k)];
+                cpx<R> Zc = c_conj(L[padc(k == 0 ? 0 : M - k)]);
+                cpx<R> s = c_scale(c_add(Zk, Zc), (R)0.5);
+                cpx<R> q = c_mul(to_c<R>(aw), c_scale(c_sub(Zk, Zc), (R)0.5));
+                aw = m64(aw, st1);
+                const cpx<R> bk = to_c<R>(bw);
+                bw = m64(bw, st2);
+                cpx<R> X = {s.x + q.y, s.y - q.x};
+                // DCT-II: X_k = 2 Re(e^{-i pi k/(2N)} V_k), X_{N-k} = -2 Im(...)
+                if (k == 0) {
+                    o1[m] = (R)2 * X.x;  // V_0 real
+                    p1[m] = 0;
+                    o2[m] = (R)2 * h * (Zk.x - Zk.y);  // 2 Re(e^{-i pi/4} V_M), V_M real
+                    p2[m] = M;
+                } else {
+                    cpx<R> w = c_mul(bk, X);
+                    o1[m] = (R)2 * w.x;
+                    p1[m] = k;
+                    o2[m] = (R)-2 * w.y;
+                    p2[m] = N - k;
+                }
+                if constexpr (KIND == KIND_DD) {
+                    p1[m] = N - 1 - p1[m];
+                    p2[m] = N - 1 - p2[m];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            lr(L, p1[m]) = o1[m];
+            lr(L, p2[m]) = o2[m];
+        }
+    } else if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+        const R f = FWD ? (R)2 : (R)1;
+        c64 dw = s2;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + T * m;
+            cpx<R> y = c_mul(v[m], to_c<R>(dw));
+            dw = m64(dw, st2);
+            lr(L, 2 * k) = f * y.x;
+            lr(L, N - 1 - 2 * k) = (KIND == KIND_DN) ? f * y.y : -f * y.y;
+        }
+    } else {
+        // inverse PP / NN / DD: z_m holds (x_{2m}, x_{2m+1}) or the Makhoul-ordered v
+#pragma unroll
+        for (int m = 0; m < E; m++) L[padc(t + T * m)] = v[m];
+    }
+    __syncthreads();
+
+    // ---- coalesced store with fused output scaling ----
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < 2 * E; c++) {
+            const int n = t + T * c;
+            bool neg;
+            const int pos = store_pos<N, KIND, FWD>(n, neg);
+            double val = (double)lr(L, pos);
+            if (neg) val = -val;
+            val = a.out_scale * val;
+            out[base + (long)n * stride] = (TOut)val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// launch configuration per (R, M, AX)
+// ---------------------------------------------------------------------------------------
+constexpr int line_threads(int M, int AX)
+{
+    // E = M / T elements per thread: 8 for power-of-two lengths, 12 with a factor 3
+    (void)AX;
+    switch (M) {
+    case 8: return 1;
+    case 12: return 1;
+    case 16: return 2;
+    case 24: return 2;
+    case 32: return 4;
+    case 48: return 4;
+    case 64: return 8;
+    case 96: return 8;
+    case 128: return 16;
+    case 192: return 16;
+    case 256: return 32;
+    case 384: return 32;
+    case 512: return 64;
+    default: return 0;
+    }
+}
+
+template <typename R, int M, int AX>
+constexpr int line_lines()
+{
+    constexpr int T = line_threads(M, AX);
+    constexpr int bytes = line_slots<M>() * 2 * (int)sizeof(R);
+    int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
+    if (lines < 1) lines = 1;
+    if (lines > 64) lines = 64;
+    const int budget = (AX == 0) ? 40 * 1024 : 80 * 1024;
+    while (lines > 1 && lines * bytes > budget) lines /= 2;
+    while (lines > 1 && lines * T > 512) lines /= 2;
+    return lines;
+}
+
+template <typename R, int M, int AX, int FWD, int KIND>
+hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
+{
+    constexpr int T = line_threads(M, AX);
+    constexpr int LINES = line_lines<R, M, AX>();
+    const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>);
+    auto kern = k_line<double, double, R, M, T, LINES, AX, FWD, KIND>;
+    if (lds > 64 * 1024) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+    }
+    dim3 block(T * LINES);
+    dim3 grid;
+    if (AX == 0) grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES));
+    else grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES), (unsigned)a.nplanes);
+    hipLaunchKernelGGL(kern, grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename R, int AX, int FWD, int KIND>
+hipError_t launch_line_sz(int N, const LineArgs &a, hipStream_t s)
+{
+    switch (N / 2) {
+    case 8: return launch_line_cfg<R, 8, AX, FWD, KIND>(a, s);
+    case 12: return launch_line_cfg<R, 12, AX, FWD, KIND>(a, s);
+    case 16: return launch_line_cfg<R, 16, AX, FWD, KIND>(a, s);
+    case 24: return launch_line_cfg<R, 24, AX, FWD, KIND>(a, s);
+    case 32: return launch_line_cfg<R, 32, AX, FWD, KIND>(a, s);
+    case 48: return launch_line_cfg<R, 48, AX, FWD, KIND>(a, s);
+    case 64: return launch_line_cfg<R, 64, AX, FWD, KIND>(a, s);
+    case 96: return launch_line_cfg<R, 96, AX, FWD, KIND>(a, s);
+    case 128: return launch_line_cfg<R, 128, AX, FWD, KIND>(a, s);
+    case 192: return launch_line_cfg<R, 192, AX, FWD, KIND>(a, s);
+    case 256: return launch_line_cfg<R, 256, AX, FWD, KIND>(a, s);
+    case 384: return launch_line_cfg<R, 384, AX, FWD, KIND>(a, s);
+    case 512: return launch_line_cfg<R, 512, AX, FWD, KIND>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// one instantiation per (R, KIND) lives in pp_line_<kind>.hip
+template <typename R, int KIND>
+hipError_t launch_line_kind(int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
+{
+    if (AX == 0) return FWD ? launch_line_sz<R, 0, 1, KIND>(N, a, s) : launch_line_sz<R, 0, 0, KIND>(N, a, s);
+    return FWD ? launch_line_sz<R, 1, 1, KIND>(N, a, s) : launch_line_sz<R, 1, 0, KIND>(N, a, s);
+}
+
+inline bool line_size_supported(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }
+
+}  // namespace pp

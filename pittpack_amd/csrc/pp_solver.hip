@@ -9,7 +9,8 @@
 #include <vector>
 
 #include "pittpack_hip.h"
-#include "pp_kernels.h"
+#include "pp_line.h"
+#include "pp_zsolve.h"
 
 using namespace pp;
 
@@ -17,77 +18,31 @@ namespace {
 
 constexpr double kPi = 3.1415926535897932384;  // src/include/params.h:116
 
-// ---------------------------------------------------------------------------------------
-// compiled transform sizes: N = 2M, (M, T) with E = M/T elements per thread
-// ---------------------------------------------------------------------------------------
-constexpr int threads_per_line(int M)
-{
-    switch (M) {
-    case 8: return 1;
-    case 12: return 1;
-    case 16: return 1;
-    case 24: return 2;
-    case 32: return 2;
-    case 48: return 4;
-    case 64: return 4;
-    case 96: return 4;
-    case 128: return 8;
-    case 192: return 8;
-    case 256: return 16;
-    case 384: return 16;
-    case 512: return 32;
-    default: return 0;
-    }
-}
+// line transforms: one translation unit per kind (pp_line_<kind>.hip)
+}  // namespace
+namespace pp {
+extern template hipError_t launch_line_kind<double, KIND_DD>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<double, KIND_NN>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<double, KIND_PP>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<double, KIND_ND>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<double, KIND_DN>(int, int, int, const LineArgs &, hipStream_t);
+}  // namespace pp
+namespace {
 
-template <typename R, int M, int AX>
-constexpr int lines_per_block()
+template <typename R>
+hipError_t launch_line(int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
 {
-    constexpr int T = threads_per_line(M);
-    constexpr int bytes_per_line = (M + 1) * 2 * (int)sizeof(R);
-    int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
-    if (lines > 64) lines = 64;
-    while (lines > 1 && lines * bytes_per_line > 64 * 1024) lines /= 2;
-    while (lines > 1 && lines * T > 1024) lines /= 2;
-    return lines;
-}
-
-template <typename R, int M, int AX, int FWD>
-hipError_t launch_line_t(const LineArgs &a, long nlines, int nplanes, hipStream_t s)
-{
-    constexpr int T = threads_per_line(M);
-    constexpr int LINES = lines_per_block<R, M, AX>();
-    const size_t lds = (size_t)LINES * (M + 1) * sizeof(cpx<R>);
-    dim3 block(T * LINES);
-    dim3 grid;
-    if (AX == 0) grid = dim3((unsigned)((nlines + LINES - 1) / LINES));
-    else grid = dim3((unsigned)((nlines + LINES - 1) / LINES), (unsigned)nplanes);
-    hipLaunchKernelGGL((k_line<double, double, R, M, T, LINES, AX, FWD>), grid, block, lds, s, a);
-    return hipGetLastError();
-}
-
-template <typename R, int AX, int FWD>
-hipError_t launch_line(int N, const LineArgs &a, long nlines, int nplanes, hipStream_t s)
-{
-    switch (N / 2) {
-    case 8: return launch_line_t<R, 8, AX, FWD>(a, nlines, nplanes, s);
-    case 12: return launch_line_t<R, 12, AX, FWD>(a, nlines, nplanes, s);
-    case 16: return launch_line_t<R, 16, AX, FWD>(a, nlines, nplanes, s);
-    case 24: return launch_line_t<R, 24, AX, FWD>(a, nlines, nplanes, s);
-    case 32: return launch_line_t<R, 32, AX, FWD>(a, nlines, nplanes, s);
-    case 48: return launch_line_t<R, 48, AX, FWD>(a, nlines, nplanes, s);
-    case 64: return launch_line_t<R, 64, AX, FWD>(a, nlines, nplanes, s);
-    case 96: return launch_line_t<R, 96, AX, FWD>(a, nlines, nplanes, s);
-    case 128: return launch_line_t<R, 128, AX, FWD>(a, nlines, nplanes, s);
-    case 192: return launch_line_t<R, 192, AX, FWD>(a, nlines, nplanes, s);
-    case 256: return launch_line_t<R, 256, AX, FWD>(a, nlines, nplanes, s);
-    case 384: return launch_line_t<R, 384, AX, FWD>(a, nlines, nplanes, s);
-    case 512: return launch_line_t<R, 512, AX, FWD>(a, nlines, nplanes, s);
+    switch (kind) {
+    case KIND_DD: return launch_line_kind<R, KIND_DD>(N, AX, FWD, a, s);
+    case KIND_NN: return launch_line_kind<R, KIND_NN>(N, AX, FWD, a, s);
+    case KIND_PP: return launch_line_kind<R, KIND_PP>(N, AX, FWD, a, s);
+    case KIND_ND: return launch_line_kind<R, KIND_ND>(N, AX, FWD, a, s);
+    case KIND_DN: return launch_line_kind<R, KIND_DN>(N, AX, FWD, a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-bool size_supported(int n) { return n % 2 == 0 && threads_per_line(n / 2) > 0; }
+bool size_supported(int n) { return line_size_supported(n); }
 
 // z solve: J = rows per lane, rounded up to a compiled value
 constexpr int kJ[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64};
@@ -191,7 +146,9 @@ struct pp_solver {
     double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
     DevTables tab64[2];
     int dmask = 0;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // stage-timing ring: slot s holds 6 events bracketing the five stages of one solve
+    std::vector<hipEvent_t> ev;
+    int ev_slots = 0, ev_next = 0, ev_used = 0;
     bool tables_ready = false;
 };
 
@@ -286,13 +243,12 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
     LineArgs a{};
     a.in = field;
     a.out = field;
-    a.kind = h->kind[axis];
     const DevTables &t = h->tab64[axis];
-    a.tw = t.tw;
-    a.A = t.A;
-    a.B = t.B;
-    a.C = t.C;
-    a.D = t.D;
+    a.tw = (const double *)t.tw;
+    a.A = (const double *)t.A;
+    a.B = (const double *)t.B;
+    a.C = (const double *)t.C;
+    a.D = (const double *)t.D;
     a.in_scale = 1.0;
     a.out_scale = 1.0;
     a.dtab = h->d_dtab;
@@ -310,25 +266,32 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
     return a;
 }
 
-// the five stages on `field` (in place), FP64
-int run_fp64(pp_solver *h, double *field)
+// the five stages, FP64: rhs -> field (x forward), then in place on field
+int run_fp64(pp_solver *h, const double *rhs, double *field)
 {
     hipStream_t s = h->stream;
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[0], s));
+    hipEvent_t *ev = nullptr;
+    if (h->timing) {
+        ev = &h->ev[6 * (size_t)h->ev_next];
+        h->ev_next = (h->ev_next + 1) % h->ev_slots;
+        if (h->ev_used < h->ev_slots) h->ev_used++;
+    }
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[0], s));
     // 1. forward x: rhs * dz^2, Dirichlet boundary term, x transform
     {
         LineArgs a = line_args(h, 0, field);
+        a.in = rhs;
         a.in_scale = h->dxyz[2];
         a.dmask = h->dmask;
-        PP_HIP_CHECK((launch_line<double, 0, 1>(h->nx, a, a.nlines, 1, s)));
+        PP_HIP_CHECK(launch_line<double>(h->kind[0], h->nx, 0, 1, a, s));
     }
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[1], s));
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[1], s));
     // 2. forward y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK((launch_line<double, 1, 1>(h->ny, a, a.nlines, h->nz, s)));
+        PP_HIP_CHECK(launch_line<double>(h->kind[1], h->ny, 1, 1, a, s));
     }
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[2], s));
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[2], s));
     // 3. z solve
     {
         ZArgs z{};
@@ -345,20 +308,20 @@ int run_fp64(pp_solver *h, double *field)
         z.zero_mode = h->zero_mode;
         PP_HIP_CHECK(launch_z<double>(pick_J(h->nz), z, s));
     }
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[3], s));
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[3], s));
     // 4. inverse y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK((launch_line<double, 1, 0>(h->ny, a, a.nlines, h->nz, s)));
+        PP_HIP_CHECK(launch_line<double>(h->kind[1], h->ny, 1, 0, a, s));
     }
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[4], s));
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[4], s));
     // 5. inverse x with the 1/(Nx*Ny) rescale fused
     {
         LineArgs a = line_args(h, 0, field);
         a.out_scale = 1. / (h->scale);
-        PP_HIP_CHECK((launch_line<double, 0, 0>(h->nx, a, a.nlines, 1, s)));
+        PP_HIP_CHECK(launch_line<double>(h->kind[0], h->nx, 0, 0, a, s));
     }
-    if (h->timing) PP_HIP_CHECK(hipEventRecord(h->ev[5], s));
+    if (ev) PP_HIP_CHECK(hipEventRecord(ev[5], s));
     return PP_SUCCESS;
 }
 
@@ -391,12 +354,6 @@ int pp_create(int nx, int ny, int nz, pp_handle *out)
         return PP_HIP_ERROR;
     }
     h->own_stream = true;
-    for (int i = 0; i < 6; i++) {
-        if (hipEventCreate(&h->ev[i]) != hipSuccess) {
-            pp_destroy(h);
-            return PP_HIP_ERROR;
-        }
-    }
     *out = h;
     return PP_SUCCESS;
 }
@@ -409,6 +366,7 @@ int pp_destroy(pp_handle h)
     if (h->d_field) (void)hipFree(h->d_field);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    h->ev.clear();
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PP_SUCCESS;
@@ -516,7 +474,7 @@ int pp_pittpack(pp_handle h)
     if (!h->rhs_set) return PP_CALL_ORDER;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    rc = run_fp64(h, h->d_field);
+    rc = run_fp64(h, h->d_field, h->d_field);
     if (rc) return rc;
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->rhs_set = false;  // the field now holds the solution
@@ -529,7 +487,16 @@ int pp_solve_device(pp_handle h, double *d_field)
     if (!d_field) return PP_INPUT_ARGS;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    return run_fp64(h, d_field);
+    return run_fp64(h, d_field, d_field);
+}
+
+int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!d_rhs || !d_u) return PP_INPUT_ARGS;
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    return run_fp64(h, d_rhs, d_u);
 }
 
 int pp_get_solution(pp_handle h, double *u)
@@ -571,9 +538,20 @@ int pp_set_option(pp_handle h, int key, long long value)
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zero_mode = (int)value;
         return PP_SUCCESS;
-    case PP_OPT_TIMING:
+    case PP_OPT_TIMING: {
+        if (value < 0 || value > 4096) return PP_INPUT_ARGS;
+        PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+        for (auto &e : h->ev)
+            if (e) (void)hipEventDestroy(e);
+        h->ev.assign(6 * (size_t)value, nullptr);
+        for (auto &e : h->ev)
+            if (hipEventCreate(&e) != hipSuccess) return PP_HIP_ERROR;
+        h->ev_slots = (int)value;
+        h->ev_next = 0;
+        h->ev_used = 0;
         h->timing = value != 0;
         return PP_SUCCESS;
+    }
     default: return PP_INPUT_ARGS;
     }
 }
@@ -582,9 +560,20 @@ int pp_get_stage_times(pp_handle h, float *ms5)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!ms5) return PP_INPUT_ARGS;
-    if (!h->timing) return PP_CALL_ORDER;
-    PP_HIP_CHECK(hipEventSynchronize(h->ev[5]));
-    for (int i = 0; i < 5; i++) PP_HIP_CHECK(hipEventElapsedTime(&ms5[i], h->ev[i], h->ev[i + 1]));
+    if (!h->timing || h->ev_used == 0) return PP_CALL_ORDER;
+    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int slot = 0; slot < h->ev_used; slot++) {
+        hipEvent_t *ev = &h->ev[6 * (size_t)slot];
+        for (int i = 0; i < 5; i++) {
+            float ms = 0.f;
+            PP_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            acc[i] += ms;
+        }
+    }
+    for (int i = 0; i < 5; i++) ms5[i] = (float)(acc[i] / h->ev_used);
+    h->ev_used = 0;
+    h->ev_next = 0;
     return PP_SUCCESS;
 }
 

@@ -59,7 +59,8 @@ template <typename R, int J>
 hipError_t launch_z_t(const ZArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.SX + 15) / 16), (unsigned)a.SY);
-    hipLaunchKernelGGL((k_zsolve<R, J>), grid, dim3(256), 0, s, a);
+    if (a.nz == 16 * J) hipLaunchKernelGGL((k_zsolve<R, J, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_zsolve<R, J, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -45,17 +45,18 @@ __device__ __forceinline__ float rcp_nr(float x)
 }
 
 // value of x from lane (t - S) mod 16 of this lane's 16-lane row
+// (DPP row_ror:S; every source lane is valid, so no "old" operand is needed)
 template <int S>
 __device__ __forceinline__ float rot_from_lower(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x120 + S, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x120 + S, 0xF, 0xF, true));
 }
 template <int S>
 __device__ __forceinline__ double rot_from_lower(double x)
 {
     int lo = __double2loint(x), hi = __double2hiint(x);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x120 + S, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x120 + S, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x120 + S, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x120 + S, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 // value of x from lane (t + S) mod 16
@@ -161,7 +162,8 @@ __device__ __forceinline__ void pcr_step(R *d, Row3<R> *ex, R &ta, R &tb, int t)
     }
 }
 
-template <typename R, int J>
+// FULL: nz == 16 * J (no padding rows, no per-row bounds checks)
+template <typename R, int J, bool FULL>
 __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
 {
     constexpr bool ALL = (J <= 3);
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
     const int sy = blockIdx.y;
     const bool valid = sx < a.SX;
     if (!valid) sx = a.SX - 1;
-    const int n = a.nz;
+    const int n = FULL ? 16 * J : a.nz;
     R *__restrict__ data = reinterpret_cast<R *>(a.data);
     const long zs = (long)a.SX * a.SY;
     const long col = (long)sy * a.SX + sx;
@@ -194,10 +196,12 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
     };
 
     R d[J];
+    const R *src = data + col + (long)t * zs;
+    const long zs16 = 16 * zs;
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int i = t + 16 * j;
-        d[j] = (i < n) ? data[(long)i * zs + col] : (R)0;
+        d[j] = (FULL || i < n) ? src[j * zs16] : (R)0;
     }
     Row3<R> ex[NEX];
     if constexpr (ALL) {
@@ -243,10 +247,11 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
     for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
 
     if (valid) {
+        R *dst = data + col + (long)t * zs;
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const int i = t + 16 * j;
-            if (i < n) data[(long)i * zs + col] = d[j];
+            if (FULL || i < n) dst[j * zs16] = d[j];
         }
     }
 }

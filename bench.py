@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,6 +88,8 @@ def main():
     M.assignBoundary(BC)
     M.setBox([0, 1, 0, 1, 0, 1])
     M.setCoords(2)
+    if args.zsolver is not None:
+        M.set_option(pa.OPT_ZSOLVER, args.zsolver)
     stream = torch.cuda.current_stream(dev)
     M.set_stream(stream.cuda_stream)
 

@@ -44,7 +44,10 @@ enum {
     PP_OPT_ZERO_MODE = 1,
     /* value k > 0: record per-stage HIP events for the last k solves (pp_get_stage_times);
      * 0 switches timing off */
-    PP_OPT_TIMING = 2
+    PP_OPT_TIMING = 2,
+    /* z solver: 0 = PCR across 16 lanes then Thomas per lane (cyclic rows);
+     *           1 = partitioned Thomas, contiguous rows per lane, cross-lane scans */
+    PP_OPT_ZSOLVER = 3
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

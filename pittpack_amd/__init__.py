@@ -30,6 +30,7 @@ NO_DEVICE = 29
 
 OPT_ZERO_MODE = 1
 OPT_TIMING = 2
+OPT_ZSOLVER = 3
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -197,13 +198,15 @@ class PoissonHIP:
         self.close()
 
 
-def solve(rhs, bc: str, box=None, zero_mode: int | None = None) -> np.ndarray:
+def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None) -> np.ndarray:
     """One-shot FP64 solve of an (nz, ny, nx) RHS."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     with PoissonHIP(nx, ny, nz) as M:
         if zero_mode is not None:
             M.set_option(OPT_ZERO_MODE, zero_mode)
+        if zsolver is not None:
+            M.set_option(OPT_ZSOLVER, zsolver)
         M.assignBoundary(bc)
         M.setBox(box if box is not None else [0, 1, 0, 1, 0, 1])
         M.setCoords(2)

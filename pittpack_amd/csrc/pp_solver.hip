@@ -12,6 +12,22 @@
 #include "pp_line.h"
 #include "pp_zsolve.h"
 
+namespace pp {
+// lane-rotation self test: out[lane] = value rotated from lane (t - S) for S = 1,2,4,8
+__global__ void k_selftest_rot(int *out)
+{
+    const int lane = threadIdx.x;
+    const float x = (float)lane;
+    out[lane + 0 * 64] = (int)rot_from_lower<1>(x);
+    out[lane + 1 * 64] = (int)rot_from_lower<2>(x);
+    out[lane + 2 * 64] = (int)rot_from_lower<4>(x);
+    out[lane + 3 * 64] = (int)rot_from_lower<8>(x);
+    out[lane + 4 * 64] = (int)rot_from_upper<1>((double)lane);
+    out[lane + 5 * 64] = (int)rot_from_upper<8>((double)lane);
+}
+
+}  // namespace pp
+
 using namespace pp;
 
 namespace {
@@ -44,38 +60,16 @@ hipError_t launch_line(int kind, int N, int AX, int FWD, const LineArgs &a, hipS
 
 bool size_supported(int n) { return line_size_supported(n); }
 
-// z solve: J = rows per lane, rounded up to a compiled value
-constexpr int kJ[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64};
-
-int pick_J(int nz)
-{
-    int need = (nz + 15) / 16;
-    for (int j : kJ)
-        if (j >= need) return j;
-    return 0;
-}
-
-template <typename R, int J>
-hipError_t launch_z_t(const ZArgs &a, hipStream_t s)
-{
-    dim3 grid((unsigned)((a.SX + 15) / 16), (unsigned)a.SY);
-    if (a.nz == 16 * J) hipLaunchKernelGGL((k_zsolve<R, J, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_zsolve<R, J, false>), grid, dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_z(int J, const ZArgs &a, hipStream_t s)
-{
-    switch (J) {
-#define PPJ(v) \
-    case v: return launch_z_t<R, v>(a, s);
-        PPJ(1) PPJ(2) PPJ(3) PPJ(4) PPJ(5) PPJ(6) PPJ(7) PPJ(8) PPJ(10) PPJ(12) PPJ(14) PPJ(16)
-        PPJ(20) PPJ(24) PPJ(28) PPJ(32) PPJ(40) PPJ(48) PPJ(56) PPJ(64)
-#undef PPJ
-    default: return hipErrorInvalidValue;
-    }
-}
+// z solve (pp_zsolve_pcr.hip, pp_zsolve_scan.hip).  FP64 columns hold at most 32 rows per
+// lane (nz <= 512): larger columns would spill registers in kernels that exchange data
+// across lanes.
+}  // namespace
+namespace pp {
+hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s);
+hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
+bool zsolve_supported_f64(int nz);
+}  // namespace pp
+namespace {
 
 // exactValue (src/pencilDcmp.cpp:1837-1866)
 double exact_value(double omega, double x, int tg)
@@ -140,6 +134,7 @@ struct pp_solver {
     double scale = 1.0;
     double dxyz[3] = {0, 0, 0};
     int zero_mode = 1;
+    int zsolver = 0;
     bool timing = false;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -307,7 +302,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         z.zlo_dirichlet = h->zlo_d;
         z.zhi_dirichlet = h->zhi_d;
         z.zero_mode = h->zero_mode;
-        PP_HIP_CHECK(launch_z<double>(pick_J(h->nz), z, s));
+        PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->nz, z, s) : launch_zsolve_pcr_f64(h->nz, z, s));
     }
     if (ev) PP_HIP_CHECK(hipEventRecord(ev[3], s));
     // 4. inverse y
@@ -336,7 +331,7 @@ int pp_create(int nx, int ny, int nz, pp_handle *out)
     *out = nullptr;
     if (nx < 2 || ny < 2 || nz < 4) return PP_INPUT_ARGS;
     if (!size_supported(nx) || !size_supported(ny)) return PP_UNSUPPORTED_SIZE;
-    if (pick_J(nz) == 0) return PP_UNSUPPORTED_SIZE;
+    if (!zsolve_supported_f64(nz)) return PP_UNSUPPORTED_SIZE;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PP_NO_DEVICE;
     pp_solver *h = new (std::nothrow) pp_solver();
@@ -538,6 +533,10 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_ZERO_MODE:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zero_mode = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_ZSOLVER:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->zsolver = (int)value;
         return PP_SUCCESS;
     case PP_OPT_TIMING: {
         if (value < 0 || value > 4096) return PP_INPUT_ARGS;

@@ -27,6 +27,27 @@ struct ZArgs {
     int zero_mode;                      // 0: reference row-0 rule; 1: only when singular (z NN)
 };
 
+// Column element access.  The row stride zs is uniform, so row j of a lane sits at the
+// uniform address data + j*zs (scalar registers) plus one per-lane 32-bit byte offset:
+// this keeps one VGPR per lane for all rows instead of one 64-bit address per row.
+// Valid while the field is below 4 GiB (the host picks the 64-bit form otherwise).
+template <typename R, bool OFF32>
+struct ColPtr {
+    const char *base;
+    long zsb;      // row stride in bytes
+    uint32_t off;  // lane offset in bytes (OFF32)
+    long off64;    // lane offset in bytes (!OFF32)
+    __device__ __forceinline__ ColPtr(R *data, long zs, long lane_elem)
+        : base(reinterpret_cast<const char *>(data)), zsb(zs * (long)sizeof(R)),
+          off((uint32_t)(lane_elem * (long)sizeof(R))), off64(lane_elem * (long)sizeof(R)) {}
+    __device__ __forceinline__ R *at(int j) const
+    {
+        const char *rowbase = base + (long)j * zsb;
+        if constexpr (OFF32) return (R *)(rowbase + off);
+        else return (R *)(rowbase + off64);
+    }
+};
+
 // 1/x: hardware reciprocal refined by two Newton steps (within an ulp or two of 1/x; the
 // pivots here are diagonally dominant, never near zero or denormal)
 __device__ __forceinline__ double rcp_nr(double x)
@@ -163,7 +184,7 @@ __device__ __forceinline__ void pcr_step(R *d, Row3<R> *ex, R &ta, R &tb, int t)
 }
 
 // FULL: nz == 16 * J (no padding rows, no per-row bounds checks)
-template <typename R, int J, bool FULL>
+template <typename R, int J, bool FULL, bool OFF32>
 __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
 {
     constexpr bool ALL = (J <= 3);
@@ -196,12 +217,11 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
     };
 
     R d[J];
-    const R *src = data + col + (long)t * zs;
-    const long zs16 = 16 * zs;
+    const ColPtr<R, OFF32> cp16(data, 16 * zs, col + (long)t * zs);
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int i = t + 16 * j;
-        d[j] = (FULL || i < n) ? src[j * zs16] : (R)0;
+        d[j] = (FULL || i < n) ? *cp16.at(j) : (R)0;
     }
     Row3<R> ex[NEX];
     if constexpr (ALL) {
@@ -247,26 +267,228 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
     for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
 
     if (valid) {
-        R *dst = data + col + (long)t * zs;
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const int i = t + 16 * j;
-            if (FULL || i < n) dst[j * zs16] = d[j];
+            if (FULL || i < n) *cp16.at(j) = d[j];
         }
     }
 }
 
-// lane-rotation self test: out[lane] = value rotated from lane (t - S) for S = 1,2,4,8
-__global__ void k_selftest_rot(int *out)
+// ---------------------------------------------------------------------------------------
+// Partitioned Thomas with cross-lane scans (second z solver).
+//
+// Lane t of a column owns the contiguous rows tJ .. tJ+J-1.  LU without pivoting:
+//   m_i = a_i / u_{i-1},  u_i = b_i - m_i c_{i-1},  y_i = d_i - m_i y_{i-1},
+//   x_i = (y_i - c_i x_{i+1}) / u_i.
+// The pivot entering a lane, u_{tJ-1}, follows from u_1 by the Moebius map
+// u -> b - 1/u, i.e. the vector (p, q) ~ u = p/q multiplied by G^(tJ-2), G = [[b,-1],[1,0]],
+// computed by binary powering with exact power-of-two rescaling.  The forward and backward
+// sweeps run locally per lane from zero boundary values; their affine dependence on the
+// neighbouring lane's value is stitched by Kogge-Stone scans over the 16 lanes (DPP), then
+// one fix-up pass per direction.
+// ---------------------------------------------------------------------------------------
+struct Mat2 {
+    double a, b, c, d;
+};
+
+__device__ __forceinline__ double pow2_scale(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+
+__device__ __forceinline__ Mat2 mat_sq_norm(Mat2 m)
 {
-    const int lane = threadIdx.x;
-    const float x = (float)lane;
-    out[lane + 0 * 64] = (int)rot_from_lower<1>(x);
-    out[lane + 1 * 64] = (int)rot_from_lower<2>(x);
-    out[lane + 2 * 64] = (int)rot_from_lower<4>(x);
-    out[lane + 3 * 64] = (int)rot_from_lower<8>(x);
-    out[lane + 4 * 64] = (int)rot_from_upper<1>((double)lane);
-    out[lane + 5 * 64] = (int)rot_from_upper<8>((double)lane);
+    Mat2 r = {m.a * m.a + m.b * m.c, m.a * m.b + m.b * m.d, m.c * m.a + m.d * m.c, m.c * m.b + m.d * m.d};
+    const double mx = fmax(fmax(fabs(r.a), fabs(r.b)), fmax(fabs(r.c), fabs(r.d)));
+    const int e = -__builtin_amdgcn_frexp_exp(mx);
+    return {pow2_scale(r.a, e), pow2_scale(r.b, e), pow2_scale(r.c, e), pow2_scale(r.d, e)};
+}
+
+constexpr int bits_for(int v)
+{
+    int b = 0;
+    while ((1 << b) <= v) b++;
+    return b;
+}
+
+template <int S, typename R>
+__device__ __forceinline__ R from_lower(R x) { return rot_from_lower<S>(x); }
+template <int S, typename R>
+__device__ __forceinline__ R from_upper(R x) { return rot_from_upper<S>(x); }
+
+template <typename R, int J, bool FULL, bool OFF32>
+__global__ void __launch_bounds__(256) k_zsolve_scan(ZArgs a)
+{
+    const int tid = threadIdx.x;
+    const int t = tid & 15;
+    const int cidx = tid >> 4;
+    int sx = blockIdx.x * 16 + cidx;
+    const int sy = blockIdx.y;
+    const bool valid = sx < a.SX;
+    if (!valid) sx = a.SX - 1;
+    const int n = FULL ? 16 * J : a.nz;
+    R *__restrict__ data = reinterpret_cast<R *>(a.data);
+    const long zs = (long)a.SX * a.SY;
+    const long col = (long)sy * a.SX + sx;
+
+    const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sx], a.Cx)), __dmul_rn(a.lamy[sy], a.Cy));
+    bool pin = fabs(eig + 2.) < 1.e-10;
+    if (pin && a.zero_mode == 1 && (a.zlo_dirichlet || a.zhi_dirichlet)) pin = false;
+    const double b0 = pin ? 1.0 : (a.zlo_dirichlet ? eig - 1. : eig + 1.);
+    const double c0 = pin ? 0.0 : 1.0;
+    const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
+
+    const int i0 = t * J;  // first row of this lane
+    R d[J];
+    const ColPtr<R, OFF32> cpz(data, zs, col + (long)i0 * zs);
+#pragma unroll
+    for (int j = 0; j < J; j++) d[j] = (FULL || i0 + j < n) ? *cpz.at(j) : (R)0;
+
+    // ---- pivot entering the lane: 1/u_{i0-1} ----
+    double iu_in = 0.0;
+    if (i0 >= 1 && i0 <= n - 1) {
+        if (i0 == 1) {
+            iu_in = 1.0 / b0;
+        } else {
+            // (p, q) for u_1 = b - c0/b0, then G^(i0-2)
+            double p = eig * b0 - c0, q = b0;
+            Mat2 Q = {eig, -1.0, 1.0, 0.0};
+            const int e = i0 - 2;
+            constexpr int KB = bits_for(16 * J);
+#pragma unroll
+            for (int k = 0; k < KB; k++) {
+                if ((e >> k) & 1) {
+                    const double np = Q.a * p + Q.b * q, nq = Q.c * p + Q.d * q;
+                    const int sc = -__builtin_amdgcn_frexp_exp(fmax(fabs(np), fabs(nq)));
+                    p = pow2_scale(np, sc);
+                    q = pow2_scale(nq, sc);
+                }
+                if (k + 1 < KB) Q = mat_sq_norm(Q);
+            }
+            iu_in = q / p;
+        }
+    }
+
+    // row coefficients a_i (sub), b_i (diag), c_i (super) of global row i = i0 + j.
+    // FULL: row 0 is (lane 0, j = 0) and row n-1 is (lane 15, j = J-1).
+    const bool first = (t == 0), lastl = (t == 15);
+    auto is_row0 = [&](int j) -> bool { return FULL ? (first && j == 0) : (i0 + j == 0); };
+    auto is_rowN = [&](int j) -> bool { return FULL ? (lastl && j == J - 1) : (i0 + j == n - 1); };
+    auto is_pad = [&](int j) -> bool { return FULL ? false : (i0 + j >= n); };
+    auto ai = [&](int j) -> R { return (is_row0(j) || is_pad(j)) ? (R)0 : (R)1; };
+    auto bi = [&](int j) -> R {
+        if (is_row0(j)) return (R)b0;
+        if (is_rowN(j)) return (R)bn;
+        if (is_pad(j)) return (R)1;
+        return (R)eig;
+    };
+    auto ci = [&](int j) -> R {
+        if (is_row0(j)) return (R)c0;
+        if (is_rowN(j) || is_pad(j)) return (R)0;
+        return (R)1;
+    };
+    // c_{i-1} for the row above global row i0 + j (j = 0: the previous lane's last row)
+    auto cprev = [&](int j) -> R {
+        if (j > 0) return ci(j - 1);
+        if (i0 == 0) return (R)0;
+        if (i0 == 1) return (R)c0;
+        return (FULL || i0 - 1 < n - 1) ? (R)1 : (R)0;
+    };
+
+    // Every pass after a scan is a true recurrence on the scanned boundary value, so no
+    // scan-independent per-row product is left for the compiler to hoist: only d[] and
+    // iu[] (1/u) stay live across passes.
+
+    // ---- pass A: pivots 1/u_i, and the lane's forward map y_end = yh + piA * y_{i0-1} ----
+    R iu[J];
+    R yh = (R)0, piA = (R)1;
+    {
+        R iup = (R)iu_in;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const R m = ai(j) * iup;
+            const R u = bi(j) - m * cprev(j);
+            const R iuj = rcp_nr(u);
+            yh = d[j] - m * yh;
+            piA = piA * (-m);
+            iu[j] = iuj;
+            iup = iuj;
+        }
+    }
+
+    // ---- forward scan over lanes: y_end(t) = B + A * y_end(t-1) ----
+    R A = piA, B = yh;
+    {
+#define PP_FWD_STEP(S)                          \
+    {                                           \
+        const R Al = from_lower<S>(A);          \
+        const R Bl = from_lower<S>(B);          \
+        if (t >= S) {                           \
+            B = B + A * Bl;                     \
+            A = A * Al;                         \
+        }                                       \
+    }
+        PP_FWD_STEP(1) PP_FWD_STEP(2) PP_FWD_STEP(4) PP_FWD_STEP(8)
+#undef PP_FWD_STEP
+    }
+    const R yin_r = from_lower<1>(B);
+    const R y_in = (t >= 1) ? yin_r : (R)0;
+
+    // ---- pass B: forward sweep y_i = d_i - m_i y_{i-1} from the exact y_{i0-1} ----
+    {
+        R yprev = y_in, iup = (R)iu_in;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const R m = ai(j) * iup;
+            const R y = d[j] - m * yprev;
+            d[j] = y;
+            yprev = y;
+            iup = iu[j];
+        }
+    }
+
+    // ---- pass B2: the lane's backward map x_start = xh + sig * x_{i0+J} ----
+    R xh = (R)0, sig = (R)1;
+#pragma unroll
+    for (int j = J - 1; j >= 0; j--) {
+        const R cc = ci(j);
+        xh = (d[j] - cc * xh) * iu[j];
+        sig = sig * (-cc * iu[j]);
+    }
+
+    // ---- backward scan over lanes: x_start(t) = B + A * x_start(t+1) ----
+    A = sig;
+    B = xh;
+    {
+#define PP_BWD_STEP(S)                          \
+    {                                           \
+        const R Au = from_upper<S>(A);          \
+        const R Bu = from_upper<S>(B);          \
+        if (t + S < 16) {                       \
+            B = B + A * Bu;                     \
+            A = A * Au;                         \
+        }                                       \
+    }
+        PP_BWD_STEP(1) PP_BWD_STEP(2) PP_BWD_STEP(4) PP_BWD_STEP(8)
+#undef PP_BWD_STEP
+    }
+    const R xout_r = from_upper<1>(B);
+    const R x_out = (t < 15) ? xout_r : (R)0;
+
+    // ---- pass C: backward sweep x_i = (y_i - c_i x_{i+1}) / u_i from the exact x_{i0+J} ----
+    {
+        R xn = x_out;
+#pragma unroll
+        for (int j = J - 1; j >= 0; j--) {
+            const R x = (d[j] - ci(j) * xn) * iu[j];
+            d[j] = x;
+            xn = x;
+        }
+    }
+
+    if (valid) {
+#pragma unroll
+        for (int j = 0; j < J; j++)
+            if (FULL || i0 + j < n) *cpz.at(j) = d[j];
+    }
 }
 
 }  // namespace pp

@@ -31,18 +31,36 @@ def test_selftest_lane_rotation():
     assert pa.selftest() == 0
 
 
+@pytest.mark.parametrize("zsolver", [0, 1], ids=["pcr", "scan"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("bc", COMBOS)
-def test_parity_fp64(bc, shape):
-    print(f"parity {bc} {shape}", flush=True)
-    rng = np.random.default_rng(hash((bc, shape)) % 2**32)
+def test_parity_fp64(bc, shape, zsolver):
+    print(f"parity {bc} {shape} zsolver={zsolver}", flush=True)
+    rng = np.random.default_rng(sum(map(ord, bc)) * 1000 + shape[0] * 37 + shape[2])
     f = rng.uniform(-1, 1, shape)
     box = [0, 1, 0, 1, 0, 1]
     for zero_mode in (0, 1):
         ref = oracle.solve(f, bc, box, zero_mode=zero_mode)
-        got = pa.solve(f, bc, box, zero_mode=zero_mode)
+        got = pa.solve(f, bc, box, zero_mode=zero_mode, zsolver=zsolver)
         err = rel_maxdiff(got, ref)
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
+
+
+@pytest.mark.parametrize("zsolver", [0, 1], ids=["pcr", "scan"])
+@pytest.mark.parametrize("nz", [4, 5, 15, 16, 17, 31, 33, 48, 49, 64, 100, 128, 255, 256, 500, 511, 512])
+def test_parity_z_lengths(nz, zsolver):
+    """every z length class: J = 1..64 rows per lane, padded and full columns; uniform
+    spacing dx = dy = dz as in the benchmark grids"""
+    print(f"z length {nz} zsolver={zsolver}", flush=True)
+    rng = np.random.default_rng(nz)
+    shape = (nz, 16, 16)
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]
+    f = rng.uniform(-1, 1, shape)
+    for bc in ("NNNNDD", "PPPPNN", "DDDDDN"):
+        ref = oracle.solve(f, bc, box, zero_mode=1)
+        got = pa.solve(f, bc, box, zero_mode=1, zsolver=zsolver)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"{bc} nz={nz}: rel max diff {err:.3e}"
 
 
 @pytest.mark.parametrize("bc", ["PPDDNN", "PPNNDD", "PPDNDD"])
@@ -101,4 +119,7 @@ def test_call_order_and_errors():
         assert e.value.code == pa.UNSUPPORTED_BC
     with pytest.raises(pa.PittPackError) as e:
         pa.PoissonHIP(30, 32, 16)
+    assert e.value.code == pa.UNSUPPORTED_SIZE
+    with pytest.raises(pa.PittPackError) as e:
+        pa.PoissonHIP(32, 32, 513)
     assert e.value.code == pa.UNSUPPORTED_SIZE

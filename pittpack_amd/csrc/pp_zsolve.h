@@ -183,16 +183,70 @@ __device__ __forceinline__ void pcr_step(R *d, Row3<R> *ex, R &ta, R &tb, int t)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// LDS staging of a block's tile: 16 adjacent columns x all rows.  Global traffic moves in
+// full 128-byte rows (16 columns x 8 B, four rows per wave instruction); the lanes then
+// read their own rows from LDS.  Row stride 17 elements (odd) spreads a lane-strided
+// column read over the banks.  `lrow(r)` maps tile row r to its LDS row.  CPB columns per
+// block (16: 128-byte rows; 8: 64-byte rows, half the LDS per block).
+// ---------------------------------------------------------------------------------------
+
+template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
+__device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow lrow)
+{
+    const int tid = threadIdx.x;
+    const int c = tid % CPB, g = tid / CPB;
+    const int col0 = blockIdx.x * CPB;
+    const bool cvalid = col0 + c < a.SX;
+    const long zs = (long)a.SX * a.SY;
+    const long lane_elem = (long)blockIdx.y * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
+    const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
+#pragma unroll
+    for (int q = 0; q < J; q++) {
+        const int r = g + 16 * q;
+        R v = (R)0;
+        if (cvalid && (FULL || r < n)) v = *p.at(q);
+        tile[lrow(r) * (CPB + 1) + c] = v;
+    }
+}
+
+template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
+__device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n, LRow lrow)
+{
+    const int tid = threadIdx.x;
+    const int c = tid % CPB, g = tid / CPB;
+    const int col0 = blockIdx.x * CPB;
+    const bool cvalid = col0 + c < a.SX;
+    const long zs = (long)a.SX * a.SY;
+    const long lane_elem = (long)blockIdx.y * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
+    const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
+#pragma unroll
+    for (int q = 0; q < J; q++) {
+        const int r = g + 16 * q;
+        if (cvalid && (FULL || r < n)) *p.at(q) = tile[lrow(r) * (CPB + 1) + c];
+    }
+}
+
+// LDS rows of the two layouts: cyclic (lane t owns rows t + 16 j) keeps tile order;
+// blocked (lane t owns rows tJ .. tJ+J-1) gives each lane block an odd row stride LJ.
+template <int J>
+constexpr int blocked_lj() { return (J % 2 == 1) ? J : J + 1; }
+
+template <typename R, int J, int CPB>
+constexpr size_t ztile_bytes_cyclic() { return (size_t)16 * J * (CPB + 1) * sizeof(R); }
+template <typename R, int J, int CPB>
+constexpr size_t ztile_bytes_blocked() { return (size_t)16 * blocked_lj<J>() * (CPB + 1) * sizeof(R); }
+
 // FULL: nz == 16 * J (no padding rows, no per-row bounds checks)
-template <typename R, int J, bool FULL, bool OFF32>
-__global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
+template <typename R, int J, bool FULL, bool OFF32, int CPB = 16>
+__global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a)
 {
     constexpr bool ALL = (J <= 3);
     constexpr int NEX = ALL ? J : 3;
     const int tid = threadIdx.x;
     const int t = tid & 15;
     const int c = tid >> 4;
-    int sx = blockIdx.x * 16 + c;
+    int sx = blockIdx.x * CPB + c;
     const int sy = blockIdx.y;
     const bool valid = sx < a.SX;
     if (!valid) sx = a.SX - 1;
@@ -216,13 +270,14 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
         return {(R)1, (R)eig, (R)1};
     };
 
+    extern __shared__ __align__(16) unsigned char zsm[];
+    R *tile = reinterpret_cast<R *>(zsm);
+    auto lrow = [](int r) { return r; };
+    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    __syncthreads();
     R d[J];
-    const ColPtr<R, OFF32> cp16(data, 16 * zs, col + (long)t * zs);
 #pragma unroll
-    for (int j = 0; j < J; j++) {
-        const int i = t + 16 * j;
-        d[j] = (FULL || i < n) ? *cp16.at(j) : (R)0;
-    }
+    for (int j = 0; j < J; j++) d[j] = tile[(t + 16 * j) * (CPB + 1) + c];
     Row3<R> ex[NEX];
     if constexpr (ALL) {
 #pragma unroll
@@ -266,13 +321,12 @@ __global__ void __launch_bounds__(256) k_zsolve(ZArgs a)
 #pragma unroll
     for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
 
-    if (valid) {
 #pragma unroll
-        for (int j = 0; j < J; j++) {
-            const int i = t + 16 * j;
-            if (FULL || i < n) *cp16.at(j) = d[j];
-        }
-    }
+    for (int j = 0; j < J; j++) tile[(t + 16 * j) * (CPB + 1) + c] = d[j];
+    __syncthreads();
+    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    (void)valid;
+    (void)data;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -314,13 +368,13 @@ __device__ __forceinline__ R from_lower(R x) { return rot_from_lower<S>(x); }
 template <int S, typename R>
 __device__ __forceinline__ R from_upper(R x) { return rot_from_upper<S>(x); }
 
-template <typename R, int J, bool FULL, bool OFF32>
-__global__ void __launch_bounds__(256) k_zsolve_scan(ZArgs a)
+template <typename R, int J, bool FULL, bool OFF32, int CPB = 16>
+__global__ void __launch_bounds__(16 * CPB) k_zsolve_scan(ZArgs a)
 {
     const int tid = threadIdx.x;
     const int t = tid & 15;
     const int cidx = tid >> 4;
-    int sx = blockIdx.x * 16 + cidx;
+    int sx = blockIdx.x * CPB + cidx;
     const int sy = blockIdx.y;
     const bool valid = sx < a.SX;
     if (!valid) sx = a.SX - 1;
@@ -337,10 +391,15 @@ __global__ void __launch_bounds__(256) k_zsolve_scan(ZArgs a)
     const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
 
     const int i0 = t * J;  // first row of this lane
+    extern __shared__ __align__(16) unsigned char zsm[];
+    R *tile = reinterpret_cast<R *>(zsm);
+    constexpr int LJ = blocked_lj<J>();
+    auto lrow = [](int r) { return (r / J) * LJ + (r % J); };
+    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    __syncthreads();
     R d[J];
-    const ColPtr<R, OFF32> cpz(data, zs, col + (long)i0 * zs);
 #pragma unroll
-    for (int j = 0; j < J; j++) d[j] = (FULL || i0 + j < n) ? *cpz.at(j) : (R)0;
+    for (int j = 0; j < J; j++) d[j] = tile[(t * LJ + j) * (CPB + 1) + cidx];
 
     // ---- pivot entering the lane: 1/u_{i0-1} ----
     double iu_in = 0.0;
@@ -484,11 +543,13 @@ __global__ void __launch_bounds__(256) k_zsolve_scan(ZArgs a)
         }
     }
 
-    if (valid) {
 #pragma unroll
-        for (int j = 0; j < J; j++)
-            if (FULL || i0 + j < n) *cpz.at(j) = d[j];
-    }
+    for (int j = 0; j < J; j++) tile[(t * LJ + j) * (CPB + 1) + cidx] = d[j];
+    __syncthreads();
+    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    (void)valid;
+    (void)data;
+    (void)col;
 }
 
 }  // namespace pp

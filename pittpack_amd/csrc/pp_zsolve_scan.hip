@@ -1,17 +1,54 @@
 // pp_zsolve_scan.hip -- launches of the partitioned Thomas z solve (k_zsolve_scan).  Any
 // padding below the last row is exact for this solver, so J is rounded up to a compiled
 // value.
+#include <cstdlib>
+
 #include "pp_zsolve.h"
 
 namespace pp {
 
+// columns per block: 8 (64-byte rows, 4 blocks per CU) or 16 (128-byte rows);
+// PP_ZCPB in the environment overrides for measurement
+static int zcpb()
+{
+    static int v = [] {
+        const char *e = getenv("PP_ZCPB");
+        int x = e ? atoi(e) : 16;
+        return (x == 8) ? 8 : 16;
+    }();
+    return v;
+}
+
+template <typename R, int J, int CPB>
+static hipError_t launch_scan_c(const ZArgs &a, hipStream_t s);
+
 template <typename R, int J>
 static hipError_t launch_scan(const ZArgs &a, hipStream_t s)
 {
-    dim3 grid((unsigned)((a.SX + 15) / 16), (unsigned)a.SY);
+    if (zcpb() == 8) return launch_scan_c<R, J, 8>(a, s);
+    return launch_scan_c<R, J, 16>(a, s);
+}
+
+template <typename R, int J, int CPB>
+static hipError_t launch_scan_c(const ZArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
     const bool full = (a.nz == 16 * J);
     const bool off32 = (double)a.SX * a.SY * a.nz * sizeof(R) < 4294967296.0;
-#define PP_ZL(F, O) hipLaunchKernelGGL((k_zsolve_scan<R, J, F, O>), grid, dim3(256), 0, s, a)
+    constexpr size_t lds = ztile_bytes_blocked<R, J, CPB>();
+#define PP_ZL(F, O)                                                                                            \
+    do {                                                                                                       \
+        auto kern = k_zsolve_scan<R, J, F, O, CPB>;                                                                            \
+        if (lds > 64 * 1024) {                                                                                 \
+            static bool attr = false;                                                                          \
+            if (!attr) {                                                                                       \
+                hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+                if (e != hipSuccess) return e;                                                                 \
+                attr = true;                                                                                   \
+            }                                                                                                  \
+        }                                                                                                      \
+        hipLaunchKernelGGL(kern, grid, dim3(16 * CPB), lds, s, a);                                                  \
+    } while (0)
     if (full) { if (off32) PP_ZL(true, true); else PP_ZL(true, false); }
     else { if (off32) PP_ZL(false, true); else PP_ZL(false, false); }
 #undef PP_ZL

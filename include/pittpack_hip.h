@@ -47,7 +47,14 @@ enum {
     PP_OPT_TIMING = 2,
     /* z solver: 0 = PCR across 16 lanes then Thomas per lane (cyclic rows);
      *           1 = partitioned Thomas, contiguous rows per lane, cross-lane scans */
-    PP_OPT_ZSOLVER = 3
+    PP_OPT_ZSOLVER = 3,
+    /* 0 (default): FP64 throughout.  1: mixed form -- FP32 transforms and z solves with
+     * FP64 twiddles and eigenvalues, FP64 solves of the near-singular columns
+     * (|eig+2| < tau), the pinned (0,0) column exactly in FP64, then PP_OPT_REFINE steps of
+     * FP64-residual refinement.  Columns up to 1024 rows (FP64: 512). */
+    PP_OPT_PRECISION = 4,
+    PP_OPT_REFINE = 5, /* refinement steps of the mixed form (default 2) */
+    PP_OPT_TAU = 6     /* pp_set_option_double: near-singular threshold (default 2e-3) */
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
@@ -100,10 +107,16 @@ int pp_set_stream(pp_handle h, void *stream);
 
 int pp_set_option(pp_handle h, int key, long long value);
 
-/* Mean durations (ms) of the five stages over the solves recorded since the last call:
- * x forward, y forward, z solve, y inverse, x inverse.  Requires PP_OPT_TIMING.
- * Synchronises the handle's stream. */
+int pp_set_option_double(pp_handle h, int key, double value);
+
+/* Mean durations (ms) of the five stages of the FP64 form over the solves recorded since
+ * the last call: x forward, y forward, z solve, y inverse, x inverse.  Requires
+ * PP_OPT_TIMING.  Synchronises the handle's stream. */
 int pp_get_stage_times(pp_handle h, float *ms5);
+
+/* Mean durations (ms) between consecutive kernel launches of the recorded solves (any
+ * form), in launch order; *count receives the number of intervals.  Resets the record. */
+int pp_get_kernel_times(pp_handle h, float *ms, int max, int *count);
 
 /* Device pointer of the handle's field (nx*ny*nz doubles). */
 int pp_get_device_field(pp_handle h, double **d_field);

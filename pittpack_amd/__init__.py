@@ -31,6 +31,9 @@ NO_DEVICE = 29
 OPT_ZERO_MODE = 1
 OPT_TIMING = 2
 OPT_ZSOLVER = 3
+OPT_PRECISION = 4
+OPT_REFINE = 5
+OPT_TAU = 6
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -38,7 +41,7 @@ EXPORTED = (
     "pp_assign_rhs", "pp_assign_rhs_device", "pp_pittpack", "pp_get_solution",
     "pp_get_solution_device", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
     "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
-    "pp_selftest",
+    "pp_selftest", "pp_set_option_double", "pp_get_kernel_times",
 )
 
 
@@ -77,6 +80,8 @@ def load_library(path: str | None = None):
     L.pp_set_stream.argtypes = [vp, vp]
     L.pp_set_option.argtypes = [vp, i, ctypes.c_longlong]
     L.pp_get_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.pp_set_option_double.argtypes = [vp, i, ctypes.c_double]
+    L.pp_get_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, ctypes.POINTER(i)]
     L.pp_get_device_field.argtypes = [vp, ctypes.POINTER(vp)]
     L.pp_get_error_enum.argtypes = [i]
     L.pp_get_error_enum.restype = ctypes.c_char_p
@@ -163,6 +168,9 @@ class PoissonHIP:
     def set_option(self, key: int, value: int) -> None:
         _check(self._L.pp_set_option(self._h, int(key), int(value)), "set_option")
 
+    def set_option_double(self, key: int, value: float) -> None:
+        _check(self._L.pp_set_option_double(self._h, int(key), float(value)), "set_option_double")
+
     def set_stream(self, stream_ptr: int) -> None:
         _check(self._L.pp_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "set_stream")
 
@@ -179,6 +187,13 @@ class PoissonHIP:
         ms = (ctypes.c_float * 5)()
         _check(self._L.pp_get_stage_times(self._h, ms), "stage_times")
         return list(ms)
+
+    def kernel_times(self):
+        """Mean ms between consecutive launch marks of the recorded solves, in launch order."""
+        ms = (ctypes.c_float * 64)()
+        n = ctypes.c_int(0)
+        _check(self._L.pp_get_kernel_times(self._h, ms, 64, ctypes.byref(n)), "kernel_times")
+        return list(ms)[: n.value]
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value is not None:
@@ -198,8 +213,9 @@ class PoissonHIP:
         self.close()
 
 
-def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None) -> np.ndarray:
-    """One-shot FP64 solve of an (nz, ny, nx) RHS."""
+def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None,
+          precision: int = 0, refine: int | None = None, tau: float | None = None) -> np.ndarray:
+    """One-shot solve of an (nz, ny, nx) FP64 RHS (precision 0: FP64; 1: mixed + refinement)."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     with PoissonHIP(nx, ny, nz) as M:
@@ -207,6 +223,12 @@ def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | N
             M.set_option(OPT_ZERO_MODE, zero_mode)
         if zsolver is not None:
             M.set_option(OPT_ZSOLVER, zsolver)
+        if precision:
+            M.set_option(OPT_PRECISION, precision)
+        if refine is not None:
+            M.set_option(OPT_REFINE, refine)
+        if tau is not None:
+            M.set_option_double(OPT_TAU, tau)
         M.assignBoundary(bc)
         M.setBox(box if box is not None else [0, 1, 0, 1, 0, 1])
         M.setCoords(2)

@@ -46,6 +46,12 @@ struct LineArgs {
     const double *dtab;
     int dmask;
     int nx, ny, nz;
+    // x forward only: FP64 sum over each line of the (scaled, Dirichlet-corrected) input,
+    // line_sums[y + ny*z] (null: not wanted)
+    double *line_sums;
+    // x inverse only: out = (accumulate ? out : 0) + out_scale * value + plane_add[z]
+    const double *plane_add;
+    int accumulate;
 };
 
 struct c64 {
@@ -157,6 +163,7 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
     }
 
     // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
+    double lsum = 0.0;
 #pragma unroll
     for (int c = 0; c < 2 * E; c++) {
         const int n = t + T * c;
@@ -177,10 +184,20 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
         } else {
             v = v * a.in_scale;
         }
+        lsum += v;
         bool neg;
         const int pos = load_pos<N, KIND, FWD>(n, neg);
         lr(L, pos) = neg ? (R)(-v) : (R)v;
     }
+    if constexpr (AX == 0 && FWD) {
+        if (a.line_sums) {
+            // the T threads of a line are adjacent lanes of one wave: butterfly reduction
+#pragma unroll
+            for (int off = T / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, T);
+            if (t == 0 && valid) a.line_sums[(long)blockIdx.x * LINES + l] = lsum;
+        }
+    }
+    (void)lsum;
     __syncthreads();
 
     // ---- pre-processing: FFT input in registers, layout k = t + T*m ----
@@ -343,6 +360,10 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
             double val = (double)lr(L, pos);
             if (neg) val = -val;
             val = a.out_scale * val;
+            if constexpr (AX == 0 && !FWD) {
+                if (a.plane_add) val += a.plane_add[lz];
+                if (a.accumulate) val += (double)out[base + (long)n * stride];
+            }
             out[base + (long)n * stride] = (TOut)val;
         }
     }
@@ -387,13 +408,13 @@ constexpr int line_lines()
     return lines;
 }
 
-template <typename R, int M, int AX, int FWD, int KIND>
+template <typename TIn, typename TOut, typename R, int M, int AX, int FWD, int KIND>
 hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
 {
     constexpr int T = line_threads(M, AX);
     constexpr int LINES = line_lines<R, M, AX>();
     const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>);
-    auto kern = k_line<double, double, R, M, T, LINES, AX, FWD, KIND>;
+    auto kern = k_line<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>;
     if (lds > 64 * 1024) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -410,33 +431,48 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <typename R, int AX, int FWD, int KIND>
+template <typename TIn, typename TOut, typename R, int AX, int FWD, int KIND>
 hipError_t launch_line_sz(int N, const LineArgs &a, hipStream_t s)
 {
     switch (N / 2) {
-    case 8: return launch_line_cfg<R, 8, AX, FWD, KIND>(a, s);
-    case 12: return launch_line_cfg<R, 12, AX, FWD, KIND>(a, s);
-    case 16: return launch_line_cfg<R, 16, AX, FWD, KIND>(a, s);
-    case 24: return launch_line_cfg<R, 24, AX, FWD, KIND>(a, s);
-    case 32: return launch_line_cfg<R, 32, AX, FWD, KIND>(a, s);
-    case 48: return launch_line_cfg<R, 48, AX, FWD, KIND>(a, s);
-    case 64: return launch_line_cfg<R, 64, AX, FWD, KIND>(a, s);
-    case 96: return launch_line_cfg<R, 96, AX, FWD, KIND>(a, s);
-    case 128: return launch_line_cfg<R, 128, AX, FWD, KIND>(a, s);
-    case 192: return launch_line_cfg<R, 192, AX, FWD, KIND>(a, s);
-    case 256: return launch_line_cfg<R, 256, AX, FWD, KIND>(a, s);
-    case 384: return launch_line_cfg<R, 384, AX, FWD, KIND>(a, s);
-    case 512: return launch_line_cfg<R, 512, AX, FWD, KIND>(a, s);
+#define PPM(m) \
+    case m: return launch_line_cfg<TIn, TOut, R, m, AX, FWD, KIND>(a, s);
+        PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256)
+        PPM(384) PPM(512)
+#undef PPM
     default: return hipErrorInvalidValue;
     }
 }
 
-// one instantiation per (R, KIND) lives in pp_line_<kind>.hip
-template <typename R, int KIND>
-hipError_t launch_line_kind(int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
+// Precision forms of one transform kind (instantiated once per kind in pp_line_<kind>.hip):
+//   F64        : FP64 in/out/compute, every axis and direction
+//   F32        : FP32 in/out/compute, every axis and direction
+//   F64_TO_F32 : x forward reading the FP64 right-hand side, FP32 compute and output
+//   F32_TO_F64 : x inverse writing (or accumulating into) the FP64 solution
+enum LineForm : int { FORM_F64 = 0, FORM_F32 = 1, FORM_F64_TO_F32 = 2, FORM_F32_TO_F64 = 3 };
+
+template <int KIND>
+hipError_t launch_line_kind(int form, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
 {
-    if (AX == 0) return FWD ? launch_line_sz<R, 0, 1, KIND>(N, a, s) : launch_line_sz<R, 0, 0, KIND>(N, a, s);
-    return FWD ? launch_line_sz<R, 1, 1, KIND>(N, a, s) : launch_line_sz<R, 1, 0, KIND>(N, a, s);
+    switch (form) {
+    case FORM_F64:
+        if (AX == 0) return FWD ? launch_line_sz<double, double, double, 0, 1, KIND>(N, a, s)
+                                : launch_line_sz<double, double, double, 0, 0, KIND>(N, a, s);
+        return FWD ? launch_line_sz<double, double, double, 1, 1, KIND>(N, a, s)
+                   : launch_line_sz<double, double, double, 1, 0, KIND>(N, a, s);
+    case FORM_F32:
+        if (AX == 0) return FWD ? launch_line_sz<float, float, float, 0, 1, KIND>(N, a, s)
+                                : launch_line_sz<float, float, float, 0, 0, KIND>(N, a, s);
+        return FWD ? launch_line_sz<float, float, float, 1, 1, KIND>(N, a, s)
+                   : launch_line_sz<float, float, float, 1, 0, KIND>(N, a, s);
+    case FORM_F64_TO_F32:
+        if (AX == 0 && FWD) return launch_line_sz<double, float, float, 0, 1, KIND>(N, a, s);
+        return hipErrorInvalidValue;
+    case FORM_F32_TO_F64:
+        if (AX == 0 && !FWD) return launch_line_sz<float, double, float, 0, 0, KIND>(N, a, s);
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
 }
 
 inline bool line_size_supported(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }

@@ -1,7 +1,7 @@
 // pp_line_dn.hip -- instantiates the DN line transforms (all compiled sizes, x and y axes,
-// forward and inverse).  One translation unit per transform kind so they build in parallel.
+// forward and inverse, FP64 and FP32 forms).  One translation unit per transform kind so they build in parallel.
 #include "pp_line.h"
 
 namespace pp {
-template hipError_t launch_line_kind<double, KIND_DN>(int, int, int, const LineArgs &, hipStream_t);
+template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
 }  // namespace pp

@@ -1,0 +1,208 @@
+// pp_mixed.hip -- kernels of the mixed-precision form: FP32 transforms and z solves, with
+// FP64 where FP32 cannot deliver FP64 accuracy after refinement.
+//
+//  * k_zfix64    : FP64 Thomas for the near-singular (kx, ky) columns (|eig+2| < tau); the
+//                  FP32 z solve leaves those columns untouched.  Same algebra as
+//                  thomasLowMem without the pinning branch (src/triDiag.cpp:1693-1822).
+//  * k_mode00    : exact FP64 path of the pinned (0,0) column (x and y periodic or Neumann,
+//                  z Neumann-Neumann, or the reference row-0 rule): the column's right-hand
+//                  side is the xy-sum of the effective RHS per plane, the pinned system
+//                  (row 0: u0 = r0, src/triDiag.cpp:1629-1687) is solved in FP64, and its
+//                  physical contribution -- constant over each plane -- is added by the final
+//                  inverse-x store.
+//  * k_residual  : r = (rhs*dz^2 - Dirichlet term) - L u, FP64, for the iterative
+//                  refinement; L is the dz^2-scaled 7-point operator with the ghost-cell
+//                  closures the spectral solve diagonalises (P wrap, N mirror, D antisym).
+#include <hip/hip_runtime.h>
+
+#include "pp_line.h"
+#include "pp_mixed.h"
+#include "pp_zsolve.h"
+
+namespace pp {
+
+struct Row3d {
+    double a, b, c;
+};
+
+// non-pinned column coefficients of global row i
+__device__ __forceinline__ Row3d zrow(int i, int n, double eig, double b0, double bn)
+{
+    if (i == 0) return {0.0, b0, 1.0};
+    if (i == n - 1) return {1.0, bn, 0.0};
+    return {1.0, eig, 1.0};
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_zfix64(ZArgs a, const int2 *cols)
+{
+    extern __shared__ double zf[];
+    double *col = zf;           // nz values
+    double *cp = zf + a.nz;     // nz values
+    const int2 sc = cols[blockIdx.x];
+    const long zs = (long)a.SX * a.SY;
+    const long off = (long)sc.y * a.SX + sc.x;
+    R *data = reinterpret_cast<R *>(a.data);
+    const int n = a.nz;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = (double)data[off + (long)i * zs];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sc.x], a.Cx)), __dmul_rn(a.lamy[sc.y], a.Cy));
+        const double b0 = a.zlo_dirichlet ? eig - 1. : eig + 1.;
+        const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
+        Row3d r = zrow(0, n, eig, b0, bn);
+        double inv = 1.0 / r.b;
+        cp[0] = r.c * inv;
+        col[0] = col[0] * inv;
+        for (int i = 1; i < n; i++) {
+            r = zrow(i, n, eig, b0, bn);
+            inv = 1.0 / (r.b - r.a * cp[i - 1]);
+            cp[i] = r.c * inv;
+            col[i] = (col[i] - r.a * col[i - 1]) * inv;
+        }
+        for (int i = n - 2; i >= 0; i--) col[i] = col[i] - cp[i] * col[i + 1];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) data[off + (long)i * zs] = (R)col[i];
+}
+
+// One block, FP64, for the pinned (0,0) column.  R(z) = sum_y line_sums[y + ny z] is the
+// column's right-hand side (plane sums of the forward-x input).
+//  initial (correct == 0): solve P u = R with P the pinned operator (row 0: u0 = r0,
+//      interior 1 -2 1, last row bn); plane_add[z] = u(z)/(NxNy);
+//      plane_add[nz] = (b0 u0 + u1 - S0)/(NxNy), the row-0 defect of the unpinned operator,
+//      and plane_add[nz+1] = S0 = R(0).
+//  correction (correct == 1): the refinement residual's plane sums give rows z >= 1 of
+//      P du = R; row 0 is S0 - (plane-0 sum of the current u), assembled from the residual
+//      kernel's per-block partial sums upart[0..nparts).  plane_add[z] = du(z)/(NxNy).
+__global__ void __launch_bounds__(1024) k_mode00(const double *line_sums, int ny, int nz, double b0, double bn,
+                                                 double inv_scale, double *plane_add, const double *upart, int nparts,
+                                                 int correct)
+{
+    extern __shared__ double m0[];
+    double *r = m0;
+    double *cp = m0 + nz;
+    __shared__ double red[1024];
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+        double acc = 0.0;
+        for (int y = 0; y < ny; y++) acc += line_sums[(long)z * ny + y];
+        r[z] = acc;
+    }
+    double part = 0.0;
+    if (correct)
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) part += upart[i];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double s0 = correct ? plane_add[nz + 1] : r[0];
+        if (correct) r[0] = s0 - red[0];
+        cp[0] = 0.0;
+        for (int i = 1; i < nz; i++) {
+            const double b = (i == nz - 1) ? bn : -2.0;
+            const double c = (i == nz - 1) ? 0.0 : 1.0;
+            const double inv = 1.0 / (b - cp[i - 1]);
+            cp[i] = c * inv;
+            r[i] = (r[i] - r[i - 1]) * inv;
+        }
+        for (int i = nz - 2; i >= 0; i--) r[i] = r[i] - cp[i] * r[i + 1];
+        if (!correct) {
+            plane_add[nz] = (b0 * r[0] + r[1] - s0) * inv_scale;
+            plane_add[nz + 1] = s0;
+        }
+    }
+    __syncthreads();
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) plane_add[z] = r[z] * inv_scale;
+}
+
+constexpr int RES_ZC = 8;  // planes marched per thread
+
+template <typename TOut>
+__global__ void __launch_bounds__(256) k_residual(ResArgs a)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = blockIdx.z * RES_ZC;
+    const bool valid = x < a.nx && y < a.ny;
+    const long sy = a.nx, sz = (long)a.nx * a.ny;
+    const double *u = a.u;
+    if (a.upart && blockIdx.z == 0) {
+        // plane-0 sum of u, one partial per (x, y) block
+        __shared__ double red[256];
+        red[threadIdx.x] = valid ? u[x + sy * y] : 0.0;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.upart[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    }
+    if (!valid) return;
+    TOut *r = reinterpret_cast<TOut *>(a.r);
+    const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
+    const double *ef = a.dtab + a.nx + a.ny + a.nz;
+    auto ghost = [](char bc, double inside, double wrap) -> double {
+        return bc == 'P' ? wrap : (bc == 'N' ? inside : -inside);
+    };
+    auto at = [&](int xx, int yy, int zz) -> double { return u[xx + sy * yy + sz * zz]; };
+    double uc = at(x, y, z0);
+    double um = (z0 > 0) ? at(x, y, z0 - 1) : ghost(a.bc[4], uc, at(x, y, a.nz - 1));
+    for (int k = 0; k < RES_ZC; k++) {
+        const int z = z0 + k;
+        if (z >= a.nz) break;
+        const double up = (z + 1 < a.nz) ? at(x, y, z + 1) : ghost(a.bc[5], uc, at(x, y, 0));
+        const double uxm = (x > 0) ? at(x - 1, y, z) : ghost(a.bc[0], uc, at(a.nx - 1, y, z));
+        const double uxp = (x + 1 < a.nx) ? at(x + 1, y, z) : ghost(a.bc[1], uc, at(0, y, z));
+        const double uym = (y > 0) ? at(x, y - 1, z) : ghost(a.bc[2], uc, at(x, a.ny - 1, z));
+        const double uyp = (y + 1 < a.ny) ? at(x, y + 1, z) : ghost(a.bc[3], uc, at(x, 0, z));
+        const double Lu = (uxp - 2.0 * uc + uxm) * a.Cx + (uyp - 2.0 * uc + uym) * a.Cy + (up - 2.0 * uc + um);
+        // effective right-hand side, same formula and order as the forward-x load
+        double v = a.f[x + sy * y + sz * z];
+        v = v * a.c3;
+        v = v * a.c3;
+        if (a.dmask) {
+            if ((a.dmask & 16) && z == 0) v = v - 2. * ex[x] * ey[y] * ef[4];
+            if ((a.dmask & 32) && z == a.nz - 1) v = v - 2 * ex[x] * ey[y] * ef[5];
+            if ((a.dmask & 1) && x == 0) v = v - 2. * ef[0] * ey[y] * ez[z];
+            if ((a.dmask & 2) && x == a.nx - 1) v = v - 2. * ef[1] * ey[y] * ez[z];
+            if ((a.dmask & 4) && y == 0) v = v - 2. * ex[x] * ef[2] * ez[z];
+            if ((a.dmask & 8) && y == a.ny - 1) v = v - 2. * ex[x] * ef[3] * ez[z];
+        }
+        double res = v - Lu;
+        if (a.pin_defect && z == 0) res += a.pin_defect[0];
+        r[x + sy * y + sz * z] = (TOut)res;
+        um = uc;
+        uc = up;
+    }
+}
+
+hipError_t launch_zfix64_f32(const ZArgs &a, const int2 *cols, int ncols, hipStream_t s)
+{
+    if (ncols == 0) return hipSuccess;
+    const size_t lds = 2 * (size_t)a.nz * sizeof(double);
+    hipLaunchKernelGGL(k_zfix64<float>, dim3(ncols), dim3(256), lds, s, a, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
+                         double *plane_add, const double *upart, int nparts, int correct, hipStream_t s)
+{
+    const size_t lds = 2 * (size_t)nz * sizeof(double);
+    hipLaunchKernelGGL(k_mode00, dim3(1), dim3(1024), lds, s, line_sums, ny, nz, b0, bn, inv_scale, plane_add, upart,
+                       nparts, correct);
+    return hipGetLastError();
+}
+
+int residual_parts(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
+
+hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.nx + 63) / 64), (unsigned)((a.ny + 3) / 4), (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
+    hipLaunchKernelGGL(k_residual<float>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace pp

@@ -10,6 +10,7 @@
 
 #include "pittpack_hip.h"
 #include "pp_line.h"
+#include "pp_mixed.h"
 #include "pp_zsolve.h"
 
 namespace pp {
@@ -37,23 +38,22 @@ constexpr double kPi = 3.1415926535897932384;  // src/include/params.h:116
 // line transforms: one translation unit per kind (pp_line_<kind>.hip)
 }  // namespace
 namespace pp {
-extern template hipError_t launch_line_kind<double, KIND_DD>(int, int, int, const LineArgs &, hipStream_t);
-extern template hipError_t launch_line_kind<double, KIND_NN>(int, int, int, const LineArgs &, hipStream_t);
-extern template hipError_t launch_line_kind<double, KIND_PP>(int, int, int, const LineArgs &, hipStream_t);
-extern template hipError_t launch_line_kind<double, KIND_ND>(int, int, int, const LineArgs &, hipStream_t);
-extern template hipError_t launch_line_kind<double, KIND_DN>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<KIND_DD>(int, int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<KIND_NN>(int, int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<KIND_PP>(int, int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<KIND_ND>(int, int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
 }  // namespace pp
 namespace {
 
-template <typename R>
-hipError_t launch_line(int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
+hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
 {
     switch (kind) {
-    case KIND_DD: return launch_line_kind<R, KIND_DD>(N, AX, FWD, a, s);
-    case KIND_NN: return launch_line_kind<R, KIND_NN>(N, AX, FWD, a, s);
-    case KIND_PP: return launch_line_kind<R, KIND_PP>(N, AX, FWD, a, s);
-    case KIND_ND: return launch_line_kind<R, KIND_ND>(N, AX, FWD, a, s);
-    case KIND_DN: return launch_line_kind<R, KIND_DN>(N, AX, FWD, a, s);
+    case KIND_DD: return launch_line_kind<KIND_DD>(form, N, AX, FWD, a, s);
+    case KIND_NN: return launch_line_kind<KIND_NN>(form, N, AX, FWD, a, s);
+    case KIND_PP: return launch_line_kind<KIND_PP>(form, N, AX, FWD, a, s);
+    case KIND_ND: return launch_line_kind<KIND_ND>(form, N, AX, FWD, a, s);
+    case KIND_DN: return launch_line_kind<KIND_DN>(form, N, AX, FWD, a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -68,6 +68,9 @@ namespace pp {
 hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s);
 hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f64(int nz);
+hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
+bool zsolve_supported_f32(int nz);
+
 }  // namespace pp
 namespace {
 
@@ -140,12 +143,30 @@ struct pp_solver {
     bool own_stream = false;
     double *d_field = nullptr;
     double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
+    std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
     DevTables tab64[2];
     int dmask = 0;
-    // stage-timing ring: slot s holds 6 events bracketing the five stages of one solve
+    // kernel-timing ring: slot s holds up to kMaxMarks events recorded between the kernel
+    // launches of one solve
+    static constexpr int kMaxMarks = 64;
     std::vector<hipEvent_t> ev;
     int ev_slots = 0, ev_next = 0, ev_used = 0;
+    hipEvent_t *ev_cur = nullptr;
+    int marks_cur = 0, marks_last = 0;
     bool tables_ready = false;
+    // mixed precision form
+    int precision = 0;   // 0 FP64, 1 FP32 + FP64 refinement
+    int refine = 2;
+    double tau = 0.05;
+    float *d_w = nullptr;          // FP32 work field
+    double *d_aux = nullptr;       // FP64 scratch field (RHS copy or solution) for host-facing calls
+    double *d_line_sums = nullptr; // ny*nz
+    double *d_plane_add = nullptr; // nz + 2
+    double *d_upart = nullptr;     // residual kernel's plane-0 partial sums of u
+    int2 *d_fix = nullptr;         // near-singular columns solved in FP64
+    int nfix = 0;
+    bool pinned00 = false;
+    bool mixed_ready = false;
 };
 
 namespace {
@@ -155,6 +176,47 @@ namespace {
         hipError_t e_ = (call);                  \
         if (e_ != hipSuccess) return PP_HIP_ERROR; \
     } while (0)
+
+int mark(pp_solver *h)
+{
+    if (h->ev_cur && h->marks_cur < pp_solver::kMaxMarks) {
+        PP_HIP_CHECK(hipEventRecord(h->ev_cur[h->marks_cur++], h->stream));
+    }
+    return PP_SUCCESS;
+}
+
+int begin_timing(pp_solver *h)
+{
+    h->ev_cur = nullptr;
+    if (!h->timing) return PP_SUCCESS;
+    h->ev_cur = &h->ev[(size_t)pp_solver::kMaxMarks * h->ev_next];
+    h->ev_next = (h->ev_next + 1) % h->ev_slots;
+    if (h->ev_used < h->ev_slots) h->ev_used++;
+    h->marks_cur = 0;
+    return mark(h);
+}
+
+void end_timing(pp_solver *h)
+{
+    if (h->ev_cur) h->marks_last = h->marks_cur;
+    h->ev_cur = nullptr;
+}
+
+void free_mixed(pp_solver *h)
+{
+    if (h->d_w) (void)hipFree(h->d_w);
+    if (h->d_aux) (void)hipFree(h->d_aux);
+    if (h->d_line_sums) (void)hipFree(h->d_line_sums);
+    if (h->d_plane_add) (void)hipFree(h->d_plane_add);
+    if (h->d_upart) (void)hipFree(h->d_upart);
+    if (h->d_fix) (void)hipFree(h->d_fix);
+    h->d_w = nullptr;
+    h->d_aux = nullptr;
+    h->d_line_sums = h->d_plane_add = h->d_upart = nullptr;
+    h->d_fix = nullptr;
+    h->nfix = 0;
+    h->mixed_ready = false;
+}
 
 void free_tables(pp_solver *h)
 {
@@ -202,6 +264,8 @@ int build_tables(pp_solver *h)
     for (int i = 0; i < h->ny; i++) ly[i] = (-2.0 + 2. * std::cos((i + h->num[1]) * h->freqs[1] / (h->ny + h->denum[1])));
     if ((rc = upload((void **)&h->d_lamx, lx))) return rc;
     if ((rc = upload((void **)&h->d_lamy, ly))) return rc;
+    h->lamx_h = lx;
+    h->lamy_h = ly;
     // Dirichlet term tables (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835, SHIFT = 1)
     const double pi = 4. * std::atan(1.0);
     const double omega[3] = {1 * pi, 2 * pi, 3 * pi};
@@ -230,7 +294,12 @@ int build_tables(pp_solver *h)
 int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
-    if (!h->tables_ready) return build_tables(h);
+    if (h->precision == 1 && !zsolve_supported_f32(h->nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 0 && !zsolve_supported_f64(h->nz)) return PP_UNSUPPORTED_SIZE;
+    if (!h->tables_ready) {
+        h->mixed_ready = false;
+        return build_tables(h);
+    }
     return PP_SUCCESS;
 }
 
@@ -266,28 +335,23 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
 int run_fp64(pp_solver *h, const double *rhs, double *field)
 {
     hipStream_t s = h->stream;
-    hipEvent_t *ev = nullptr;
-    if (h->timing) {
-        ev = &h->ev[6 * (size_t)h->ev_next];
-        h->ev_next = (h->ev_next + 1) % h->ev_slots;
-        if (h->ev_used < h->ev_slots) h->ev_used++;
-    }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[0], s));
+    int rc = begin_timing(h);
+    if (rc) return rc;
     // 1. forward x: rhs * dz^2, Dirichlet boundary term, x transform
     {
         LineArgs a = line_args(h, 0, field);
         a.in = rhs;
         a.in_scale = h->dxyz[2];
         a.dmask = h->dmask;
-        PP_HIP_CHECK(launch_line<double>(h->kind[0], h->nx, 0, 1, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[0], h->nx, 0, 1, a, s));
     }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[1], s));
+    if ((rc = mark(h))) return rc;
     // 2. forward y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK(launch_line<double>(h->kind[1], h->ny, 1, 1, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[1], h->ny, 1, 1, a, s));
     }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[2], s));
+    if ((rc = mark(h))) return rc;
     // 3. z solve
     {
         ZArgs z{};
@@ -304,21 +368,226 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         z.zero_mode = h->zero_mode;
         PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->nz, z, s) : launch_zsolve_pcr_f64(h->nz, z, s));
     }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[3], s));
+    if ((rc = mark(h))) return rc;
     // 4. inverse y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK(launch_line<double>(h->kind[1], h->ny, 1, 0, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[1], h->ny, 1, 0, a, s));
     }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[4], s));
+    if ((rc = mark(h))) return rc;
     // 5. inverse x with the 1/(Nx*Ny) rescale fused
     {
         LineArgs a = line_args(h, 0, field);
         a.out_scale = 1. / (h->scale);
-        PP_HIP_CHECK(launch_line<double>(h->kind[0], h->nx, 0, 0, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[0], h->nx, 0, 0, a, s));
     }
-    if (ev) PP_HIP_CHECK(hipEventRecord(ev[5], s));
+    if ((rc = mark(h))) return rc;
+    end_timing(h);
     return PP_SUCCESS;
+}
+
+ZArgs z_args(pp_solver *h, void *data)
+{
+    ZArgs z{};
+    z.data = data;
+    z.SX = h->nx;
+    z.SY = h->ny;
+    z.nz = h->nz;
+    z.lamx = h->d_lamx;
+    z.lamy = h->d_lamy;
+    z.Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
+    z.Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+    z.zlo_dirichlet = h->zlo_d;
+    z.zhi_dirichlet = h->zhi_d;
+    z.zero_mode = h->zero_mode;
+    z.tau = 0.0;
+    z.pinned_zero = 0;
+    return z;
+}
+
+// Buffers and the near-singular column list of the mixed form.  The list uses the same
+// eigenvalue arithmetic as the kernels (no contraction) so both sides agree on it.
+int ensure_mixed(pp_solver *h)
+{
+    if (h->mixed_ready) return PP_SUCCESS;
+    // size-dependent buffers: allocated once per handle (sizes are fixed at pp_create) and
+    // freed only by pp_destroy, so no pointer handed to a solve can be released under it
+    const size_t N = (size_t)h->nx * h->ny * h->nz;
+    if (!h->d_w && hipMalloc((void **)&h->d_w, N * sizeof(float)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    if (!h->d_line_sums &&
+        hipMalloc((void **)&h->d_line_sums, (size_t)h->ny * h->nz * sizeof(double)) != hipSuccess)
+        return PP_ALLOCATION_FAIL;
+    if (!h->d_plane_add &&
+        hipMalloc((void **)&h->d_plane_add, (size_t)(h->nz + 2) * sizeof(double)) != hipSuccess)
+        return PP_ALLOCATION_FAIL;
+    if (!h->d_upart &&
+        hipMalloc((void **)&h->d_upart, (size_t)residual_parts(h->nx, h->ny) * sizeof(double)) != hipSuccess)
+        return PP_ALLOCATION_FAIL;
+    // the near-singular column list depends on the tables, tau and the zero-mode rule
+    if (h->d_fix) {
+        PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+        PP_HIP_CHECK(hipFree(h->d_fix));
+        h->d_fix = nullptr;
+    }
+    h->nfix = 0;
+    const double Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
+    const double Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+    const std::vector<double> &lx = h->lamx_h, &ly = h->lamy_h;
+    std::vector<int2> fix;
+    h->pinned00 = false;
+    for (int sy = 0; sy < h->ny; sy++)
+        for (int sx = 0; sx < h->nx; sx++) {
+            volatile double t1 = lx[sx] * Cx;
+            volatile double t2 = -2.0 + t1;
+            volatile double t3 = ly[sy] * Cy;
+            const double eig = t2 + t3;
+            bool pin = std::fabs(eig + 2.) < 1.e-10;
+            if (pin && h->zero_mode == 1 && (h->zlo_d || h->zhi_d)) pin = false;
+            if (pin) {
+                h->pinned00 = true;
+                continue;
+            }
+            if (std::fabs(eig + 2.) < h->tau) fix.push_back(make_int2(sx, sy));
+        }
+    h->nfix = (int)fix.size();
+    if (h->nfix) {
+        if (hipMalloc((void **)&h->d_fix, fix.size() * sizeof(int2)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_fix, fix.data(), fix.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    h->mixed_ready = true;
+    return PP_SUCCESS;
+}
+
+// one FP32 solve of the field in d_w, in place (forward x already applied by the caller
+// when `x_forward_done`)
+int mixed_inner(pp_solver *h)
+{
+    hipStream_t s = h->stream;
+    int rc;
+    {
+        LineArgs a = line_args(h, 1, nullptr);
+        a.in = h->d_w;
+        a.out = h->d_w;
+        PP_HIP_CHECK(launch_line(FORM_F32, h->kind[1], h->ny, 1, 1, a, s));
+    }
+    if ((rc = mark(h))) return rc;
+    {
+        ZArgs z = z_args(h, h->d_w);
+        z.tau = h->tau;
+        z.pinned_zero = 1;
+        PP_HIP_CHECK(launch_zsolve_pcr_f32(h->nz, z, s));
+        if ((rc = mark(h))) return rc;
+        PP_HIP_CHECK(launch_zfix64_f32(z, h->d_fix, h->nfix, s));
+    }
+    if ((rc = mark(h))) return rc;
+    {
+        LineArgs a = line_args(h, 1, nullptr);
+        a.in = h->d_w;
+        a.out = h->d_w;
+        PP_HIP_CHECK(launch_line(FORM_F32, h->kind[1], h->ny, 1, 0, a, s));
+    }
+    return mark(h);
+}
+
+// Mixed precision: FP32 transforms and z solves (FP64 twiddles, eigenvalues and
+// near-singular columns), the pinned (0,0) column exactly in FP64, then `refine` steps of
+// u += solve32(f_eff - L u) with the residual in FP64.  f and u must not alias.
+int run_mixed(pp_solver *h, const double *f, double *u)
+{
+    hipStream_t s = h->stream;
+    int rc = ensure_mixed(h);
+    if (rc) return rc;
+    if ((rc = begin_timing(h))) return rc;
+    // initial solve: x forward reads the FP64 right-hand side
+    {
+        LineArgs a = line_args(h, 0, nullptr);
+        a.in = f;
+        a.out = h->d_w;
+        a.in_scale = h->dxyz[2];
+        a.dmask = h->dmask;
+        a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
+        PP_HIP_CHECK(launch_line(FORM_F64_TO_F32, h->kind[0], h->nx, 0, 1, a, s));
+    }
+    if ((rc = mark(h))) return rc;
+    if ((rc = mixed_inner(h))) return rc;
+    if (h->pinned00) {
+        const double b0 = h->zlo_d ? -2.0 - 1. : -2.0 + 1.;
+        const double bn = h->zhi_d ? -2.0 - 1. : -2.0 + 1.;
+        PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->ny, h->nz, b0, bn, 1. / h->scale, h->d_plane_add, nullptr, 0, 0, s));
+    }
+    {
+        LineArgs a = line_args(h, 0, nullptr);
+        a.in = h->d_w;
+        a.out = u;
+        a.out_scale = 1. / (h->scale);
+        a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
+        a.accumulate = 0;
+        PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->kind[0], h->nx, 0, 0, a, s));
+    }
+    if ((rc = mark(h))) return rc;
+    const double Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
+    const double Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+    for (int it = 0; it < h->refine; it++) {
+        ResArgs r{};
+        r.f = f;
+        r.u = u;
+        r.r = h->d_w;
+        r.nx = h->nx;
+        r.ny = h->ny;
+        r.nz = h->nz;
+        r.c3 = h->dxyz[2];
+        r.Cx = Cx;
+        r.Cy = Cy;
+        for (int i = 0; i < 6; i++) r.bc[i] = h->bc[i];
+        r.dtab = h->d_dtab;
+        r.dmask = h->dmask;
+        r.pin_defect = h->pinned00 ? h->d_plane_add + h->nz : nullptr;
+        r.upart = h->pinned00 ? h->d_upart : nullptr;
+        PP_HIP_CHECK(launch_residual_f32(r, s));
+        if ((rc = mark(h))) return rc;
+        {
+            LineArgs a = line_args(h, 0, nullptr);
+            a.in = h->d_w;
+            a.out = h->d_w;
+            a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
+            PP_HIP_CHECK(launch_line(FORM_F32, h->kind[0], h->nx, 0, 1, a, s));
+        }
+        if ((rc = mark(h))) return rc;
+        if ((rc = mixed_inner(h))) return rc;
+        if (h->pinned00) {
+            // the FP32 passes leak rounding into the plane means of u, which the zeroed
+            // (0,0) column of the FP32 correction cannot see: correct that column in FP64
+            const double b0 = h->zlo_d ? -2.0 - 1. : -2.0 + 1.;
+            const double bn = h->zhi_d ? -2.0 - 1. : -2.0 + 1.;
+            PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->ny, h->nz, b0, bn, 1. / h->scale, h->d_plane_add, h->d_upart,
+                                       residual_parts(h->nx, h->ny), 1, s));
+        }
+        {
+            LineArgs a = line_args(h, 0, nullptr);
+            a.in = h->d_w;
+            a.out = u;
+            a.out_scale = 1. / (h->scale);
+            a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
+            a.accumulate = 1;
+            PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->kind[0], h->nx, 0, 0, a, s));
+        }
+        if ((rc = mark(h))) return rc;
+    }
+    end_timing(h);
+    return PP_SUCCESS;
+}
+
+// dispatch on the precision form; rhs and out may alias (the mixed form then works on a copy)
+int run_solve(pp_solver *h, const double *rhs, double *out)
+{
+    if (h->precision == 0) return run_fp64(h, rhs, out);
+    if (rhs == out) {
+        const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+        if (!h->d_aux && hipMalloc((void **)&h->d_aux, bytes) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpyAsync(h->d_aux, rhs, bytes, hipMemcpyDeviceToDevice, h->stream));
+        return run_mixed(h, h->d_aux, out);
+    }
+    return run_mixed(h, rhs, out);
 }
 
 }  // namespace
@@ -331,7 +600,7 @@ int pp_create(int nx, int ny, int nz, pp_handle *out)
     *out = nullptr;
     if (nx < 2 || ny < 2 || nz < 4) return PP_INPUT_ARGS;
     if (!size_supported(nx) || !size_supported(ny)) return PP_UNSUPPORTED_SIZE;
-    if (!zsolve_supported_f64(nz)) return PP_UNSUPPORTED_SIZE;
+    if (!zsolve_supported_f64(nz) && !zsolve_supported_f32(nz)) return PP_UNSUPPORTED_SIZE;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PP_NO_DEVICE;
     pp_solver *h = new (std::nothrow) pp_solver();
@@ -359,6 +628,7 @@ int pp_destroy(pp_handle h)
     if (!h) return PP_INVALID_HANDLE;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_tables(h);
+    free_mixed(h);
     if (h->d_field) (void)hipFree(h->d_field);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -470,7 +740,7 @@ int pp_pittpack(pp_handle h)
     if (!h->rhs_set) return PP_CALL_ORDER;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    rc = run_fp64(h, h->d_field, h->d_field);
+    rc = run_solve(h, h->d_field, h->d_field);
     if (rc) return rc;
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->rhs_set = false;  // the field now holds the solution
@@ -483,7 +753,7 @@ int pp_solve_device(pp_handle h, double *d_field)
     if (!d_field) return PP_INPUT_ARGS;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    return run_fp64(h, d_field, d_field);
+    return run_solve(h, d_field, d_field);
 }
 
 int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u)
@@ -492,7 +762,7 @@ int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u)
     if (!d_rhs || !d_u) return PP_INPUT_ARGS;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    return run_fp64(h, d_rhs, d_u);
+    return run_solve(h, d_rhs, d_u);
 }
 
 int pp_get_solution(pp_handle h, double *u)
@@ -533,17 +803,26 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_ZERO_MODE:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zero_mode = (int)value;
+        h->mixed_ready = false;
         return PP_SUCCESS;
     case PP_OPT_ZSOLVER:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zsolver = (int)value;
         return PP_SUCCESS;
+    case PP_OPT_PRECISION:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->precision = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_REFINE:
+        if (value < 0 || value > 16) return PP_INPUT_ARGS;
+        h->refine = (int)value;
+        return PP_SUCCESS;
     case PP_OPT_TIMING: {
-        if (value < 0 || value > 4096) return PP_INPUT_ARGS;
+        if (value < 0 || value > 1024) return PP_INPUT_ARGS;
         PP_HIP_CHECK(hipStreamSynchronize(h->stream));
         for (auto &e : h->ev)
             if (e) (void)hipEventDestroy(e);
-        h->ev.assign(6 * (size_t)value, nullptr);
+        h->ev.assign((size_t)pp_solver::kMaxMarks * value, nullptr);
         for (auto &e : h->ev)
             if (hipEventCreate(&e) != hipSuccess) return PP_HIP_ERROR;
         h->ev_slots = (int)value;
@@ -556,25 +835,51 @@ int pp_set_option(pp_handle h, int key, long long value)
     }
 }
 
-int pp_get_stage_times(pp_handle h, float *ms5)
+int pp_get_kernel_times(pp_handle h, float *ms, int max, int *count)
 {
     if (!h) return PP_INVALID_HANDLE;
-    if (!ms5) return PP_INPUT_ARGS;
-    if (!h->timing || h->ev_used == 0) return PP_CALL_ORDER;
+    if (!ms || !count || max < 0) return PP_INPUT_ARGS;
+    if (!h->timing || h->ev_used == 0 || h->marks_last < 2) return PP_CALL_ORDER;
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
-    double acc[5] = {0, 0, 0, 0, 0};
+    const int n = h->marks_last - 1;
+    std::vector<double> acc(n, 0.0);
     for (int slot = 0; slot < h->ev_used; slot++) {
-        hipEvent_t *ev = &h->ev[6 * (size_t)slot];
-        for (int i = 0; i < 5; i++) {
-            float ms = 0.f;
-            PP_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-            acc[i] += ms;
+        hipEvent_t *ev = &h->ev[(size_t)pp_solver::kMaxMarks * slot];
+        for (int i = 0; i < n; i++) {
+            float t = 0.f;
+            PP_HIP_CHECK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            acc[i] += t;
         }
     }
-    for (int i = 0; i < 5; i++) ms5[i] = (float)(acc[i] / h->ev_used);
+    *count = n;
+    for (int i = 0; i < n && i < max; i++) ms[i] = (float)(acc[i] / h->ev_used);
     h->ev_used = 0;
     h->ev_next = 0;
     return PP_SUCCESS;
+}
+
+int pp_get_stage_times(pp_handle h, float *ms5)
+{
+    if (!ms5) return PP_INPUT_ARGS;
+    float ms[pp_solver::kMaxMarks];
+    int n = 0;
+    int rc = pp_get_kernel_times(h, ms, pp_solver::kMaxMarks, &n);
+    if (rc) return rc;
+    if (n < 5) return PP_CALL_ORDER;
+    for (int i = 0; i < 5; i++) ms5[i] = ms[i];
+    return PP_SUCCESS;
+}
+
+int pp_set_option_double(pp_handle h, int key, double value)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (key == PP_OPT_TAU) {
+        if (!(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
+        h->tau = value;
+        h->mixed_ready = false;
+        return PP_SUCCESS;
+    }
+    return PP_INPUT_ARGS;
 }
 
 int pp_get_device_field(pp_handle h, double **d_field)

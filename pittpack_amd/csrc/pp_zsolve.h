@@ -25,6 +25,11 @@ struct ZArgs {
     double Cx, Cy;
     int zlo_dirichlet, zhi_dirichlet;  // 1 = D, 0 = N
     int zero_mode;                      // 0: reference row-0 rule; 1: only when singular (z NN)
+    // mixed precision: columns with |eig+2| < tau (and not pinned) are left untouched for the
+    // FP64 column solve; pinned_zero: the pinned (0,0) column is written as zeros (its
+    // contribution is solved exactly in FP64 elsewhere)
+    double tau;
+    int pinned_zero;
 };
 
 // Column element access.  The row stride zs is uniform, so row j of a lane sits at the
@@ -321,8 +326,12 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a)
 #pragma unroll
     for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
 
+    const bool skip = !pin && fabs(eig + 2.) < a.tau;
+    const bool zero = pin && a.pinned_zero;
+    if (!skip) {
 #pragma unroll
-    for (int j = 0; j < J; j++) tile[(t + 16 * j) * (CPB + 1) + c] = d[j];
+        for (int j = 0; j < J; j++) tile[(t + 16 * j) * (CPB + 1) + c] = zero ? (R)0 : d[j];
+    }
     __syncthreads();
     ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
     (void)valid;
@@ -543,8 +552,12 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve_scan(ZArgs a)
         }
     }
 
+    const bool skip = !pin && fabs(eig + 2.) < a.tau;
+    const bool zero = pin && a.pinned_zero;
+    if (!skip) {
 #pragma unroll
-    for (int j = 0; j < J; j++) tile[(t * LJ + j) * (CPB + 1) + cidx] = d[j];
+        for (int j = 0; j < J; j++) tile[(t * LJ + j) * (CPB + 1) + cidx] = zero ? (R)0 : d[j];
+    }
     __syncthreads();
     ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
     (void)valid;

@@ -35,7 +35,8 @@ static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
     const bool full = (a.nz == 16 * J);
-    const bool off32 = (double)a.SX * a.SY * a.nz * sizeof(R) < 4294967296.0;
+    // staged tiles address rows from a uniform base; lane offsets span 16 rows
+    if (16.0 * a.SX * a.SY * sizeof(R) >= 4294967296.0) return hipErrorInvalidValue;
     constexpr size_t lds = ztile_bytes_cyclic<R, J, CPB>();
 #define PP_ZL(F, O)                                                                                            \
     do {                                                                                                       \
@@ -50,26 +51,46 @@ static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s)
         }                                                                                                      \
         hipLaunchKernelGGL(kern, grid, dim3(16 * CPB), lds, s, a);                                                  \
     } while (0)
-    if (full) { if (off32) PP_ZL(true, true); else PP_ZL(true, false); }
-    else { if (off32) PP_ZL(false, true); else PP_ZL(false, false); }
+    if (full) PP_ZL(true, true);
+    else PP_ZL(false, true);
 #undef PP_ZL
     return hipGetLastError();
 }
 
 bool zsolve_supported_f64(int nz) { return nz >= 4 && nz <= 16 * 32; }
 
+#define PPJ32(R)                                                                                        \
+    PPJ(R, 1) PPJ(R, 2) PPJ(R, 3) PPJ(R, 4) PPJ(R, 5) PPJ(R, 6) PPJ(R, 7) PPJ(R, 8) PPJ(R, 9) PPJ(R, 10) \
+    PPJ(R, 11) PPJ(R, 12) PPJ(R, 13) PPJ(R, 14) PPJ(R, 15) PPJ(R, 16) PPJ(R, 17) PPJ(R, 18)            \
+    PPJ(R, 19) PPJ(R, 20) PPJ(R, 21) PPJ(R, 22) PPJ(R, 23) PPJ(R, 24) PPJ(R, 25) PPJ(R, 26)            \
+    PPJ(R, 27) PPJ(R, 28) PPJ(R, 29) PPJ(R, 30) PPJ(R, 31) PPJ(R, 32)
+#define PPJ(R, v) \
+    case v: return launch_pcr<R, v>(a, s);
+
 hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s)
 {
-    const int J = (nz + 15) / 16;
-    switch (J) {
-#define PPJ(v) \
-    case v: return launch_pcr<double, v>(a, s);
-        PPJ(1) PPJ(2) PPJ(3) PPJ(4) PPJ(5) PPJ(6) PPJ(7) PPJ(8) PPJ(9) PPJ(10) PPJ(11) PPJ(12)
-        PPJ(13) PPJ(14) PPJ(15) PPJ(16) PPJ(17) PPJ(18) PPJ(19) PPJ(20) PPJ(21) PPJ(22) PPJ(23)
-        PPJ(24) PPJ(25) PPJ(26) PPJ(27) PPJ(28) PPJ(29) PPJ(30) PPJ(31) PPJ(32)
-#undef PPJ
+    switch ((nz + 15) / 16) {
+        PPJ32(double)
     default: return hipErrorInvalidValue;
     }
 }
+
+bool zsolve_supported_f32(int nz) { return nz >= 4 && nz <= 16 * 64; }
+
+hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s)
+{
+    switch ((nz + 15) / 16) {
+        PPJ32(float)
+        PPJ(float, 33) PPJ(float, 34) PPJ(float, 35) PPJ(float, 36) PPJ(float, 37) PPJ(float, 38)
+        PPJ(float, 39) PPJ(float, 40) PPJ(float, 41) PPJ(float, 42) PPJ(float, 43) PPJ(float, 44)
+        PPJ(float, 45) PPJ(float, 46) PPJ(float, 47) PPJ(float, 48) PPJ(float, 49) PPJ(float, 50)
+        PPJ(float, 51) PPJ(float, 52) PPJ(float, 53) PPJ(float, 54) PPJ(float, 55) PPJ(float, 56)
+        PPJ(float, 57) PPJ(float, 58) PPJ(float, 59) PPJ(float, 60) PPJ(float, 61) PPJ(float, 62)
+        PPJ(float, 63) PPJ(float, 64)
+    default: return hipErrorInvalidValue;
+    }
+}
+#undef PPJ
+#undef PPJ32
 
 }  // namespace pp

@@ -34,7 +34,7 @@ static hipError_t launch_scan_c(const ZArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
     const bool full = (a.nz == 16 * J);
-    const bool off32 = (double)a.SX * a.SY * a.nz * sizeof(R) < 4294967296.0;
+    if (16.0 * a.SX * a.SY * sizeof(R) >= 4294967296.0) return hipErrorInvalidValue;
     constexpr size_t lds = ztile_bytes_blocked<R, J, CPB>();
 #define PP_ZL(F, O)                                                                                            \
     do {                                                                                                       \
@@ -49,8 +49,8 @@ static hipError_t launch_scan_c(const ZArgs &a, hipStream_t s)
         }                                                                                                      \
         hipLaunchKernelGGL(kern, grid, dim3(16 * CPB), lds, s, a);                                                  \
     } while (0)
-    if (full) { if (off32) PP_ZL(true, true); else PP_ZL(true, false); }
-    else { if (off32) PP_ZL(false, true); else PP_ZL(false, false); }
+    if (full) PP_ZL(true, true);
+    else PP_ZL(false, true);
 #undef PP_ZL
     return hipGetLastError();
 }

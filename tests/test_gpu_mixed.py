@@ -1,0 +1,102 @@
+"""GPU parity of the mixed-precision form against the CPU oracle (through the C ABI).
+
+The mixed form runs the transforms and z solves in FP32 (FP64 twiddles and eigenvalues),
+solves the near-singular columns (|eig+2| < tau) and the pinned (0,0) column in FP64, then
+refines with the FP64 residual of the 7-point operator.  After the default two refinement
+steps the result must meet the FP64 bar: max-norm difference <= 1e-12 relative to
+max|u_oracle|.  Without refinement it is an FP32 solve (checked at 1e-5).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pittpack_amd as pa
+from pittpack_amd import rng as prng
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+SHAPES = [(10, 16, 32), (33, 64, 16), (7, 16, 96)]
+COMBOS = ["PPPPNN", "NNNNDD", "DDDDDD", "NNNNNN", "PPPPDD", "DDNDDD", "NDDNDN", "DNNNDD"]
+
+
+def rel_maxdiff(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", COMBOS)
+def test_mixed_parity(bc, shape):
+    print(f"mixed {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=sum(map(ord, bc)) + shape[0])
+    for zero_mode in (0, 1):
+        ref = oracle.solve(f, bc, zero_mode=zero_mode)
+        got = pa.solve(f, bc, zero_mode=zero_mode, precision=1)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
+
+
+@pytest.mark.parametrize("bc", ["PPPPNN", "NNNNDD", "DDNDDD"])
+def test_mixed_refinement_contracts(bc):
+    """FP32 accuracy without refinement, FP64 accuracy after one or two steps"""
+    print(f"mixed refine {bc}", flush=True)
+    f = prng.uniform_pm1((32, 48, 64), seed=5)
+    ref = oracle.solve(f, bc, zero_mode=1)
+    errs = [rel_maxdiff(pa.solve(f, bc, precision=1, refine=r), ref) for r in (0, 1, 2)]
+    assert 1e-9 < errs[0] <= 1e-5, errs
+    assert errs[1] <= 1e-11, errs
+    assert errs[2] <= TOL, errs
+
+
+@pytest.mark.parametrize("tau", [0.0, 2e-3, 0.5])
+def test_mixed_tau(tau):
+    """the FP64 column threshold changes the speed of refinement, not its limit"""
+    print(f"mixed tau {tau}", flush=True)
+    f = prng.uniform_pm1((24, 32, 32), seed=9)
+    ref = oracle.solve(f, "NNNNDD", zero_mode=1)
+    got = pa.solve(f, "NNNNDD", precision=1, refine=3, tau=tau)
+    assert rel_maxdiff(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("nz", [600, 1000, 1024])
+def test_mixed_long_columns(nz):
+    """columns longer than the FP64 form's 512 rows"""
+    print(f"mixed nz={nz}", flush=True)
+    shape = (nz, 16, 16)
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]
+    f = prng.uniform_pm1(shape, seed=nz)
+    for bc in ("NNNNDD", "PPPPNN", "DDDDDN"):
+        ref = oracle.solve(f, bc, box, zero_mode=1)
+        got = pa.solve(f, bc, box, zero_mode=1, precision=1)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"{bc} nz={nz}: rel max diff {err:.3e}"
+
+
+def test_mixed_device_forms():
+    """in-place and out-of-place device calls on torch buffers, stage timing record"""
+    import torch
+
+    print("mixed device forms", flush=True)
+    dev = torch.device("cuda:0")
+    shape = (32, 64, 64)
+    f = prng.uniform_pm1(shape, seed=3)
+    ref = oracle.solve(f, "DDNDDD", zero_mode=1)
+    with pa.PoissonHIP(64, 64, 32) as M:
+        M.assignBoundary("DDNDDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.set_option(pa.OPT_PRECISION, 1)
+        M.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+        M.set_option(pa.OPT_TIMING, 2)
+        fd = torch.from_numpy(f).to(dev)
+        ud = torch.empty_like(fd)
+        M.solve_device_oop(fd.data_ptr(), ud.data_ptr())
+        torch.cuda.synchronize()
+        assert rel_maxdiff(ud.cpu().numpy(), ref) <= TOL
+        assert np.array_equal(fd.cpu().numpy(), f)  # right-hand side kept
+        kt = M.kernel_times()
+        assert len(kt) >= 6 and all(t > 0 for t in kt)
+        M.solve_device(fd.data_ptr())
+        torch.cuda.synchronize()
+        assert rel_maxdiff(fd.cpu().numpy(), ref) <= TOL

@@ -121,5 +121,14 @@ def test_call_order_and_errors():
         pa.PoissonHIP(30, 32, 16)
     assert e.value.code == pa.UNSUPPORTED_SIZE
     with pytest.raises(pa.PittPackError) as e:
-        pa.PoissonHIP(32, 32, 513)
+        pa.PoissonHIP(32, 32, 1025)
     assert e.value.code == pa.UNSUPPORTED_SIZE
+    # FP64 columns stop at 512 rows; the mixed form takes up to 1024
+    with pa.PoissonHIP(32, 32, 513) as M:
+        M.assignBoundary("NNNNDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.assignRhs(np.zeros((513, 32, 32)))
+        with pytest.raises(pa.PittPackError) as e:
+            M.pittPack()
+        assert e.value.code == pa.UNSUPPORTED_SIZE

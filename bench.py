@@ -6,10 +6,14 @@ Neumann-Dirichlet in y, Dirichlet-Dirichlet in z, box [0,1]^3, RHS U[-1,1] FP64 
 counter generator with seed 0.  One step = one full solve (x forward, y forward, z solve,
 y inverse, x inverse) of the resident RHS into a resident solution array.
 
+`--precision 1` times the mixed form instead (FP32 transforms and z solves, FP64
+near-singular columns, `--refine` FP64-residual refinement steps) on the same workload and
+reports its error against the FP64 form.
+
 Prints one JSON line: grid points solved per second, the roofline of the dominant kernel
-(HIP events on the solver's stream inside the timed region) and a CPU baseline (the
-oracle, i.e. the C restatement of the reference algorithm, on a bounded 256^3 sample of
-the same workload, one thread).
+(HIP events recorded on the solver's stream between its launches, inside the timed region)
+and a CPU baseline (the oracle, i.e. the C restatement of the reference algorithm, on a
+bounded 256^3 sample of the same workload, one thread).
 """
 from __future__ import annotations
 
@@ -24,11 +28,25 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BC = "DDNDDD"
-STAGES = ["x_forward", "y_forward", "z_solve", "y_inverse", "x_inverse"]
+
+# launch order of one solve and the algorithmic bytes per grid point of each kernel
+FP64_KERNELS = [("x_forward", 16), ("y_forward", 16), ("z_solve", 16), ("y_inverse", 16), ("x_inverse", 16)]
+MIXED_INIT = [("x_forward_f64_to_f32", 12), ("y_forward_f32", 8), ("z_solve_f32", 8), ("z_fix_f64", 0),
+              ("y_inverse_f32", 8), ("x_inverse_f32_to_f64", 12)]
+MIXED_STEP = [("residual_f64", 20), ("x_forward_f32", 8), ("y_forward_f32", 8), ("z_solve_f32", 8),
+              ("z_fix_f64", 0), ("y_inverse_f32", 8), ("x_inverse_accumulate", 20)]
+
+
+def kernel_table(precision: int, refine: int):
+    if precision == 0:
+        return FP64_KERNELS
+    out = list(MIXED_INIT)
+    for r in range(refine):
+        out += [(f"{k}.{r + 1}", b) for k, b in MIXED_STEP]
+    return out
 
 
 def cpu_baseline(n: int = 256, bc: str = BC):
-    import numpy as np
     import oracle
     from pittpack_amd import rng
 
@@ -46,15 +64,16 @@ def cpu_baseline(n: int = 256, bc: str = BC):
     }
 
 
-def load_traffic(name: str):
-    p = os.path.join(ROOT, "profiles", name)
-    if os.path.exists(p):
-        try:
-            with open(p) as fh:
-                return json.load(fh)
-        except Exception:
-            return None
-    return None
+def load_traffic(workload_key: str, kernel: str):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes (scaled by the copy-kernel calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as fh:
+            t = json.load(fh)
+        return t[workload_key][kernel]["bytes_per_launch"]
+    except Exception:
+        return None
 
 
 def main():
@@ -63,12 +82,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64 (headline), 1 mixed + refinement")
+    ap.add_argument("--refine", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (no multi-GPU path); use --gpus 1"}))
         sys.exit(2)
 
     import numpy as np
@@ -90,6 +111,9 @@ def main():
     M.setCoords(2)
     if args.zsolver is not None:
         M.set_option(pa.OPT_ZSOLVER, args.zsolver)
+    if args.precision:
+        M.set_option(pa.OPT_PRECISION, args.precision)
+        M.set_option(pa.OPT_REFINE, args.refine)
     stream = torch.cuda.current_stream(dev)
     M.set_stream(stream.cuda_stream)
 
@@ -105,31 +129,44 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
 
-    stage_ms = M.stage_times()
+    kt = M.kernel_times()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     pts = float(n) ** 3
     value = pts / (ms_per_step * 1e-3)
 
-    bytes_per_stage = 16.0 * pts  # FP64: each stage reads and writes the field once
-    dom = int(np.argmax(stage_ms))
-    achieved = bytes_per_stage / (stage_ms[dom] * 1e-3) / 1e9
-    traffic = load_traffic("traffic_fp64_512.json")
-    tr_val = None
-    if traffic and STAGES[dom] in traffic:
-        tr_val = traffic[STAGES[dom]]
+    table = kernel_table(args.precision, args.refine)
+    names = [k for k, _ in table]
+    kms = dict(zip(names, kt))
+    bytes_pp = dict(table)
+    dom = max((k for k in names if bytes_pp[k] > 0), key=lambda k: kms[k])
+    alg_bytes = bytes_pp[dom] * pts
+    achieved = alg_bytes / (kms[dom] * 1e-3) / 1e9
+    wl_key = f"{'fp64' if args.precision == 0 else 'mixed'}_{BC}_{n}"
+    traffic = load_traffic(wl_key, dom.split(".")[0])
 
-    # device copy bandwidth of the same array, for reference
+    extra = {}
+    if args.precision:
+        # error of the timed form against the FP64 form on the same input
+        M.set_option(pa.OPT_PRECISION, 0)
+        u64 = torch.empty_like(f)
+        M.solve_device_oop(f.data_ptr(), u64.data_ptr())
+        torch.cuda.synchronize()
+        extra["rel_max_diff_vs_fp64"] = float((u - u64).abs().max() / u64.abs().max())
+        del u64
+
+    # a plain elementwise copy of the same array (8 B read + 8 B written per point)
     torch.cuda.synchronize()
     c0 = time.perf_counter()
     for _ in range(5):
-        u.copy_(f)
+        torch.neg(f, out=u)
     torch.cuda.synchronize()
     copy_gbs = 5 * 2 * pts * 8 / (time.perf_counter() - c0) / 1e9
 
     M.close()
 
+    form = "FP64 drop-in" if args.precision == 0 else f"mixed FP32 + {args.refine} FP64-residual refinement steps"
     out = {
-        "metric": "grid points solved/sec (FP64 drop-in, 512^3)",
+        "metric": f"grid points solved/sec ({'FP64 drop-in' if args.precision == 0 else 'mixed+refined'}, {n}^3)",
         "value": value,
         "unit": "grid points/s",
         "n_gpus": 1,
@@ -139,10 +176,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 0 else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP64 drop-in, {n}^3, bc x DD / y ND / z DD, box [0,1]^3, "
+            "workload": f"{form}, {n}^3, bc x DD / y ND / z DD, box [0,1]^3, "
                         "RHS U[-1,1] seed 0 (counter generator)",
             "n": n,
             "bc": BC,
@@ -150,17 +187,19 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": STAGES[dom],
+            "kernel": dom,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": tr_val,
+            "traffic": traffic,
+            "alg_bytes_per_launch": alg_bytes,
         },
-        "stages_ms": dict(zip(STAGES, stage_ms)),
-        "stages_gbs": {k: bytes_per_stage / (v * 1e-3) / 1e9 for k, v in zip(STAGES, stage_ms)},
+        "kernels_ms": kms,
+        "kernels_gbs": {k: bytes_pp[k] * pts / (kms[k] * 1e-3) / 1e9 for k in names if bytes_pp[k] > 0},
         "copy_gbs": copy_gbs,
     }
+    out.update(extra)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

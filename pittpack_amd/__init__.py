@@ -63,6 +63,13 @@ def load_library(path: str | None = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise PittPackError(HIP_ERROR, f"HIP library not built ({p}); run __graft_entry__.build()")
+    # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7).  Whichever
+    # is loaded first serves the whole process, and torch cannot initialise its GPU support
+    # on the system runtime, so when torch is installed it is loaded before this library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(p)
     vp, i, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
     L.pp_create.argtypes = [i, i, i, ctypes.POINTER(vp)]

@@ -130,6 +130,48 @@ int pp_size_supported(int n);
 /* GPU self test of the 16-lane rotation the z solve relies on; 0 when it behaves as assumed */
 int pp_selftest(void);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched form (no reference counterpart; the reference solves one grid per PencilDcmp
+ * object, src/pencilDcmp.cpp:5801-6430).  Many independent solves of different sizes,
+ * boundary conditions and boxes: their right-hand sides and solutions sit back to back in
+ * device arenas and each stage runs as one launch per shape class (line length and
+ * transform kind for x and y, rows per lane for z) over all members of the class.  A
+ * member's FP64 result is bitwise the one the single-solve handle computes.
+ * Order: pp_batch_create, [pp_batch_set_option(PP_OPT_PRECISION / ZERO_MODE),
+ * pp_batch_set_option_double(PP_OPT_TAU)], pp_batch_add (each member), pp_batch_finalize,
+ * fill the right-hand sides (pp_batch_set_rhs / pp_batch_field / pp_batch_fill_uniform),
+ * pp_batch_solve (asynchronous on the batch's stream; right-hand sides kept), read
+ * (pp_batch_get_solution / pp_batch_solution).
+ * ------------------------------------------------------------------------------------- */
+typedef struct pp_batch_s *pp_batch;
+
+int pp_batch_create(pp_batch *b);
+int pp_batch_destroy(pp_batch b);
+/* member with the reference's set-up arguments (sizes, bc[6], box X[6]); *id = its index */
+int pp_batch_add(pp_batch b, int nx, int ny, int nz, const char *bc, const double *X, int *id);
+int pp_batch_finalize(pp_batch b);
+int pp_batch_count(pp_batch b, int *members, long long *points);
+/* launches per stage: x, y, z (one per shape class) */
+int pp_batch_launch_count(pp_batch b, int *per_stage3);
+/* device pointers of a member's right-hand side and solution (nx*ny*nz doubles each) */
+int pp_batch_field(pp_batch b, int id, double **d_rhs);
+int pp_batch_solution(pp_batch b, int id, double **d_u);
+int pp_batch_set_rhs(pp_batch b, int id, const double *rhs);
+int pp_batch_get_solution(pp_batch b, int id, double *u);
+int pp_batch_get_rhs(pp_batch b, int id, double *rhs);
+/* every member's right-hand side from the counter generator with seed (seed + id),
+ * optionally rounded to FP32 */
+int pp_batch_fill_uniform(pp_batch b, unsigned long long seed, int round_f32);
+int pp_batch_set_stream(pp_batch b, void *stream);
+/* PP_OPT_ZERO_MODE, PP_OPT_PRECISION (before pp_batch_add), PP_OPT_REFINE, PP_OPT_TIMING */
+int pp_batch_set_option(pp_batch b, int key, long long value);
+int pp_batch_set_option_double(pp_batch b, int key, double value);
+int pp_batch_solve(pp_batch b);
+/* mean time (ms) between consecutive launch marks of the recorded batch solves
+ * (FP64: x fwd, y fwd, z, y inv, x inv) */
+int pp_batch_get_kernel_times(pp_batch b, float *ms, int max, int *count);
+int pp_batch_synchronize(pp_batch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,10 @@ EXPORTED = (
     "pp_get_solution_device", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
     "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
     "pp_selftest", "pp_set_option_double", "pp_get_kernel_times",
+    "pp_batch_create", "pp_batch_destroy", "pp_batch_add", "pp_batch_finalize", "pp_batch_count",
+    "pp_batch_launch_count", "pp_batch_field", "pp_batch_set_rhs", "pp_batch_get_solution",
+    "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
+    "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
 )
 
 
@@ -94,6 +98,25 @@ def load_library(path: str | None = None):
     L.pp_get_error_enum.restype = ctypes.c_char_p
     L.pp_size_supported.argtypes = [i]
     L.pp_selftest.argtypes = []
+    ip, fp = ctypes.POINTER(i), ctypes.POINTER(ctypes.c_float)
+    L.pp_batch_create.argtypes = [ctypes.POINTER(vp)]
+    L.pp_batch_destroy.argtypes = [vp]
+    L.pp_batch_add.argtypes = [vp, i, i, i, ctypes.c_char_p, dp, ip]
+    L.pp_batch_finalize.argtypes = [vp]
+    L.pp_batch_count.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
+    L.pp_batch_launch_count.argtypes = [vp, ip]
+    L.pp_batch_field.argtypes = [vp, i, ctypes.POINTER(vp)]
+    L.pp_batch_set_rhs.argtypes = [vp, i, dp]
+    L.pp_batch_get_solution.argtypes = [vp, i, dp]
+    L.pp_batch_fill_uniform.argtypes = [vp, ctypes.c_ulonglong, i]
+    L.pp_batch_set_stream.argtypes = [vp, vp]
+    L.pp_batch_set_option.argtypes = [vp, i, ctypes.c_longlong]
+    L.pp_batch_solve.argtypes = [vp]
+    L.pp_batch_get_kernel_times.argtypes = [vp, fp, i, ip]
+    L.pp_batch_solution.argtypes = [vp, i, ctypes.POINTER(vp)]
+    L.pp_batch_get_rhs.argtypes = [vp, i, dp]
+    L.pp_batch_set_option_double.argtypes = [vp, i, ctypes.c_double]
+    L.pp_batch_synchronize.argtypes = [vp]
     for name in EXPORTED:
         if name != "pp_get_error_enum":
             getattr(L, name).restype = ctypes.c_int
@@ -205,6 +228,117 @@ class PoissonHIP:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value is not None:
             self._L.pp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Batched form: many independent FP64 solves, one launch per stage and shape class.
+
+        B = Batch()
+        ids = [B.add(nx, ny, nz, bc, box) for ...]
+        B.finalize()
+        B.set_rhs(i, f_i) ...;  B.solve();  u_i = B.get_solution(i)
+    """
+
+    def __init__(self, precision: int = 0, refine: int | None = None, tau: float | None = None,
+                 zero_mode: int | None = None):
+        self._L = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self._L.pp_batch_create(ctypes.byref(self._h)), "pp_batch_create")
+        self.shapes = []
+        if precision:
+            self.set_option(OPT_PRECISION, precision)
+        if refine is not None:
+            self.set_option(OPT_REFINE, refine)
+        if tau is not None:
+            _check(self._L.pp_batch_set_option_double(self._h, OPT_TAU, float(tau)), "batch_set_option_double")
+        if zero_mode is not None:
+            self.set_option(OPT_ZERO_MODE, zero_mode)
+
+    def add(self, nx: int, ny: int, nz: int, bc: str, box=None) -> int:
+        X = np.ascontiguousarray(box if box is not None else [0, 1, 0, 1, 0, 1], dtype=np.float64)
+        bcb = bc.encode() if isinstance(bc, str) else bytes(bc)
+        i = ctypes.c_int(-1)
+        _check(self._L.pp_batch_add(self._h, int(nx), int(ny), int(nz), bcb, _dp(X), ctypes.byref(i)), "batch_add")
+        self.shapes.append((nz, ny, nx))
+        return i.value
+
+    def finalize(self) -> None:
+        _check(self._L.pp_batch_finalize(self._h), "batch_finalize")
+
+    def count(self):
+        m, p = ctypes.c_int(0), ctypes.c_longlong(0)
+        _check(self._L.pp_batch_count(self._h, ctypes.byref(m), ctypes.byref(p)), "batch_count")
+        return m.value, p.value
+
+    def launch_count(self):
+        a = (ctypes.c_int * 3)()
+        _check(self._L.pp_batch_launch_count(self._h, a), "batch_launch_count")
+        return list(a)
+
+    def field_ptr(self, i: int) -> int:
+        """device pointer of member i's right-hand side"""
+        p = ctypes.c_void_p()
+        _check(self._L.pp_batch_field(self._h, int(i), ctypes.byref(p)), "batch_field")
+        return p.value
+
+    def solution_ptr(self, i: int) -> int:
+        p = ctypes.c_void_p()
+        _check(self._L.pp_batch_solution(self._h, int(i), ctypes.byref(p)), "batch_solution")
+        return p.value
+
+    def set_rhs(self, i: int, rhs) -> None:
+        f = np.ascontiguousarray(rhs, dtype=np.float64)
+        if f.shape != self.shapes[i]:
+            raise ValueError(f"rhs shape {f.shape} != {self.shapes[i]}")
+        _check(self._L.pp_batch_set_rhs(self._h, int(i), _dp(f)), "batch_set_rhs")
+
+    def get_solution(self, i: int) -> np.ndarray:
+        u = np.empty(self.shapes[i], dtype=np.float64)
+        _check(self._L.pp_batch_get_solution(self._h, int(i), _dp(u)), "batch_get_solution")
+        return u
+
+    def get_rhs(self, i: int) -> np.ndarray:
+        f = np.empty(self.shapes[i], dtype=np.float64)
+        _check(self._L.pp_batch_get_rhs(self._h, int(i), _dp(f)), "batch_get_rhs")
+        return f
+
+    def fill_uniform(self, seed: int, round_f32: bool = False) -> None:
+        _check(self._L.pp_batch_fill_uniform(self._h, int(seed), int(bool(round_f32))), "batch_fill_uniform")
+
+    def set_stream(self, stream_ptr: int) -> None:
+        _check(self._L.pp_batch_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "batch_set_stream")
+
+    def set_option(self, key: int, value: int) -> None:
+        _check(self._L.pp_batch_set_option(self._h, int(key), int(value)), "batch_set_option")
+
+    def solve(self) -> None:
+        _check(self._L.pp_batch_solve(self._h), "batch_solve")
+
+    def synchronize(self) -> None:
+        _check(self._L.pp_batch_synchronize(self._h), "batch_synchronize")
+
+    def kernel_times(self):
+        ms = (ctypes.c_float * 64)()
+        n = ctypes.c_int(0)
+        _check(self._L.pp_batch_get_kernel_times(self._h, ms, 64, ctypes.byref(n)), "batch_kernel_times")
+        return list(ms)[: n.value]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value is not None:
+            self._L.pp_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,0 +1,25 @@
+// pp_launch.h -- host launchers shared by the single-solve handle and the batched form.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pp_line.h"
+#include "pp_zsolve.h"
+
+namespace pp {
+
+// line transform of one kind (pp_line_<kind>.hip) in precision form `form` (LineForm)
+hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s);
+
+// z solves (pp_zsolve_pcr.hip, pp_zsolve_scan.hip).  FP64 columns hold at most 32 rows per
+// lane (nz <= 512), FP32 columns at most 64 (nz <= 1024).
+hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s);
+hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
+bool zsolve_supported_f64(int nz);
+hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
+bool zsolve_supported_f32(int nz);
+
+// counter-generator right-hand side (pp_solver.hip): out[i] = 2 u - 1,
+// u = (splitmix64(seed * 2^32 + i) >> 11) * 2^-53, optionally rounded to FP32
+hipError_t launch_fill_uniform(double *out, long n, unsigned long long seed, int round_f32, hipStream_t s);
+
+}  // namespace pp

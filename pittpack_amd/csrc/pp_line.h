@@ -29,6 +29,24 @@ namespace pp {
 
 enum AxisKind : int { KIND_DD = 0, KIND_NN = 1, KIND_PP = 2, KIND_ND = 3, KIND_DN = 4 };
 
+// Batched launches: one launch covers the blocks of many independent solves of the same
+// line length and kind.  Block b works on items[b] of solve `solve`, whose fields sit at
+// `offset` in the launch's in/out arenas.
+struct LineSolve {
+    long offset;       // element offset of the solve's field in the in/out arenas
+    int nx, ny, nz;
+    int dmask;         // Dirichlet faces (x forward with BF_RHS)
+    double dz;         // x forward with BF_RHS: input * dz * dz
+    double inv_scale;  // x inverse with BF_SCALE: 1/(nx*ny)
+    const double *dtab;
+    long sums_off;   // offset of the solve's ny*nz line sums in line_sums
+    long plane_off;  // offset of the solve's nz (+2) plane values in plane_add
+};
+struct LineItem {
+    int solve, bx, by;
+};
+enum : int { BF_RHS = 1, BF_SCALE = 2 };
+
 struct LineArgs {
     const void *in;
     void *out;
@@ -52,6 +70,11 @@ struct LineArgs {
     // x inverse only: out = (accumulate ? out : 0) + out_scale * value + plane_add[z]
     const double *plane_add;
     int accumulate;
+    // batched launch (null: one solve, grid = blocks): per-block items, per-solve fields,
+    // and which per-solve scalings apply (BF_RHS, BF_SCALE)
+    const LineItem *items;
+    const LineSolve *solves;
+    int bflags;
 };
 
 struct c64 {
@@ -97,8 +120,34 @@ __device__ __forceinline__ R &lr(cpx<R> *L, int p)
 }
 
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
-__global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
+__global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
 {
+    LineArgs a = a0;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (a0.items) {
+        const LineItem it = a0.items[blockIdx.x];
+        const LineSolve sv = a0.solves[it.solve];
+        a.in = reinterpret_cast<const TIn *>(a0.in) + sv.offset;
+        a.out = reinterpret_cast<TOut *>(a0.out) + sv.offset;
+        a.nx = sv.nx;
+        a.ny = sv.ny;
+        a.nz = sv.nz;
+        if constexpr (AX == 0) {
+            a.nlines = (long)sv.ny * sv.nz;
+        } else {
+            a.nlines = sv.nx;
+            a.ncols = sv.nx;
+            a.nplanes = sv.nz;
+        }
+        a.in_scale = (a0.bflags & BF_RHS) ? sv.dz : 1.0;
+        a.dmask = (a0.bflags & BF_RHS) ? sv.dmask : 0;
+        a.dtab = sv.dtab;
+        a.out_scale = (a0.bflags & BF_SCALE) ? sv.inv_scale : 1.0;
+        if (a0.line_sums) a.line_sums = a0.line_sums + sv.sums_off;
+        if (a0.plane_add) a.plane_add = a0.plane_add + sv.plane_off;
+        bx = it.bx;
+        by = it.by;
+    }
     constexpr int N = 2 * M;
     constexpr int E = M / T;
     constexpr int LS = line_slots<M>();
@@ -124,7 +173,7 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
     bool valid;
     int ly = 0, lz = 0;
     if constexpr (AX == 0) {
-        long line = (long)blockIdx.x * LINES + l;
+        long line = (long)bx * LINES + l;
         valid = line < a.nlines;
         if (!valid) line = 0;
         base = line * N;
@@ -132,10 +181,10 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
         ly = (int)(line % a.ny);
         lz = (int)(line / a.ny);
     } else {
-        long col = (long)blockIdx.x * LINES + l;
+        long col = (long)bx * LINES + l;
         valid = col < a.ncols;
         if (!valid) col = 0;
-        base = (long)blockIdx.y * N * a.ncols + col;
+        base = (long)by * N * a.ncols + col;
         stride = a.ncols;
     }
 
@@ -194,7 +243,7 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a)
             // the T threads of a line are adjacent lanes of one wave: butterfly reduction
 #pragma unroll
             for (int off = T / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, T);
-            if (t == 0 && valid) a.line_sums[(long)blockIdx.x * LINES + l] = lsum;
+            if (t == 0 && valid) a.line_sums[(long)bx * LINES + l] = lsum;
         }
     }
     (void)lsum;
@@ -425,7 +474,8 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
     }
     dim3 block(T * LINES);
     dim3 grid;
-    if (AX == 0) grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES));
+    if (a.items) grid = dim3((unsigned)a.nlines);  // batched: nlines holds the item count
+    else if (AX == 0) grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES));
     else grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES), (unsigned)a.nplanes);
     hipLaunchKernelGGL(kern, grid, block, lds, s, a);
     return hipGetLastError();
@@ -476,5 +526,19 @@ hipError_t launch_line_kind(int form, int N, int AX, int FWD, const LineArgs &a,
 }
 
 inline bool line_size_supported(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }
+
+// lines (x) or columns (y) per block of the launch for line length N, precision R
+template <typename R>
+inline int line_block_lines(int N, int AX)
+{
+    switch (N / 2) {
+#define PPM(m) \
+    case m: return AX == 0 ? line_lines<R, m, 0>() : line_lines<R, m, 1>();
+        PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256)
+        PPM(384) PPM(512)
+#undef PPM
+    default: return 0;
+    }
+}
 
 }  // namespace pp

@@ -26,9 +26,25 @@ struct ResArgs {
     double *upart;
 };
 
+// per-block partial sums of u over plane 0 written by the residual kernel
 int residual_parts(int nx, int ny);
+// blocks of one residual launch over an nx x ny x nz grid
+int residual_blocks(int nx, int ny, int nz);
 
-hipError_t launch_zfix64_f32(const ZArgs &a, const int2 *cols, int ncols, hipStream_t s);
+// one pinned member of a batched (0,0)-column launch
+struct Mode00Item {
+    const double *line_sums;
+    int ny, nz;
+    double b0, bn, inv_scale;
+    double *plane_add;
+    const double *upart;
+    int nparts;
+};
+
+// cols[i] = {sx, sy, member, 0}; nz_max bounds every listed column's length
+hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s);
+hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, int correct, hipStream_t s);
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, hipStream_t s);
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
                          double *plane_add, const double *upart, int nparts, int correct, hipStream_t s);
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s);

@@ -33,13 +33,28 @@ __device__ __forceinline__ Row3d zrow(int i, int n, double eig, double b0, doubl
     return {1.0, eig, 1.0};
 }
 
+// cols[b] = {sx, sy, member, 0}; batched launches (a.solves set) take the member's fields
 template <typename R>
-__global__ void __launch_bounds__(256) k_zfix64(ZArgs a, const int2 *cols)
+__global__ void __launch_bounds__(256) k_zfix64(ZArgs a0, const int4 *cols)
 {
     extern __shared__ double zf[];
+    const int4 sc = cols[blockIdx.x];
+    ZArgs a = a0;
+    if (a0.solves) {
+        const ZSolve sv = a0.solves[sc.z];
+        a.data = reinterpret_cast<R *>(a0.data) + sv.offset;
+        a.SX = sv.SX;
+        a.SY = sv.SY;
+        a.nz = sv.nz;
+        a.zlo_dirichlet = sv.zlo_dirichlet;
+        a.zhi_dirichlet = sv.zhi_dirichlet;
+        a.Cx = sv.Cx;
+        a.Cy = sv.Cy;
+        a.lamx = sv.lamx;
+        a.lamy = sv.lamy;
+    }
     double *col = zf;           // nz values
     double *cp = zf + a.nz;     // nz values
-    const int2 sc = cols[blockIdx.x];
     const long zs = (long)a.SX * a.SY;
     const long off = (long)sc.y * a.SX + sc.x;
     R *data = reinterpret_cast<R *>(a.data);
@@ -75,9 +90,8 @@ __global__ void __launch_bounds__(256) k_zfix64(ZArgs a, const int2 *cols)
 //  correction (correct == 1): the refinement residual's plane sums give rows z >= 1 of
 //      P du = R; row 0 is S0 - (plane-0 sum of the current u), assembled from the residual
 //      kernel's per-block partial sums upart[0..nparts).  plane_add[z] = du(z)/(NxNy).
-__global__ void __launch_bounds__(1024) k_mode00(const double *line_sums, int ny, int nz, double b0, double bn,
-                                                 double inv_scale, double *plane_add, const double *upart, int nparts,
-                                                 int correct)
+__device__ void mode00_body(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
+                            double *plane_add, const double *upart, int nparts, int correct)
 {
     extern __shared__ double m0[];
     double *r = m0;
@@ -118,18 +132,43 @@ __global__ void __launch_bounds__(1024) k_mode00(const double *line_sums, int ny
     for (int z = threadIdx.x; z < nz; z += blockDim.x) plane_add[z] = r[z] * inv_scale;
 }
 
+__global__ void __launch_bounds__(1024) k_mode00(const double *line_sums, int ny, int nz, double b0, double bn,
+                                                 double inv_scale, double *plane_add, const double *upart, int nparts,
+                                                 int correct)
+{
+    mode00_body(line_sums, ny, nz, b0, bn, inv_scale, plane_add, upart, nparts, correct);
+}
+
+// batched: block b handles pinned member items[b]
+__global__ void __launch_bounds__(1024) k_mode00_batch(const Mode00Item *items, int correct)
+{
+    const Mode00Item it = items[blockIdx.x];
+    mode00_body(it.line_sums, it.ny, it.nz, it.b0, it.bn, it.inv_scale, it.plane_add, it.upart, it.nparts, correct);
+}
+
 constexpr int RES_ZC = 8;  // planes marched per thread
 
+// batched launches: block b works on items[b] = {member, bx, by, bz} with solves[member]
 template <typename TOut>
-__global__ void __launch_bounds__(256) k_residual(ResArgs a)
+__global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z0 = blockIdx.z * RES_ZC;
+    ResArgs a = a0;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z, gx = gridDim.x;
+    if (items) {
+        const int4 it = items[blockIdx.x];
+        a = solves[it.x];
+        bx = it.y;
+        by = it.z;
+        bz = it.w;
+        gx = (a.nx + 63) / 64;
+    }
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by * 4 + (threadIdx.x >> 6);
+    const int z0 = bz * RES_ZC;
     const bool valid = x < a.nx && y < a.ny;
     const long sy = a.nx, sz = (long)a.nx * a.ny;
     const double *u = a.u;
-    if (a.upart && blockIdx.z == 0) {
+    if (a.upart && bz == 0) {
         // plane-0 sum of u, one partial per (x, y) block
         __shared__ double red[256];
         red[threadIdx.x] = valid ? u[x + sy * y] : 0.0;
@@ -138,7 +177,7 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a)
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.upart[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+        if (threadIdx.x == 0) a.upart[by * gx + bx] = red[0];
     }
     if (!valid) return;
     TOut *r = reinterpret_cast<TOut *>(a.r);
@@ -179,10 +218,10 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a)
     }
 }
 
-hipError_t launch_zfix64_f32(const ZArgs &a, const int2 *cols, int ncols, hipStream_t s)
+hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s)
 {
     if (ncols == 0) return hipSuccess;
-    const size_t lds = 2 * (size_t)a.nz * sizeof(double);
+    const size_t lds = 2 * (size_t)nz_max * sizeof(double);
     hipLaunchKernelGGL(k_zfix64<float>, dim3(ncols), dim3(256), lds, s, a, cols);
     return hipGetLastError();
 }
@@ -196,12 +235,30 @@ hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, dou
     return hipGetLastError();
 }
 
+hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, int correct, hipStream_t s)
+{
+    if (nitems == 0) return hipSuccess;
+    const size_t lds = 2 * (size_t)nz_max * sizeof(double);
+    hipLaunchKernelGGL(k_mode00_batch, dim3(nitems), dim3(1024), lds, s, items, correct);
+    return hipGetLastError();
+}
+
 int residual_parts(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
+
+int residual_blocks(int nx, int ny, int nz) { return residual_parts(nx, ny) * ((nz + RES_ZC - 1) / RES_ZC); }
 
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.nx + 63) / 64), (unsigned)((a.ny + 3) / 4), (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
-    hipLaunchKernelGGL(k_residual<float>, grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_residual<float>, grid, dim3(256), 0, s, a, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, hipStream_t s)
+{
+    if (nitems == 0) return hipSuccess;
+    ResArgs a{};
+    hipLaunchKernelGGL(k_residual<float>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
     return hipGetLastError();
 }
 

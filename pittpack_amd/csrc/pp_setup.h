@@ -1,0 +1,177 @@
+// pp_setup.h -- host-side set-up of one solve, shared by the single-solve handle and the
+// batched form: boundary conditions -> transform kinds, Dirichlet tags, eigenvalue
+// parameters and scale (assignBoundary / detectTransforms / extractTag / setEigenVal /
+// setScale, src/pencilDcmp.cpp:3533-3732), grid spacing (setDxyz, 826-840), the eigenvalue
+// factors (getEigenVal, 4393-4417), the Dirichlet boundary-term tables (modifyRhsDirichlet
+// + exactValue, 4623-4835 and 1837-1866) and the FP64 twiddle tables of a line length.
+#pragma once
+#include <cmath>
+#include <vector>
+
+#include "pittpack_hip.h"
+#include "pp_line.h"
+
+namespace pp {
+
+constexpr double kPi = 3.1415926535897932384;  // src/include/params.h:116
+
+// exactValue (src/pencilDcmp.cpp:1837-1866)
+inline double exact_value(double omega, double x, int tg)
+{
+    double ans = 0.0;
+    if (std::fabs(omega) < 1.0e-6) return 1.0;
+    if (tg == 0) ans = std::sin(omega * x);
+    else if (tg == 1) ans = std::cos(omega * x);
+    else if (tg == -1) ans = std::sin(omega * x);
+    else if (tg == -2) ans = std::sin(omega * x);
+    return ans;
+}
+
+inline int axis_kind(char lo, char hi)
+{
+    if (lo == 'D' && hi == 'D') return KIND_DD;
+    if (lo == 'N' && hi == 'N') return KIND_NN;
+    if (lo == 'P' && hi == 'P') return KIND_PP;
+    if (lo == 'N' && hi == 'D') return KIND_ND;
+    if (lo == 'D' && hi == 'N') return KIND_DN;
+    return -1;
+}
+
+// per-axis twiddle tables in FP64 (host) for a real line of length N = 2M
+struct AxisTables {
+    std::vector<double> tw, A, B, C, D;  // interleaved complex
+};
+
+inline void make_axis_tables(int N, AxisTables &t)
+{
+    const int M = N / 2;
+    auto put = [](std::vector<double> &v, int cnt, auto f) {
+        v.resize(2 * (size_t)cnt);
+        for (int k = 0; k < cnt; k++) {
+            double ang = f(k);
+            v[2 * k] = std::cos(ang);
+            v[2 * k + 1] = std::sin(ang);
+        }
+    };
+    put(t.tw, M, [&](int e) { return -2.0 * kPi * (double)e / (double)M; });
+    put(t.A, M + 1, [&](int k) { return -2.0 * kPi * (double)k / (double)N; });
+    put(t.B, M + 1, [&](int k) { return -kPi * (double)k / (2.0 * N); });
+    put(t.C, M, [&](int n) { return -kPi * (4.0 * n + 1.0) / (4.0 * N); });
+    put(t.D, M, [&](int k) { return -kPi * (double)k / (double)N; });
+}
+
+// everything one solve's kernels need from (sizes, bc, box)
+struct SolveSetup {
+    int nx = 0, ny = 0, nz = 0;
+    char bc[6] = {0, 0, 0, 0, 0, 0};
+    double X[6] = {0, 1, 0, 1, 0, 1};
+    int kind[2] = {-1, -1};
+    int zlo_d = 0, zhi_d = 0;
+    int dtag[3] = {0, 0, 0};
+    double num[2] = {0, 0}, denum[2] = {0, 0}, freqs[2] = {kPi, kPi};
+    double scale = 1.0;
+    double dxyz[3] = {0, 0, 0};
+    int dmask = 0;
+};
+
+// assignBoundary + detectTransforms + extractTag + setEigenVal + setScale
+inline int setup_boundary(SolveSetup &g, const char *bc)
+{
+    for (int i = 0; i < 6; i++)
+        if (bc[i] != 'P' && bc[i] != 'N' && bc[i] != 'D') return PP_UNSUPPORTED_BC;
+    const int kx = axis_kind(bc[0], bc[1]), ky = axis_kind(bc[2], bc[3]);
+    if (kx < 0 || ky < 0) return PP_UNSUPPORTED_BC;
+    if (bc[4] == 'P' || bc[5] == 'P') return PP_UNSUPPORTED_BC;  // periodic z: not built yet
+    for (int i = 0; i < 6; i++) g.bc[i] = bc[i];
+    g.kind[0] = kx;
+    g.kind[1] = ky;
+    g.zlo_d = bc[4] == 'D';
+    g.zhi_d = bc[5] == 'D';
+    // detectTransforms + extractTag (src/pencilDcmp.cpp:3636-3732): tags used by the
+    // Dirichlet boundary term (exactValue's tg)
+    char tr[3];
+    for (int i = 0; i < 3; i++) {
+        const char lo = bc[2 * i], hi = bc[2 * i + 1];
+        tr[i] = 'U';
+        if (lo == 'D' && hi == 'D') tr[i] = 'S';
+        else if (lo == 'N' && hi == 'N') tr[i] = 'C';
+        else if (lo == 'P' && hi == 'P') tr[i] = 'F';
+    }
+    int count = 0;
+    for (int i = 0; i < 3; i++) count += (tr[i] == 'U');
+    if (count == 0) tr[2] = 'U';
+    for (int i = 0; i < 3; i++) g.dtag[i] = tr[i] == 'S' ? 0 : tr[i] == 'C' ? 1 : tr[i] == 'F' ? 2 : -1;
+    if (bc[4] == 'D') g.dtag[2] = -1;
+    else if (bc[4] == 'N') g.dtag[2] = -2;
+    // setEigenVal (3533-3576) with num = 1/2 for the quarter-wave axes
+    const int k2[2] = {kx, ky};
+    for (int a = 0; a < 2; a++) {
+        g.freqs[a] = kPi;
+        g.denum[a] = 0.0;
+        switch (k2[a]) {
+        case KIND_DD: g.num[a] = 1.0; break;
+        case KIND_NN: g.num[a] = 0.0; break;
+        case KIND_PP: g.num[a] = 0.0; g.freqs[a] = 2. * g.freqs[a]; break;
+        default: g.num[a] = 0.5; break;
+        }
+    }
+    // setScale (3670-3683)
+    g.scale = (double)g.nx * (double)g.ny;
+    g.dmask = 0;
+    for (int f = 0; f < 6; f++)
+        if (bc[f] == 'D') g.dmask |= (1 << f);
+    return PP_SUCCESS;
+}
+
+// setDxyz (src/pencilDcmp.cpp:826-840); at one rank every orientation spans the whole box
+inline void setup_coords(SolveSetup &g)
+{
+    const double *X = g.X;
+    g.dxyz[0] = -(X[0] - X[1]) / (g.nx);
+    g.dxyz[1] = -(X[2] - X[3]) / (g.ny);
+    g.dxyz[2] = -(X[4] - X[5]) / (g.nz);
+}
+
+// eigenvalue factors -2 + 2 cos((j + num) * freq / (n + denum)) (src/pencilDcmp.cpp:4410-4413)
+inline std::vector<double> eig_factors(const SolveSetup &g, int axis)
+{
+    const int n = axis == 0 ? g.nx : g.ny;
+    std::vector<double> l(n);
+    for (int j = 0; j < n; j++) l[j] = (-2.0 + 2. * std::cos((j + g.num[axis]) * g.freqs[axis] / (n + g.denum[axis])));
+    return l;
+}
+
+// Dirichlet term tables (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835, SHIFT = 1):
+// [ex(nx) ey(ny) ez(nz) x-faces(2) y-faces(2) z-faces(2)]
+inline std::vector<double> dirichlet_table(const SolveSetup &g)
+{
+    const double pi = 4. * std::atan(1.0);
+    const double omega[3] = {1 * pi, 2 * pi, 3 * pi};
+    const double Xa = g.X[0], Ya = g.X[2], Za = g.X[4];
+    const double c1 = g.dxyz[0], c2 = g.dxyz[1], c3 = g.dxyz[2];
+    const double shift = 1.0;
+    std::vector<double> dt((size_t)g.nx + g.ny + g.nz + 6);
+    size_t o = 0;
+    for (int i = 0; i < g.nx; i++) dt[o++] = exact_value(omega[0], Xa + i * c1 + shift * c1 * .5, g.dtag[0]);
+    for (int j = 0; j < g.ny; j++) dt[o++] = exact_value(omega[1], Ya + j * c2 + shift * c2 * 0.5, g.dtag[1]);
+    for (int k = 0; k < g.nz; k++) dt[o++] = exact_value(omega[2], Za + k * c3 + shift * c3 * .5, g.dtag[2]);
+    dt[o++] = exact_value(omega[0], Xa, g.dtag[0]);
+    dt[o++] = exact_value(omega[0], Xa + ((g.nx - 1) + 1) * c1, g.dtag[0]);
+    dt[o++] = exact_value(omega[1], Ya, g.dtag[1]);
+    dt[o++] = exact_value(omega[1], Ya + ((g.ny - 1) + 1) * c2, g.dtag[1]);
+    dt[o++] = exact_value(omega[2], Za, g.dtag[2]);
+    dt[o++] = exact_value(omega[2], Za + (g.nz - 1) * c3 + c3, g.dtag[2]);
+    return dt;
+}
+
+// eigenvalue of column (sx, sy) with the kernels' rounding (no contraction)
+inline double column_eig(const std::vector<double> &lx, const std::vector<double> &ly, int sx, int sy, double Cx,
+                         double Cy)
+{
+    volatile double t1 = lx[sx] * Cx;
+    volatile double t2 = -2.0 + t1;
+    volatile double t3 = ly[sy] * Cy;
+    return t2 + t3;
+}
+
+}  // namespace pp

@@ -10,7 +10,9 @@
 
 #include "pittpack_hip.h"
 #include "pp_line.h"
+#include "pp_launch.h"
 #include "pp_mixed.h"
+#include "pp_setup.h"
 #include "pp_zsolve.h"
 
 namespace pp {
@@ -31,21 +33,14 @@ __global__ void k_selftest_rot(int *out)
 
 using namespace pp;
 
-namespace {
-
-constexpr double kPi = 3.1415926535897932384;  // src/include/params.h:116
-
-// line transforms: one translation unit per kind (pp_line_<kind>.hip)
-}  // namespace
 namespace pp {
 extern template hipError_t launch_line_kind<KIND_DD>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_NN>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_PP>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_ND>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
-}  // namespace pp
-namespace {
 
+// line transforms: one translation unit per kind (pp_line_<kind>.hip)
 hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
 {
     switch (kind) {
@@ -58,66 +53,31 @@ hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArg
     }
 }
 
-bool size_supported(int n) { return line_size_supported(n); }
+__global__ void k_fill_uniform(double *out, long n, unsigned long long seed, int round_f32)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = (seed << 32) + (unsigned long long)i + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    double v = 2.0 * u - 1.0;
+    if (round_f32) v = (double)(float)v;
+    out[i] = v;
+}
 
-// z solve (pp_zsolve_pcr.hip, pp_zsolve_scan.hip).  FP64 columns hold at most 32 rows per
-// lane (nz <= 512): larger columns would spill registers in kernels that exchange data
-// across lanes.
-}  // namespace
-namespace pp {
-hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s);
-hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
-bool zsolve_supported_f64(int nz);
-hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
-bool zsolve_supported_f32(int nz);
+hipError_t launch_fill_uniform(double *out, long n, unsigned long long seed, int round_f32, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, seed, round_f32);
+    return hipGetLastError();
+}
 
 }  // namespace pp
 namespace {
 
-// exactValue (src/pencilDcmp.cpp:1837-1866)
-double exact_value(double omega, double x, int tg)
-{
-    double ans = 0.0;
-    if (std::fabs(omega) < 1.0e-6) return 1.0;
-    if (tg == 0) ans = std::sin(omega * x);
-    else if (tg == 1) ans = std::cos(omega * x);
-    else if (tg == -1) ans = std::sin(omega * x);
-    else if (tg == -2) ans = std::sin(omega * x);
-    return ans;
-}
-
-int axis_kind(char lo, char hi)
-{
-    if (lo == 'D' && hi == 'D') return KIND_DD;
-    if (lo == 'N' && hi == 'N') return KIND_NN;
-    if (lo == 'P' && hi == 'P') return KIND_PP;
-    if (lo == 'N' && hi == 'D') return KIND_ND;
-    if (lo == 'D' && hi == 'N') return KIND_DN;
-    return -1;
-}
-
-// per-axis twiddle tables in FP64 (host) for a real line of length N = 2M
-struct AxisTables {
-    std::vector<double> tw, A, B, C, D;  // interleaved complex
-};
-
-void make_axis_tables(int N, AxisTables &t)
-{
-    const int M = N / 2;
-    auto put = [](std::vector<double> &v, int cnt, auto f) {
-        v.resize(2 * (size_t)cnt);
-        for (int k = 0; k < cnt; k++) {
-            double ang = f(k);
-            v[2 * k] = std::cos(ang);
-            v[2 * k + 1] = std::sin(ang);
-        }
-    };
-    put(t.tw, M, [&](int e) { return -2.0 * kPi * (double)e / (double)M; });
-    put(t.A, M + 1, [&](int k) { return -2.0 * kPi * (double)k / (double)N; });
-    put(t.B, M + 1, [&](int k) { return -kPi * (double)k / (2.0 * N); });
-    put(t.C, M, [&](int n) { return -kPi * (4.0 * n + 1.0) / (4.0 * N); });
-    put(t.D, M, [&](int k) { return -kPi * (double)k / (double)N; });
-}
+bool size_supported(int n) { return line_size_supported(n); }
 
 struct DevTables {
     void *tw = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr;
@@ -126,16 +86,8 @@ struct DevTables {
 }  // namespace
 
 struct pp_solver {
-    int nx = 0, ny = 0, nz = 0;
-    char bc[6] = {0, 0, 0, 0, 0, 0};
+    SolveSetup g;  // sizes, boundary conditions, box, spacing, eigenvalue parameters
     bool bc_set = false, box_set = false, coords_set = false, rhs_set = false;
-    double Xbox[6] = {0, 1, 0, 1, 0, 1};
-    int kind[2] = {-1, -1};
-    int zlo_d = 0, zhi_d = 0;
-    int dtag[3] = {0, 0, 0};
-    double num[2] = {0, 0}, denum[2] = {0, 0}, freqs[2] = {kPi, kPi};
-    double scale = 1.0;
-    double dxyz[3] = {0, 0, 0};
     int zero_mode = 1;
     int zsolver = 0;
     bool timing = false;
@@ -145,7 +97,6 @@ struct pp_solver {
     double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
     DevTables tab64[2];
-    int dmask = 0;
     // kernel-timing ring: slot s holds up to kMaxMarks events recorded between the kernel
     // launches of one solve
     static constexpr int kMaxMarks = 64;
@@ -163,7 +114,7 @@ struct pp_solver {
     double *d_line_sums = nullptr; // ny*nz
     double *d_plane_add = nullptr; // nz + 2
     double *d_upart = nullptr;     // residual kernel's plane-0 partial sums of u
-    int2 *d_fix = nullptr;         // near-singular columns solved in FP64
+    int4 *d_fix = nullptr;         // near-singular columns solved in FP64: {sx, sy, 0, 0}
     int nfix = 0;
     bool pinned00 = false;
     bool mixed_ready = false;
@@ -246,7 +197,7 @@ int upload(void **dst, const std::vector<double> &v)
 int build_tables(pp_solver *h)
 {
     free_tables(h);
-    const int n[2] = {h->nx, h->ny};
+    const int n[2] = {h->g.nx, h->g.ny};
     int rc;
     for (int a = 0; a < 2; a++) {
         AxisTables t;
@@ -258,35 +209,12 @@ int build_tables(pp_solver *h)
         if ((rc = upload(&d.C, t.C))) return rc;
         if ((rc = upload(&d.D, t.D))) return rc;
     }
-    // eigenvalue factors -2 + 2 cos((j + num) * freq / (n + denum)) (src/pencilDcmp.cpp:4410-4413)
-    std::vector<double> lx(h->nx), ly(h->ny);
-    for (int j = 0; j < h->nx; j++) lx[j] = (-2.0 + 2. * std::cos((j + h->num[0]) * h->freqs[0] / (h->nx + h->denum[0])));
-    for (int i = 0; i < h->ny; i++) ly[i] = (-2.0 + 2. * std::cos((i + h->num[1]) * h->freqs[1] / (h->ny + h->denum[1])));
+    const std::vector<double> lx = eig_factors(h->g, 0), ly = eig_factors(h->g, 1);
     if ((rc = upload((void **)&h->d_lamx, lx))) return rc;
     if ((rc = upload((void **)&h->d_lamy, ly))) return rc;
     h->lamx_h = lx;
     h->lamy_h = ly;
-    // Dirichlet term tables (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835, SHIFT = 1)
-    const double pi = 4. * std::atan(1.0);
-    const double omega[3] = {1 * pi, 2 * pi, 3 * pi};
-    const double Xa = h->Xbox[0], Ya = h->Xbox[2], Za = h->Xbox[4];
-    const double c1 = h->dxyz[0], c2 = h->dxyz[1], c3 = h->dxyz[2];
-    const double shift = 1.0;
-    std::vector<double> dt((size_t)h->nx + h->ny + h->nz + 6);
-    size_t o = 0;
-    for (int i = 0; i < h->nx; i++) dt[o++] = exact_value(omega[0], Xa + i * c1 + shift * c1 * .5, h->dtag[0]);
-    for (int j = 0; j < h->ny; j++) dt[o++] = exact_value(omega[1], Ya + j * c2 + shift * c2 * 0.5, h->dtag[1]);
-    for (int k = 0; k < h->nz; k++) dt[o++] = exact_value(omega[2], Za + k * c3 + shift * c3 * .5, h->dtag[2]);
-    dt[o++] = exact_value(omega[0], Xa, h->dtag[0]);
-    dt[o++] = exact_value(omega[0], Xa + ((h->nx - 1) + 1) * c1, h->dtag[0]);
-    dt[o++] = exact_value(omega[1], Ya, h->dtag[1]);
-    dt[o++] = exact_value(omega[1], Ya + ((h->ny - 1) + 1) * c2, h->dtag[1]);
-    dt[o++] = exact_value(omega[2], Za, h->dtag[2]);
-    dt[o++] = exact_value(omega[2], Za + (h->nz - 1) * c3 + c3, h->dtag[2]);
-    if ((rc = upload((void **)&h->d_dtab, dt))) return rc;
-    h->dmask = 0;
-    for (int f = 0; f < 6; f++)
-        if (h->bc[f] == 'D') h->dmask |= (1 << f);
+    if ((rc = upload((void **)&h->d_dtab, dirichlet_table(h->g)))) return rc;
     h->tables_ready = true;
     return PP_SUCCESS;
 }
@@ -294,8 +222,8 @@ int build_tables(pp_solver *h)
 int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
-    if (h->precision == 1 && !zsolve_supported_f32(h->nz)) return PP_UNSUPPORTED_SIZE;
-    if (h->precision == 0 && !zsolve_supported_f64(h->nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 1 && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 0 && !zsolve_supported_f64(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (!h->tables_ready) {
         h->mixed_ready = false;
         return build_tables(h);
@@ -318,15 +246,15 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
     a.out_scale = 1.0;
     a.dtab = h->d_dtab;
     a.dmask = 0;
-    a.nx = h->nx;
-    a.ny = h->ny;
-    a.nz = h->nz;
+    a.nx = h->g.nx;
+    a.ny = h->g.ny;
+    a.nz = h->g.nz;
     if (axis == 0) {
-        a.nlines = (long)h->ny * h->nz;
+        a.nlines = (long)h->g.ny * h->g.nz;
     } else {
-        a.nlines = h->nx;
-        a.ncols = h->nx;
-        a.nplanes = h->nz;
+        a.nlines = h->g.nx;
+        a.ncols = h->g.nx;
+        a.nplanes = h->g.nz;
     }
     return a;
 }
@@ -341,45 +269,45 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
     {
         LineArgs a = line_args(h, 0, field);
         a.in = rhs;
-        a.in_scale = h->dxyz[2];
-        a.dmask = h->dmask;
-        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[0], h->nx, 0, 1, a, s));
+        a.in_scale = h->g.dxyz[2];
+        a.dmask = h->g.dmask;
+        PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[0], h->g.nx, 0, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     // 2. forward y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[1], h->ny, 1, 1, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     // 3. z solve
     {
         ZArgs z{};
         z.data = field;
-        z.SX = h->nx;
-        z.SY = h->ny;
-        z.nz = h->nz;
+        z.SX = h->g.nx;
+        z.SY = h->g.ny;
+        z.nz = h->g.nz;
         z.lamx = h->d_lamx;
         z.lamy = h->d_lamy;
-        z.Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
-        z.Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
-        z.zlo_dirichlet = h->zlo_d;
-        z.zhi_dirichlet = h->zhi_d;
+        z.Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+        z.Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+        z.zlo_dirichlet = h->g.zlo_d;
+        z.zhi_dirichlet = h->g.zhi_d;
         z.zero_mode = h->zero_mode;
-        PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->nz, z, s) : launch_zsolve_pcr_f64(h->nz, z, s));
+        PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
     }
     if ((rc = mark(h))) return rc;
     // 4. inverse y
     {
         LineArgs a = line_args(h, 1, field);
-        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[1], h->ny, 1, 0, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
     // 5. inverse x with the 1/(Nx*Ny) rescale fused
     {
         LineArgs a = line_args(h, 0, field);
-        a.out_scale = 1. / (h->scale);
-        PP_HIP_CHECK(launch_line(FORM_F64, h->kind[0], h->nx, 0, 0, a, s));
+        a.out_scale = 1. / (h->g.scale);
+        PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
     end_timing(h);
@@ -390,15 +318,15 @@ ZArgs z_args(pp_solver *h, void *data)
 {
     ZArgs z{};
     z.data = data;
-    z.SX = h->nx;
-    z.SY = h->ny;
-    z.nz = h->nz;
+    z.SX = h->g.nx;
+    z.SY = h->g.ny;
+    z.nz = h->g.nz;
     z.lamx = h->d_lamx;
     z.lamy = h->d_lamy;
-    z.Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
-    z.Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
-    z.zlo_dirichlet = h->zlo_d;
-    z.zhi_dirichlet = h->zhi_d;
+    z.Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+    z.Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+    z.zlo_dirichlet = h->g.zlo_d;
+    z.zhi_dirichlet = h->g.zhi_d;
     z.zero_mode = h->zero_mode;
     z.tau = 0.0;
     z.pinned_zero = 0;
@@ -412,16 +340,16 @@ int ensure_mixed(pp_solver *h)
     if (h->mixed_ready) return PP_SUCCESS;
     // size-dependent buffers: allocated once per handle (sizes are fixed at pp_create) and
     // freed only by pp_destroy, so no pointer handed to a solve can be released under it
-    const size_t N = (size_t)h->nx * h->ny * h->nz;
+    const size_t N = (size_t)h->g.nx * h->g.ny * h->g.nz;
     if (!h->d_w && hipMalloc((void **)&h->d_w, N * sizeof(float)) != hipSuccess) return PP_ALLOCATION_FAIL;
     if (!h->d_line_sums &&
-        hipMalloc((void **)&h->d_line_sums, (size_t)h->ny * h->nz * sizeof(double)) != hipSuccess)
+        hipMalloc((void **)&h->d_line_sums, (size_t)h->g.ny * h->g.nz * sizeof(double)) != hipSuccess)
         return PP_ALLOCATION_FAIL;
     if (!h->d_plane_add &&
-        hipMalloc((void **)&h->d_plane_add, (size_t)(h->nz + 2) * sizeof(double)) != hipSuccess)
+        hipMalloc((void **)&h->d_plane_add, (size_t)(h->g.nz + 2) * sizeof(double)) != hipSuccess)
         return PP_ALLOCATION_FAIL;
     if (!h->d_upart &&
-        hipMalloc((void **)&h->d_upart, (size_t)residual_parts(h->nx, h->ny) * sizeof(double)) != hipSuccess)
+        hipMalloc((void **)&h->d_upart, (size_t)residual_parts(h->g.nx, h->g.ny) * sizeof(double)) != hipSuccess)
         return PP_ALLOCATION_FAIL;
     // the near-singular column list depends on the tables, tau and the zero-mode rule
     if (h->d_fix) {
@@ -430,29 +358,26 @@ int ensure_mixed(pp_solver *h)
         h->d_fix = nullptr;
     }
     h->nfix = 0;
-    const double Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
-    const double Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+    const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+    const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
     const std::vector<double> &lx = h->lamx_h, &ly = h->lamy_h;
-    std::vector<int2> fix;
+    std::vector<int4> fix;
     h->pinned00 = false;
-    for (int sy = 0; sy < h->ny; sy++)
-        for (int sx = 0; sx < h->nx; sx++) {
-            volatile double t1 = lx[sx] * Cx;
-            volatile double t2 = -2.0 + t1;
-            volatile double t3 = ly[sy] * Cy;
-            const double eig = t2 + t3;
+    for (int sy = 0; sy < h->g.ny; sy++)
+        for (int sx = 0; sx < h->g.nx; sx++) {
+            const double eig = column_eig(lx, ly, sx, sy, Cx, Cy);
             bool pin = std::fabs(eig + 2.) < 1.e-10;
-            if (pin && h->zero_mode == 1 && (h->zlo_d || h->zhi_d)) pin = false;
+            if (pin && h->zero_mode == 1 && (h->g.zlo_d || h->g.zhi_d)) pin = false;
             if (pin) {
                 h->pinned00 = true;
                 continue;
             }
-            if (std::fabs(eig + 2.) < h->tau) fix.push_back(make_int2(sx, sy));
+            if (std::fabs(eig + 2.) < h->tau) fix.push_back(make_int4(sx, sy, 0, 0));
         }
     h->nfix = (int)fix.size();
     if (h->nfix) {
-        if (hipMalloc((void **)&h->d_fix, fix.size() * sizeof(int2)) != hipSuccess) return PP_ALLOCATION_FAIL;
-        PP_HIP_CHECK(hipMemcpy(h->d_fix, fix.data(), fix.size() * sizeof(int2), hipMemcpyHostToDevice));
+        if (hipMalloc((void **)&h->d_fix, fix.size() * sizeof(int4)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_fix, fix.data(), fix.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     h->mixed_ready = true;
     return PP_SUCCESS;
@@ -468,23 +393,23 @@ int mixed_inner(pp_solver *h)
         LineArgs a = line_args(h, 1, nullptr);
         a.in = h->d_w;
         a.out = h->d_w;
-        PP_HIP_CHECK(launch_line(FORM_F32, h->kind[1], h->ny, 1, 1, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     {
         ZArgs z = z_args(h, h->d_w);
         z.tau = h->tau;
         z.pinned_zero = 1;
-        PP_HIP_CHECK(launch_zsolve_pcr_f32(h->nz, z, s));
+        PP_HIP_CHECK(launch_zsolve_pcr_f32(h->g.nz, z, s));
         if ((rc = mark(h))) return rc;
-        PP_HIP_CHECK(launch_zfix64_f32(z, h->d_fix, h->nfix, s));
+        PP_HIP_CHECK(launch_zfix64_f32(z, h->d_fix, h->nfix, h->g.nz, s));
     }
     if ((rc = mark(h))) return rc;
     {
         LineArgs a = line_args(h, 1, nullptr);
         a.in = h->d_w;
         a.out = h->d_w;
-        PP_HIP_CHECK(launch_line(FORM_F32, h->kind[1], h->ny, 1, 0, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
     return mark(h);
 }
@@ -503,45 +428,45 @@ int run_mixed(pp_solver *h, const double *f, double *u)
         LineArgs a = line_args(h, 0, nullptr);
         a.in = f;
         a.out = h->d_w;
-        a.in_scale = h->dxyz[2];
-        a.dmask = h->dmask;
+        a.in_scale = h->g.dxyz[2];
+        a.dmask = h->g.dmask;
         a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
-        PP_HIP_CHECK(launch_line(FORM_F64_TO_F32, h->kind[0], h->nx, 0, 1, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F64_TO_F32, h->g.kind[0], h->g.nx, 0, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     if ((rc = mixed_inner(h))) return rc;
     if (h->pinned00) {
-        const double b0 = h->zlo_d ? -2.0 - 1. : -2.0 + 1.;
-        const double bn = h->zhi_d ? -2.0 - 1. : -2.0 + 1.;
-        PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->ny, h->nz, b0, bn, 1. / h->scale, h->d_plane_add, nullptr, 0, 0, s));
+        const double b0 = h->g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
+        const double bn = h->g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
+        PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, nullptr, 0, 0, s));
     }
     {
         LineArgs a = line_args(h, 0, nullptr);
         a.in = h->d_w;
         a.out = u;
-        a.out_scale = 1. / (h->scale);
+        a.out_scale = 1. / (h->g.scale);
         a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
         a.accumulate = 0;
-        PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->kind[0], h->nx, 0, 0, a, s));
+        PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
-    const double Cx = (h->dxyz[2] / h->dxyz[0]) * (h->dxyz[2] / h->dxyz[0]);
-    const double Cy = (h->dxyz[2] / h->dxyz[1]) * (h->dxyz[2] / h->dxyz[1]);
+    const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+    const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
     for (int it = 0; it < h->refine; it++) {
         ResArgs r{};
         r.f = f;
         r.u = u;
         r.r = h->d_w;
-        r.nx = h->nx;
-        r.ny = h->ny;
-        r.nz = h->nz;
-        r.c3 = h->dxyz[2];
+        r.nx = h->g.nx;
+        r.ny = h->g.ny;
+        r.nz = h->g.nz;
+        r.c3 = h->g.dxyz[2];
         r.Cx = Cx;
         r.Cy = Cy;
-        for (int i = 0; i < 6; i++) r.bc[i] = h->bc[i];
+        for (int i = 0; i < 6; i++) r.bc[i] = h->g.bc[i];
         r.dtab = h->d_dtab;
-        r.dmask = h->dmask;
-        r.pin_defect = h->pinned00 ? h->d_plane_add + h->nz : nullptr;
+        r.dmask = h->g.dmask;
+        r.pin_defect = h->pinned00 ? h->d_plane_add + h->g.nz : nullptr;
         r.upart = h->pinned00 ? h->d_upart : nullptr;
         PP_HIP_CHECK(launch_residual_f32(r, s));
         if ((rc = mark(h))) return rc;
@@ -550,26 +475,26 @@ int run_mixed(pp_solver *h, const double *f, double *u)
             a.in = h->d_w;
             a.out = h->d_w;
             a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
-            PP_HIP_CHECK(launch_line(FORM_F32, h->kind[0], h->nx, 0, 1, a, s));
+            PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[0], h->g.nx, 0, 1, a, s));
         }
         if ((rc = mark(h))) return rc;
         if ((rc = mixed_inner(h))) return rc;
         if (h->pinned00) {
             // the FP32 passes leak rounding into the plane means of u, which the zeroed
             // (0,0) column of the FP32 correction cannot see: correct that column in FP64
-            const double b0 = h->zlo_d ? -2.0 - 1. : -2.0 + 1.;
-            const double bn = h->zhi_d ? -2.0 - 1. : -2.0 + 1.;
-            PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->ny, h->nz, b0, bn, 1. / h->scale, h->d_plane_add, h->d_upart,
-                                       residual_parts(h->nx, h->ny), 1, s));
+            const double b0 = h->g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
+            const double bn = h->g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
+            PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, h->d_upart,
+                                       residual_parts(h->g.nx, h->g.ny), 1, s));
         }
         {
             LineArgs a = line_args(h, 0, nullptr);
             a.in = h->d_w;
             a.out = u;
-            a.out_scale = 1. / (h->scale);
+            a.out_scale = 1. / (h->g.scale);
             a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
             a.accumulate = 1;
-            PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->kind[0], h->nx, 0, 0, a, s));
+            PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
         }
         if ((rc = mark(h))) return rc;
     }
@@ -582,7 +507,7 @@ int run_solve(pp_solver *h, const double *rhs, double *out)
 {
     if (h->precision == 0) return run_fp64(h, rhs, out);
     if (rhs == out) {
-        const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+        const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
         if (!h->d_aux && hipMalloc((void **)&h->d_aux, bytes) != hipSuccess) return PP_ALLOCATION_FAIL;
         PP_HIP_CHECK(hipMemcpyAsync(h->d_aux, rhs, bytes, hipMemcpyDeviceToDevice, h->stream));
         return run_mixed(h, h->d_aux, out);
@@ -605,9 +530,9 @@ int pp_create(int nx, int ny, int nz, pp_handle *out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PP_NO_DEVICE;
     pp_solver *h = new (std::nothrow) pp_solver();
     if (!h) return PP_ALLOCATION_FAIL;
-    h->nx = nx;
-    h->ny = ny;
-    h->nz = nz;
+    h->g.nx = nx;
+    h->g.ny = ny;
+    h->g.nz = nz;
     const size_t bytes = (size_t)nx * ny * nz * sizeof(double);
     if (hipMalloc((void **)&h->d_field, bytes) != hipSuccess) {
         delete h;
@@ -642,46 +567,8 @@ int pp_assign_boundary(pp_handle h, const char *bc)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!bc) return PP_INPUT_ARGS;
-    for (int i = 0; i < 6; i++)
-        if (bc[i] != 'P' && bc[i] != 'N' && bc[i] != 'D') return PP_UNSUPPORTED_BC;
-    int kx = axis_kind(bc[0], bc[1]), ky = axis_kind(bc[2], bc[3]);
-    if (kx < 0 || ky < 0) return PP_UNSUPPORTED_BC;
-    if (bc[4] == 'P' || bc[5] == 'P') return PP_UNSUPPORTED_BC;  // periodic z: not built yet
-    for (int i = 0; i < 6; i++) h->bc[i] = bc[i];
-    h->kind[0] = kx;
-    h->kind[1] = ky;
-    h->zlo_d = bc[4] == 'D';
-    h->zhi_d = bc[5] == 'D';
-    // detectTransforms + extractTag (src/pencilDcmp.cpp:3636-3732): tags used by the
-    // Dirichlet boundary term (exactValue's tg)
-    char tr[3];
-    for (int i = 0; i < 3; i++) {
-        char lo = bc[2 * i], hi = bc[2 * i + 1];
-        tr[i] = 'U';
-        if (lo == 'D' && hi == 'D') tr[i] = 'S';
-        else if (lo == 'N' && hi == 'N') tr[i] = 'C';
-        else if (lo == 'P' && hi == 'P') tr[i] = 'F';
-    }
-    int count = 0;
-    for (int i = 0; i < 3; i++) count += (tr[i] == 'U');
-    if (count == 0) tr[2] = 'U';
-    for (int i = 0; i < 3; i++) h->dtag[i] = tr[i] == 'S' ? 0 : tr[i] == 'C' ? 1 : tr[i] == 'F' ? 2 : -1;
-    if (bc[4] == 'D') h->dtag[2] = -1;
-    else if (bc[4] == 'N') h->dtag[2] = -2;
-    // setEigenVal (3533-3576) with num = 1/2 for the quarter-wave axes
-    int k2[2] = {kx, ky};
-    for (int a = 0; a < 2; a++) {
-        h->freqs[a] = kPi;
-        h->denum[a] = 0.0;
-        switch (k2[a]) {
-        case KIND_DD: h->num[a] = 1.0; break;
-        case KIND_NN: h->num[a] = 0.0; break;
-        case KIND_PP: h->num[a] = 0.0; h->freqs[a] = 2. * h->freqs[a]; break;
-        default: h->num[a] = 0.5; break;
-        }
-    }
-    // setScale (3670-3683)
-    h->scale = (double)h->nx * (double)h->ny;
+    const int rc = setup_boundary(h->g, bc);
+    if (rc) return rc;
     h->bc_set = true;
     h->tables_ready = false;
     return PP_SUCCESS;
@@ -691,7 +578,7 @@ int pp_set_box(pp_handle h, const double *X)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!X) return PP_INPUT_ARGS;
-    for (int i = 0; i < 6; i++) h->Xbox[i] = X[i];
+    for (int i = 0; i < 6; i++) h->g.X[i] = X[i];
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
     h->box_set = true;
     h->tables_ready = false;
@@ -703,11 +590,7 @@ int pp_set_coords(pp_handle h, int dir)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (dir < 0 || dir > 2) return PP_INPUT_ARGS;
-    // setDxyz (src/pencilDcmp.cpp:826-840); at one rank every orientation spans the whole box
-    const double *X = h->Xbox;
-    h->dxyz[0] = -(X[0] - X[1]) / (h->nx);
-    h->dxyz[1] = -(X[2] - X[3]) / (h->ny);
-    h->dxyz[2] = -(X[4] - X[5]) / (h->nz);
+    setup_coords(h->g);
     h->coords_set = true;
     h->tables_ready = false;
     return PP_SUCCESS;
@@ -717,7 +600,7 @@ int pp_assign_rhs(pp_handle h, const double *rhs)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!rhs) return PP_INPUT_ARGS;
-    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(h->d_field, rhs, bytes, hipMemcpyHostToDevice, h->stream));
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->rhs_set = true;
@@ -728,7 +611,7 @@ int pp_assign_rhs_device(pp_handle h, const double *d_rhs)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!d_rhs) return PP_INPUT_ARGS;
-    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(h->d_field, d_rhs, bytes, hipMemcpyDeviceToDevice, h->stream));
     h->rhs_set = true;
     return PP_SUCCESS;
@@ -769,7 +652,7 @@ int pp_get_solution(pp_handle h, double *u)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!u) return PP_INPUT_ARGS;
-    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(u, h->d_field, bytes, hipMemcpyDeviceToHost, h->stream));
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     return PP_SUCCESS;
@@ -779,7 +662,7 @@ int pp_get_solution_device(pp_handle h, double *d_u)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (!d_u) return PP_INPUT_ARGS;
-    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(d_u, h->d_field, bytes, hipMemcpyDeviceToDevice, h->stream));
     return PP_SUCCESS;
 }

@@ -17,6 +17,9 @@ namespace pp {
 // ---------------------------------------------------------------------------------------
 // z solve
 // ---------------------------------------------------------------------------------------
+struct ZItem;
+struct ZSolve;
+
 struct ZArgs {
     void *data;  // [nz][SY][SX], in place
     int SX, SY, nz;
@@ -30,6 +33,22 @@ struct ZArgs {
     // contribution is solved exactly in FP64 elsewhere)
     double tau;
     int pinned_zero;
+    // batched launch (null: one solve): block b solves the columns of items[b]
+    const ZItem *items;
+    const ZSolve *solves;
+    int nitems;
+};
+
+// one solve of a batched z launch: its field sits at `offset` in the launch's data arena
+struct ZSolve {
+    long offset;
+    int SX, SY, nz;
+    int zlo_dirichlet, zhi_dirichlet;
+    double Cx, Cy;
+    const double *lamx, *lamy;
+};
+struct ZItem {
+    int solve, bx, by;
 };
 
 // Column element access.  The row stride zs is uniform, so row j of a lane sits at the
@@ -197,14 +216,14 @@ __device__ __forceinline__ void pcr_step(R *d, Row3<R> *ex, R &ta, R &tb, int t)
 // ---------------------------------------------------------------------------------------
 
 template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
-__device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow lrow)
+__device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow lrow, int bx, int by)
 {
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
-    const int col0 = blockIdx.x * CPB;
+    const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;
-    const long lane_elem = (long)blockIdx.y * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
+    const long lane_elem = (long)by * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
     const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
 #pragma unroll
     for (int q = 0; q < J; q++) {
@@ -216,14 +235,14 @@ __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow 
 }
 
 template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
-__device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n, LRow lrow)
+__device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n, LRow lrow, int bx, int by)
 {
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
-    const int col0 = blockIdx.x * CPB;
+    const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;
-    const long lane_elem = (long)blockIdx.y * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
+    const long lane_elem = (long)by * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
     const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
 #pragma unroll
     for (int q = 0; q < J; q++) {
@@ -244,15 +263,33 @@ constexpr size_t ztile_bytes_blocked() { return (size_t)16 * blocked_lj<J>() * (
 
 // FULL: nz == 16 * J (no padding rows, no per-row bounds checks)
 template <typename R, int J, bool FULL, bool OFF32, int CPB = 16>
-__global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a)
+__global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
 {
     constexpr bool ALL = (J <= 3);
     constexpr int NEX = ALL ? J : 3;
+    ZArgs a = a0;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (a0.items) {
+        const ZItem it = a0.items[blockIdx.x];
+        const ZSolve sv = a0.solves[it.solve];
+        a.data = reinterpret_cast<R *>(a0.data) + sv.offset;
+        a.SX = sv.SX;
+        a.SY = sv.SY;
+        a.nz = sv.nz;
+        a.zlo_dirichlet = sv.zlo_dirichlet;
+        a.zhi_dirichlet = sv.zhi_dirichlet;
+        a.Cx = sv.Cx;
+        a.Cy = sv.Cy;
+        a.lamx = sv.lamx;
+        a.lamy = sv.lamy;
+        bx = it.bx;
+        by = it.by;
+    }
     const int tid = threadIdx.x;
     const int t = tid & 15;
     const int c = tid >> 4;
-    int sx = blockIdx.x * CPB + c;
-    const int sy = blockIdx.y;
+    int sx = bx * CPB + c;
+    const int sy = by;
     const bool valid = sx < a.SX;
     if (!valid) sx = a.SX - 1;
     const int n = FULL ? 16 * J : a.nz;
@@ -278,7 +315,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a)
     extern __shared__ __align__(16) unsigned char zsm[];
     R *tile = reinterpret_cast<R *>(zsm);
     auto lrow = [](int r) { return r; };
-    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, bx, by);
     __syncthreads();
     R d[J];
 #pragma unroll
@@ -333,7 +370,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a)
         for (int j = 0; j < J; j++) tile[(t + 16 * j) * (CPB + 1) + c] = zero ? (R)0 : d[j];
     }
     __syncthreads();
-    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, bx, by);
     (void)valid;
     (void)data;
 }
@@ -404,7 +441,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve_scan(ZArgs a)
     R *tile = reinterpret_cast<R *>(zsm);
     constexpr int LJ = blocked_lj<J>();
     auto lrow = [](int r) { return (r / J) * LJ + (r % J); };
-    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, blockIdx.x, blockIdx.y);
     __syncthreads();
     R d[J];
 #pragma unroll
@@ -559,7 +596,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve_scan(ZArgs a)
         for (int j = 0; j < J; j++) tile[(t * LJ + j) * (CPB + 1) + cidx] = zero ? (R)0 : d[j];
     }
     __syncthreads();
-    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow);
+    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, blockIdx.x, blockIdx.y);
     (void)valid;
     (void)data;
     (void)col;

@@ -34,9 +34,16 @@ template <typename R, int J, int CPB>
 static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
-    const bool full = (a.nz == 16 * J);
-    // staged tiles address rows from a uniform base; lane offsets span 16 rows
-    if (16.0 * a.SX * a.SY * sizeof(R) >= 4294967296.0) return hipErrorInvalidValue;
+    bool full = (a.nz == 16 * J);
+    if (a.items) {
+        // batched: one block per item, column lengths vary inside the launch (every solve's
+        // 16 * SX * SY * sizeof(R) is checked below 4 GiB by the caller)
+        grid = dim3((unsigned)a.nitems);
+        full = false;
+    } else if (16.0 * a.SX * a.SY * sizeof(R) >= 4294967296.0) {
+        // staged tiles address rows from a uniform base; lane offsets span 16 rows
+        return hipErrorInvalidValue;
+    }
     constexpr size_t lds = ztile_bytes_cyclic<R, J, CPB>();
 #define PP_ZL(F, O)                                                                                            \
     do {                                                                                                       \

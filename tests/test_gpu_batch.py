@@ -1,0 +1,127 @@
+"""Batched form on the GPU: many independent FP64 solves of uneven sizes and boundary
+conditions in one set of launches per stage and shape class.
+
+Each member must be bitwise the single-solve handle's result for the same input (the batch
+runs the same kernels with the same arithmetic), and within the FP64 bar (1e-12 relative
+max-norm) of the oracle.  The device right-hand-side generator must reproduce the Python
+counter generator bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pittpack_amd as pa
+from pittpack_amd import rng as prng
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+# (nx, ny, nz, bc, box): mixed line lengths (radix 2 and 3), ragged z, all x/y combos
+MEMBERS = [
+    (32, 48, 17, "PPPPNN", [0, 1, 0, 1, 0, 1]),
+    (64, 32, 40, "NNNNDD", [0, 2, 0, 1, 0, 1]),
+    (48, 48, 33, "DDDDDD", [0, 1, 0, 1, 0, 1]),
+    (96, 64, 16, "DDNNNN", [0, 1, 0, 1, 0, 1]),
+    (32, 32, 100, "PPPPDD", [0, 1, 0, 1, 0, 3]),
+    (64, 96, 64, "DDNDDD", [0, 1, 0, 1, 0, 1]),
+    (48, 32, 5, "NNNNNN", [0, 1, 0, 1, 0, 1]),
+    (32, 64, 128, "NNDDND", [-1, 1, 0, 1, 0, 1]),
+    (64, 32, 40, "PPPPNN", [0, 1, 0, 1, 0, 1]),
+    (32, 48, 17, "DDDDNN", [0, 1, 0, 1, 0, 1]),
+]
+
+
+def rel_maxdiff(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def test_batch_matches_single_and_oracle():
+    print("batch members", flush=True)
+    with pa.Batch() as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in MEMBERS]
+        B.finalize()
+        assert B.count() == (len(MEMBERS), sum(nx * ny * nz for nx, ny, nz, _, _ in MEMBERS))
+        x, y, z = B.launch_count()
+        assert x == len({(m[0], m[3][:2]) for m in MEMBERS})
+        assert y == len({(m[1], m[3][2:4]) for m in MEMBERS})
+        assert z == len({(m[2] + 15) // 16 for m in MEMBERS})
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            f = prng.uniform_pm1((nz, ny, nx), seed=100 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            got = B.get_solution(i)
+            single = pa.solve(rhs[i], bc, box)
+            assert np.array_equal(got, single), f"member {i} {bc}: batched != single ({rel_maxdiff(got, single):.2e})"
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+            err = rel_maxdiff(got, ref)
+            assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
+
+
+def test_batch_repeated_and_device_fill():
+    """device generator == Python generator; the right-hand sides are kept, so a second solve
+    reproduces the first"""
+    print("batch fill", flush=True)
+    with pa.Batch() as B:
+        for nx, ny, nz, bc, box in MEMBERS[:4]:
+            B.add(nx, ny, nz, bc, box)
+        B.finalize()
+        B.fill_uniform(7)
+        for i in range(4):
+            nx, ny, nz = MEMBERS[i][:3]
+            assert np.array_equal(B.get_rhs(i), prng.uniform_pm1((nz, ny, nx), seed=7 + i))
+        B.solve()
+        first = [B.get_solution(i) for i in range(4)]
+        B.set_option(pa.OPT_TIMING, 1)
+        B.solve()
+        for i in range(4):
+            assert np.array_equal(B.get_solution(i), first[i])
+        kt = B.kernel_times()
+        assert len(kt) == 5 and all(t > 0 for t in kt)
+
+
+def test_batch_errors():
+    print("batch errors", flush=True)
+    with pa.Batch() as B:
+        with pytest.raises(pa.PittPackError) as e:
+            B.solve()
+        assert e.value.code == pa.CALL_ORDER
+        with pytest.raises(pa.PittPackError) as e:
+            B.add(30, 32, 16, "PPPPNN")
+        assert e.value.code == pa.UNSUPPORTED_SIZE
+        with pytest.raises(pa.PittPackError) as e:
+            B.add(32, 32, 16, "PNPPNN")
+        assert e.value.code == pa.UNSUPPORTED_BC
+        B.add(32, 32, 16, "PPPPNN")
+        B.finalize()
+        with pytest.raises(pa.PittPackError) as e:
+            B.add(32, 32, 16, "PPPPNN")
+        assert e.value.code == pa.CALL_ORDER
+
+
+@pytest.mark.parametrize("zero_mode", [0, 1])
+def test_batch_mixed_precision(zero_mode):
+    """mixed form over a batch: FP64 bar against the oracle after the default refinement,
+    and equal to the single-solve mixed handle"""
+    print(f"batch mixed zero_mode={zero_mode}", flush=True)
+    with pa.Batch(precision=1, zero_mode=zero_mode) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in MEMBERS]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            f = prng.uniform_pm1((nz, ny, nx), seed=300 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.set_option(pa.OPT_TIMING, 1)
+        B.solve()
+        assert len(B.kernel_times()) == 5 + 6 * 2
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            got = B.get_solution(i)
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=zero_mode)
+            err = rel_maxdiff(got, ref)
+            assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
+            single = pa.solve(rhs[i], bc, box, zero_mode=zero_mode, precision=1)
+            assert rel_maxdiff(got, single) <= 1e-14, f"member {i}: batched vs single mixed"

@@ -54,7 +54,8 @@ enum {
      * FP64-residual refinement.  Columns up to 1024 rows (FP64: 512). */
     PP_OPT_PRECISION = 4,
     PP_OPT_REFINE = 5, /* refinement steps of the mixed form (default 2) */
-    PP_OPT_TAU = 6     /* pp_set_option_double: near-singular threshold (default 2e-3) */
+    PP_OPT_TAU = 6,    /* pp_set_option_double: near-singular threshold (default 0.05) */
+    PP_OPT_BATCH_MEMBERS = 7 /* server: most requests per batch (default 256) */
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
@@ -171,6 +172,32 @@ int pp_batch_solve(pp_batch b);
  * (FP64: x fwd, y fwd, z, y inv, x inv) */
 int pp_batch_get_kernel_times(pp_batch b, float *ms, int max, int *count);
 int pp_batch_synchronize(pp_batch b);
+
+/* ---------------------------------------------------------------------------------------
+ * Serving form (no reference counterpart).  A request loop on its own scheduler thread:
+ * requests of any supported size, boundary condition and precision are admitted by HBM
+ * footprint into a device pool of hbm_bytes, grouped into batches (one per precision,
+ * at most PP_OPT_BATCH_MEMBERS requests) and launched on up to `streams` HIP streams, so
+ * batches admitted while others run overlap on the device.  Twiddle tables stay resident.
+ * A request's result does not depend on the batch it ran in.
+ * ------------------------------------------------------------------------------------- */
+typedef struct pp_server_s *pp_server;
+
+/* staging_bytes of pinned host memory hold solutions on their way back to callers */
+int pp_server_create(long long hbm_bytes, long long staging_bytes, int streams, pp_server *s);
+int pp_server_destroy(pp_server s);
+/* PP_OPT_REFINE, PP_OPT_BATCH_MEMBERS */
+int pp_server_set_option(pp_server s, int key, long long value);
+/* one request: sizes, bc[6], box X[6], precision (0 FP64, 1 mixed), right-hand side rhs
+ * (host, nx*ny*nz doubles, x fastest, kept alive until the request is done) or, when rhs is
+ * null, the counter generator with `seed`; want_solution = 1 to receive the solution */
+int pp_server_submit(pp_server s, int nx, int ny, int nz, const char *bc, const double *X, int precision,
+                     const double *rhs, unsigned long long seed, int want_solution, long long *ticket);
+/* blocks until the request is done; fills u (when it was requested and u is not null) and
+ * the submit-to-done latency; returns the request's status */
+int pp_server_wait(pp_server s, long long ticket, double *u, double *latency_ms);
+int pp_server_stats(pp_server s, long long *completed, long long *points, long long *batches, double *span_ms,
+                    int *max_concurrent);
 
 #ifdef __cplusplus
 }

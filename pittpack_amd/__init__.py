@@ -34,6 +34,7 @@ OPT_ZSOLVER = 3
 OPT_PRECISION = 4
 OPT_REFINE = 5
 OPT_TAU = 6
+OPT_BATCH_MEMBERS = 7
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -46,6 +47,8 @@ EXPORTED = (
     "pp_batch_launch_count", "pp_batch_field", "pp_batch_set_rhs", "pp_batch_get_solution",
     "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
     "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
+    "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
+    "pp_server_stats",
 )
 
 
@@ -115,6 +118,13 @@ def load_library(path: str | None = None):
     L.pp_batch_get_kernel_times.argtypes = [vp, fp, i, ip]
     L.pp_batch_solution.argtypes = [vp, i, ctypes.POINTER(vp)]
     L.pp_batch_get_rhs.argtypes = [vp, i, dp]
+    ll = ctypes.c_longlong
+    L.pp_server_create.argtypes = [ll, ll, i, ctypes.POINTER(vp)]
+    L.pp_server_destroy.argtypes = [vp]
+    L.pp_server_set_option.argtypes = [vp, i, ll]
+    L.pp_server_submit.argtypes = [vp, i, i, i, ctypes.c_char_p, dp, i, dp, ctypes.c_ulonglong, i, ctypes.POINTER(ll)]
+    L.pp_server_wait.argtypes = [vp, ll, dp, dp]
+    L.pp_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp, ip]
     L.pp_batch_set_option_double.argtypes = [vp, i, ctypes.c_double]
     L.pp_batch_synchronize.argtypes = [vp]
     for name in EXPORTED:
@@ -339,6 +349,75 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value is not None:
             self._L.pp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Serving form: submit solves from any thread, wait for them; batches run concurrently.
+
+        S = Server(hbm_bytes=8 << 30, staging_bytes=2 << 30, streams=4)
+        t = S.submit((nz, ny, nx), "PPPPNN", rhs=f)        # or seed=... for device-generated RHS
+        u, latency_ms = S.wait(t)
+    """
+
+    def __init__(self, hbm_bytes: int = 8 << 30, staging_bytes: int = 2 << 30, streams: int = 4):
+        self._L = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self._L.pp_server_create(int(hbm_bytes), int(staging_bytes), int(streams), ctypes.byref(self._h)),
+               "pp_server_create")
+        self._pending = {}
+
+    def set_option(self, key: int, value: int) -> None:
+        _check(self._L.pp_server_set_option(self._h, int(key), int(value)), "server_set_option")
+
+    def submit(self, shape, bc: str, box=None, precision: int = 0, rhs=None, seed: int = 0,
+               want_solution: bool = True) -> int:
+        nz, ny, nx = shape
+        X = np.ascontiguousarray(box if box is not None else [0, 1, 0, 1, 0, 1], dtype=np.float64)
+        f = None
+        if rhs is not None:
+            f = np.ascontiguousarray(rhs, dtype=np.float64)
+            if f.shape != (nz, ny, nx):
+                raise ValueError("rhs shape")
+        t = ctypes.c_longlong(0)
+        rp = _dp(f) if f is not None else None
+        _check(self._L.pp_server_submit(self._h, int(nx), int(ny), int(nz), bc.encode(), _dp(X), int(precision), rp,
+                                        int(seed), int(bool(want_solution)), ctypes.byref(t)), "server_submit")
+        # keep the host buffers alive until the request is collected
+        self._pending[t.value] = (f, X, (nz, ny, nx), bool(want_solution))
+        return t.value
+
+    def wait(self, ticket: int):
+        f, X, shape, want = self._pending.pop(ticket)
+        u = np.empty(shape, dtype=np.float64) if want else None
+        lat = ctypes.c_double(0)
+        _check(self._L.pp_server_wait(self._h, int(ticket), _dp(u) if u is not None else None, ctypes.byref(lat)),
+               "server_wait")
+        return u, lat.value
+
+    def stats(self):
+        c, p, b = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+        span, mc = ctypes.c_double(0), ctypes.c_int(0)
+        _check(self._L.pp_server_stats(self._h, ctypes.byref(c), ctypes.byref(p), ctypes.byref(b), ctypes.byref(span),
+                                       ctypes.byref(mc)), "server_stats")
+        return {"completed": c.value, "points": p.value, "batches": b.value, "span_ms": span.value,
+                "max_concurrent_batches": mc.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value is not None:
+            self._L.pp_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

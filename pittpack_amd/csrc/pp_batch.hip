@@ -2,9 +2,9 @@
 // conditions, each stage run as one launch per shape class over all members of the class.
 //
 // The reference solves one grid per PencilDcmp object (src/pencilDcmp.cpp:5801-6430); here
-// the members' fields sit back to back in one device arena and every kernel launch covers
-// the blocks of all members that share its template (line length and transform kind for
-// the x and y stages, rows per lane for the z stage).  A block finds its member through a
+// the members' fields sit back to back in device arenas and every kernel launch covers the
+// blocks of all members that share its template (line length and transform kind for the x
+// and y stages, rows per lane for the z stage).  A block finds its member through a
 // per-launch item table; per-member scalars (sizes, dz, 1/(NxNy), Dirichlet tables,
 // eigenvalue factors) come from a per-member descriptor.  Kernels and arithmetic are those
 // of the single-solve path, so a member's result does not depend on the batch around it.
@@ -12,167 +12,116 @@
 
 #include <algorithm>
 #include <cstring>
-#include <map>
 #include <new>
-#include <tuple>
 #include <vector>
 
 #include "pittpack_hip.h"
+#include "pp_batch_core.h"
 #include "pp_launch.h"
-#include "pp_mixed.h"
-#include "pp_setup.h"
-
-using namespace pp;
 
 #define PPB_CHECK(x)                                  \
     do {                                              \
         if ((x) != hipSuccess) return PP_HIP_ERROR;   \
     } while (0)
 
+namespace pp {
+
 namespace {
 
-struct BatchMember {
-    SolveSetup g;
-    long offset = 0;  // element offset of the member's field in the rhs / solution / work arenas
-    long lam_off[2] = {0, 0};
-    long dtab_off = 0;
-    // mixed form
-    bool pinned = false;  // pinned (0,0) column solved in FP64 by the plane-sum path
-    long sums_off = 0, plane_off = 0, upart_off = 0;
-};
+size_t take(size_t &cursor, size_t bytes)
+{
+    const size_t o = (cursor + 255) / 256 * 256;
+    cursor = o + bytes;
+    return o;
+}
 
-// one launch of one stage: a shape class and its block items
-struct ClassLaunch {
-    int key = 0;   // x/y: line length N; z: rows per lane J
-    int kind = 0;  // x/y: transform kind
-    int nitems = 0;
-    void *d_items = nullptr;
-};
-
-struct AxisDev {
-    double *tw = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr;
-};
+int mark(hipStream_t s, Marks *mk)
+{
+    if (mk && mk->ev && mk->n < mk->max) PPB_CHECK(hipEventRecord(mk->ev[mk->n++], s));
+    return PP_SUCCESS;
+}
 
 template <typename T>
-int upload_vec(T **dst, const std::vector<T> &v)
+int put(T *dst, const std::vector<T> &v, hipStream_t s)
 {
-    *dst = nullptr;
     if (v.empty()) return PP_SUCCESS;
-    if (hipMalloc((void **)dst, v.size() * sizeof(T)) != hipSuccess) return PP_ALLOCATION_FAIL;
-    if (hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return PP_HIP_ERROR;
+    PPB_CHECK(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     return PP_SUCCESS;
 }
 
 }  // namespace
 
-struct pp_batch_s {
-    std::vector<BatchMember> mem;
-    bool finalized = false;
-    int zero_mode = 1;
-    int precision = 0;  // 0 FP64, 1 FP32 + FP64 refinement (as PP_OPT_PRECISION)
-    int refine = 2;
-    double tau = 0.05;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    // device state built by pp_batch_finalize
-    double *d_rhs = nullptr, *d_sol = nullptr;  // right-hand sides (kept) and solutions
-    float *d_w = nullptr;                       // FP32 work arena (mixed form)
-    long arena_elems = 0;
-    double *d_lam = nullptr, *d_dtab = nullptr;
-    LineSolve *d_lsolve = nullptr;
-    ZSolve *d_zsolve = nullptr;
-    std::map<int, AxisDev> tables;  // per line length
-    std::vector<ClassLaunch> lx, ly, lz;  // FP64 launches (x, y: items sized for FP64 blocks)
-    std::vector<ClassLaunch> lx32, ly32;  // FP32 launches of the mixed form
-    // mixed form
-    double *d_sums = nullptr, *d_plane = nullptr, *d_upart = nullptr;
-    int4 *d_fix = nullptr;
-    int nfix = 0;
-    Mode00Item *d_m00 = nullptr;
-    int nm00 = 0;
-    ResArgs *d_res = nullptr;
-    int4 *d_res_items = nullptr;
-    int nres_items = 0;
-    int nz_max = 0;
-    // timing: events between the stages of the last solves
-    bool timing = false;
-    static constexpr int kMarks = 64;
-    std::vector<hipEvent_t> ev;
-    int ev_slots = 0, ev_next = 0, ev_used = 0, marks_last = 0;
-    hipEvent_t *ev_cur = nullptr;
-    int marks_cur = 0;
-};
-
-namespace {
-
-void free_device(pp_batch_s *b)
+TableCache::~TableCache()
 {
-    auto F = [](void *p) {
-        if (p) (void)hipFree(p);
-    };
-    for (void *p : {(void *)b->d_rhs, (void *)b->d_sol, (void *)b->d_w, (void *)b->d_lam, (void *)b->d_dtab,
-                    (void *)b->d_lsolve, (void *)b->d_zsolve, (void *)b->d_sums, (void *)b->d_plane,
-                    (void *)b->d_upart, (void *)b->d_fix, (void *)b->d_m00, (void *)b->d_res,
-                    (void *)b->d_res_items})
-        F(p);
-    for (auto &kv : b->tables) {
-        F(kv.second.tw);
-        F(kv.second.A);
-        F(kv.second.B);
-        F(kv.second.C);
-        F(kv.second.D);
-    }
-    b->tables.clear();
-    for (auto *v : {&b->lx, &b->ly, &b->lz, &b->lx32, &b->ly32}) {
-        for (auto &c : *v) F(c.d_items);
-        v->clear();
-    }
-    b->d_rhs = b->d_sol = nullptr;
-    b->d_w = nullptr;
-    b->d_lam = b->d_dtab = nullptr;
-    b->d_lsolve = nullptr;
-    b->d_zsolve = nullptr;
-    b->d_sums = b->d_plane = b->d_upart = nullptr;
-    b->d_fix = nullptr;
-    b->d_m00 = nullptr;
-    b->d_res = nullptr;
-    b->d_res_items = nullptr;
-    b->nfix = b->nm00 = b->nres_items = 0;
-    b->finalized = false;
+    for (auto &kv : t_)
+        for (double *p : {kv.second.tw, kv.second.A, kv.second.B, kv.second.C, kv.second.D})
+            if (p) (void)hipFree(p);
 }
 
-int mark(pp_batch_s *b)
+int TableCache::get(int N, AxisDev &out)
 {
-    if (b->ev_cur && b->marks_cur < pp_batch_s::kMarks) PPB_CHECK(hipEventRecord(b->ev_cur[b->marks_cur++], b->stream));
-    return PP_SUCCESS;
-}
-
-int table_for(pp_batch_s *b, int N)
-{
-    if (b->tables.count(N)) return PP_SUCCESS;
+    auto it = t_.find(N);
+    if (it != t_.end()) {
+        out = it->second;
+        return PP_SUCCESS;
+    }
     AxisTables t;
     make_axis_tables(N, t);
     AxisDev d;
+    auto up = [](double **dst, const std::vector<double> &v) -> int {
+        if (hipMalloc((void **)dst, v.size() * sizeof(double)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        if (hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return PP_HIP_ERROR;
+        return PP_SUCCESS;
+    };
     int rc;
-    if ((rc = upload_vec(&d.tw, t.tw)) || (rc = upload_vec(&d.A, t.A)) || (rc = upload_vec(&d.B, t.B)) ||
-        (rc = upload_vec(&d.C, t.C)) || (rc = upload_vec(&d.D, t.D)))
+    if ((rc = up(&d.tw, t.tw)) || (rc = up(&d.A, t.A)) || (rc = up(&d.B, t.B)) || (rc = up(&d.C, t.C)) ||
+        (rc = up(&d.D, t.D)))
         return rc;
-    b->tables[N] = d;
+    t_[N] = d;
+    out = d;
     return PP_SUCCESS;
+}
+
+int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int *id)
+{
+    if (!bc || !X || !id) return PP_INPUT_ARGS;
+    if (planned) return PP_CALL_ORDER;
+    const bool zok = precision == 1 ? zsolve_supported_f32(nz) : zsolve_supported_f64(nz);
+    if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
+    if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
+    if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
+    BatchMember m;
+    m.g.nx = nx;
+    m.g.ny = ny;
+    m.g.nz = nz;
+    const int rc = setup_boundary(m.g, bc);
+    if (rc) return rc;
+    for (int i = 0; i < 6; i++) m.g.X[i] = X[i];
+    setup_coords(m.g);
+    *id = (int)mem.size();
+    mem.push_back(m);
+    return PP_SUCCESS;
+}
+
+long long BatchCore::points() const
+{
+    long long p = 0;
+    for (const BatchMember &m : mem) p += (long long)m.g.nx * m.g.ny * m.g.nz;
+    return p;
 }
 
 // x (axis 0) or y (axis 1) stage: one class per (N, kind); block sizes of precision R
 template <typename R>
-int build_line_classes(pp_batch_s *b, int axis, std::vector<ClassLaunch> &out)
+int BatchCore::plan_lines(int axis, std::vector<HostClass> &out)
 {
     std::map<std::pair<int, int>, std::vector<LineItem>> cls;
-    for (int m = 0; m < (int)b->mem.size(); m++) {
-        const SolveSetup &g = b->mem[m].g;
+    for (int m = 0; m < (int)mem.size(); m++) {
+        const SolveSetup &g = mem[m].g;
         const int N = axis == 0 ? g.nx : g.ny;
-        const int kind = g.kind[axis];
         const int lines = line_block_lines<R>(N, axis);
         if (lines <= 0) return PP_UNSUPPORTED_SIZE;
-        auto &items = cls[{N, kind}];
+        auto &items = cls[{N, g.kind[axis]}];
         if (axis == 0) {
             const long nl = (long)g.ny * g.nz;
             for (long bx = 0; bx < (nl + lines - 1) / lines; bx++) items.push_back({m, (int)bx, 0});
@@ -182,165 +131,49 @@ int build_line_classes(pp_batch_s *b, int axis, std::vector<ClassLaunch> &out)
                 for (int bx = 0; bx < nbx; bx++) items.push_back({m, bx, z});
         }
     }
+    out.clear();
     for (auto &kv : cls) {
-        ClassLaunch c;
+        HostClass c;
         c.key = kv.first.first;
         c.kind = kv.first.second;
         c.nitems = (int)kv.second.size();
-        int rc = upload_vec((LineItem **)&c.d_items, kv.second);
-        if (rc) return rc;
-        out.push_back(c);
-        if ((rc = table_for(b, c.key))) return rc;
+        c.li = std::move(kv.second);
+        out.push_back(std::move(c));
     }
     return PP_SUCCESS;
 }
 
-int build_z_classes(pp_batch_s *b)
+void BatchCore::plan_z()
 {
     std::map<int, std::vector<ZItem>> cls;
-    for (int m = 0; m < (int)b->mem.size(); m++) {
-        const SolveSetup &g = b->mem[m].g;
-        const int J = (g.nz + 15) / 16;
-        auto &items = cls[J];
+    for (int m = 0; m < (int)mem.size(); m++) {
+        const SolveSetup &g = mem[m].g;
+        auto &items = cls[(g.nz + 15) / 16];
         const int nbx = (g.nx + 15) / 16;
         for (int sy = 0; sy < g.ny; sy++)
             for (int bx = 0; bx < nbx; bx++) items.push_back({m, bx, sy});
     }
+    lz_.clear();
     for (auto &kv : cls) {
-        ClassLaunch c;
+        HostClass c;
         c.key = kv.first;
         c.nitems = (int)kv.second.size();
-        int rc = upload_vec((ZItem **)&c.d_items, kv.second);
-        if (rc) return rc;
-        b->lz.push_back(c);
+        c.zi = std::move(kv.second);
+        lz_.push_back(std::move(c));
     }
-    return PP_SUCCESS;
 }
 
-// one stage of line transforms over every class: in/out arenas, precision form, flags
-int run_line_stage(pp_batch_s *b, const std::vector<ClassLaunch> &cls, int form, int axis, int fwd, const void *in,
-                   void *out, int bflags, double *sums, const double *plane, int accumulate)
+// mixed form: pinned members, near-singular columns (with the kernels' eigenvalue
+// rounding), residual launch items
+int BatchCore::plan_mixed()
 {
-    for (const ClassLaunch &c : cls) {
-        const AxisDev &t = b->tables.at(c.key);
-        LineArgs a{};
-        a.in = in;
-        a.out = out;
-        a.tw = t.tw;
-        a.A = t.A;
-        a.B = t.B;
-        a.C = t.C;
-        a.D = t.D;
-        a.in_scale = 1.0;
-        a.out_scale = 1.0;
-        a.items = (const LineItem *)c.d_items;
-        a.solves = b->d_lsolve;
-        a.nlines = c.nitems;
-        a.bflags = bflags;
-        a.line_sums = sums;
-        a.plane_add = plane;
-        a.accumulate = accumulate;
-        PPB_CHECK(launch_line(form, c.kind, c.key, axis, fwd, a, b->stream));
-    }
-    return PP_SUCCESS;
-}
-
-// z stage over every rows-per-lane class; FP32 with the mixed form's column rules
-int run_z_stage(pp_batch_s *b, bool f32)
-{
-    for (const ClassLaunch &c : b->lz) {
-        ZArgs z{};
-        z.data = f32 ? (void *)b->d_w : (void *)b->d_sol;
-        z.zero_mode = b->zero_mode;
-        z.items = (const ZItem *)c.d_items;
-        z.solves = b->d_zsolve;
-        z.nitems = c.nitems;
-        z.nz = 16 * c.key;  // selects the launch template; per-member lengths come from solves
-        if (f32) {
-            z.tau = b->tau;
-            z.pinned_zero = 1;
-            PPB_CHECK(launch_zsolve_pcr_f32(16 * c.key, z, b->stream));
-        } else {
-            PPB_CHECK(launch_zsolve_pcr_f64(16 * c.key, z, b->stream));
-        }
-    }
-    return PP_SUCCESS;
-}
-
-int run_fp64(pp_batch_s *b)
-{
-    int rc;
-    if ((rc = run_line_stage(b, b->lx, FORM_F64, 0, 1, b->d_rhs, b->d_sol, BF_RHS, nullptr, nullptr, 0))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_line_stage(b, b->ly, FORM_F64, 1, 1, b->d_sol, b->d_sol, 0, nullptr, nullptr, 0))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_z_stage(b, false))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_line_stage(b, b->ly, FORM_F64, 1, 0, b->d_sol, b->d_sol, 0, nullptr, nullptr, 0))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_line_stage(b, b->lx, FORM_F64, 0, 0, b->d_sol, b->d_sol, BF_SCALE, nullptr, nullptr, 0))) return rc;
-    return mark(b);
-}
-
-// y forward, z (FP32 + FP64 near-singular columns), y inverse on the work arena
-int mixed_inner(pp_batch_s *b)
-{
-    int rc;
-    if ((rc = run_line_stage(b, b->ly32, FORM_F32, 1, 1, b->d_w, b->d_w, 0, nullptr, nullptr, 0))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_z_stage(b, true))) return rc;
-    if (b->nfix) {
-        ZArgs z{};
-        z.data = b->d_w;
-        z.solves = b->d_zsolve;
-        PPB_CHECK(launch_zfix64_f32(z, b->d_fix, b->nfix, b->nz_max, b->stream));
-    }
-    if ((rc = mark(b))) return rc;
-    if ((rc = run_line_stage(b, b->ly32, FORM_F32, 1, 0, b->d_w, b->d_w, 0, nullptr, nullptr, 0))) return rc;
-    return mark(b);
-}
-
-// the mixed form over the batch (same steps as the single-solve handle's)
-int run_mixed(pp_batch_s *b)
-{
-    int rc;
-    double *sums = b->nm00 ? b->d_sums : nullptr;
-    const double *plane = b->nm00 ? b->d_plane : nullptr;
-    if ((rc = run_line_stage(b, b->lx32, FORM_F64_TO_F32, 0, 1, b->d_rhs, b->d_w, BF_RHS, sums, nullptr, 0))) return rc;
-    if ((rc = mark(b))) return rc;
-    if ((rc = mixed_inner(b))) return rc;
-    if (b->nm00) PPB_CHECK(launch_mode00_batch(b->d_m00, b->nm00, b->nz_max, 0, b->stream));
-    if ((rc = run_line_stage(b, b->lx32, FORM_F32_TO_F64, 0, 0, b->d_w, b->d_sol, BF_SCALE, nullptr, plane, 0)))
-        return rc;
-    if ((rc = mark(b))) return rc;
-    for (int it = 0; it < b->refine; it++) {
-        PPB_CHECK(launch_residual_f32_batch(b->d_res, b->d_res_items, b->nres_items, b->stream));
-        if ((rc = mark(b))) return rc;
-        if ((rc = run_line_stage(b, b->lx32, FORM_F32, 0, 1, b->d_w, b->d_w, 0, sums, nullptr, 0))) return rc;
-        if ((rc = mark(b))) return rc;
-        if ((rc = mixed_inner(b))) return rc;
-        if (b->nm00) PPB_CHECK(launch_mode00_batch(b->d_m00, b->nm00, b->nz_max, 1, b->stream));
-        if ((rc = run_line_stage(b, b->lx32, FORM_F32_TO_F64, 0, 0, b->d_w, b->d_sol, BF_SCALE, nullptr, plane, 1)))
-            return rc;
-        if ((rc = mark(b))) return rc;
-    }
-    return PP_SUCCESS;
-}
-
-// mixed-form tables: FP32 launch classes, work arena, pinned members, near-singular
-// columns (with the kernels' eigenvalue rounding), residual descriptors
-int build_mixed(pp_batch_s *b)
-{
-    int rc;
-    if ((rc = build_line_classes<float>(b, 0, b->lx32)) || (rc = build_line_classes<float>(b, 1, b->ly32))) return rc;
-    if (hipMalloc((void **)&b->d_w, (size_t)b->arena_elems * sizeof(float)) != hipSuccess) return PP_ALLOCATION_FAIL;
     long so = 0, po = 0, uo = 0;
-    std::vector<int4> fix;
-    std::vector<Mode00Item> m00;
-    std::vector<ResArgs> res(b->mem.size());
-    std::vector<int4> ritems;
-    for (size_t i = 0; i < b->mem.size(); i++) {
-        BatchMember &m = b->mem[i];
+    fix_.clear();
+    m00_.clear();
+    m00_mem_.clear();
+    ritems_.clear();
+    for (size_t i = 0; i < mem.size(); i++) {
+        BatchMember &m = mem[i];
         const SolveSetup &g = m.g;
         m.sums_off = so;
         so += (long)g.ny * g.nz;
@@ -356,68 +189,336 @@ int build_mixed(pp_batch_s *b)
             for (int sx = 0; sx < g.nx; sx++) {
                 const double eig = column_eig(lx, ly, sx, sy, Cx, Cy);
                 bool pin = std::fabs(eig + 2.) < 1.e-10;
-                if (pin && b->zero_mode == 1 && (g.zlo_d || g.zhi_d)) pin = false;
+                if (pin && zero_mode == 1 && (g.zlo_d || g.zhi_d)) pin = false;
                 if (pin) {
                     m.pinned = true;
                     continue;
                 }
-                if (std::fabs(eig + 2.) < b->tau) fix.push_back(make_int4(sx, sy, (int)i, 0));
+                if (std::fabs(eig + 2.) < tau) fix_.push_back(make_int4(sx, sy, (int)i, 0));
             }
-    }
-    const long nsums = so, nplane = po, nupart = uo;
-    if (hipMalloc((void **)&b->d_sums, (size_t)nsums * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&b->d_plane, (size_t)nplane * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&b->d_upart, (size_t)nupart * sizeof(double)) != hipSuccess)
-        return PP_ALLOCATION_FAIL;
-    // members without a pinned column add zeros in the inverse-x store
-    PPB_CHECK(hipMemset(b->d_plane, 0, (size_t)nplane * sizeof(double)));
-    for (size_t i = 0; i < b->mem.size(); i++) {
-        const BatchMember &m = b->mem[i];
-        const SolveSetup &g = m.g;
         if (m.pinned) {
             Mode00Item it{};
-            it.line_sums = b->d_sums + m.sums_off;
             it.ny = g.ny;
             it.nz = g.nz;
             it.b0 = g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
             it.bn = g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
             it.inv_scale = 1. / g.scale;
-            it.plane_add = b->d_plane + m.plane_off;
-            it.upart = b->d_upart + m.upart_off;
             it.nparts = residual_parts(g.nx, g.ny);
-            m00.push_back(it);
+            m00_.push_back(it);
+            m00_mem_.push_back((int)i);
         }
-        ResArgs &r = res[i];
-        r.f = b->d_rhs + m.offset;
-        r.u = b->d_sol + m.offset;
-        r.r = b->d_w + m.offset;
-        r.nx = g.nx;
-        r.ny = g.ny;
-        r.nz = g.nz;
-        r.c3 = g.dxyz[2];
-        r.Cx = (g.dxyz[2] / g.dxyz[0]) * (g.dxyz[2] / g.dxyz[0]);
-        r.Cy = (g.dxyz[2] / g.dxyz[1]) * (g.dxyz[2] / g.dxyz[1]);
-        for (int f = 0; f < 6; f++) r.bc[f] = g.bc[f];
-        r.dtab = b->d_dtab + m.dtab_off;
-        r.dmask = g.dmask;
-        r.pin_defect = m.pinned ? b->d_plane + m.plane_off + g.nz : nullptr;
-        r.upart = m.pinned ? b->d_upart + m.upart_off : nullptr;
         const int gx = (g.nx + 63) / 64, gy = (g.ny + 3) / 4, gz = (g.nz + 7) / 8;
         if ((long)gx * gy * gz != residual_blocks(g.nx, g.ny, g.nz)) return PP_INPUT_ARGS;
         for (int bz = 0; bz < gz; bz++)
             for (int by = 0; by < gy; by++)
-                for (int bx = 0; bx < gx; bx++) ritems.push_back(make_int4((int)i, bx, by, bz));
+                for (int bx = 0; bx < gx; bx++) ritems_.push_back(make_int4((int)i, bx, by, bz));
     }
-    b->nfix = (int)fix.size();
-    b->nm00 = (int)m00.size();
-    b->nres_items = (int)ritems.size();
-    if ((rc = upload_vec(&b->d_fix, fix)) || (rc = upload_vec(&b->d_m00, m00)) || (rc = upload_vec(&b->d_res, res)) ||
-        (rc = upload_vec(&b->d_res_items, ritems)))
-        return rc;
+    nsums_ = so;
+    nplane_ = po;
+    nupart_ = uo;
+    res_.assign(mem.size(), ResArgs{});
     return PP_SUCCESS;
 }
 
-}  // namespace
+int BatchCore::plan()
+{
+    if (planned) return PP_CALL_ORDER;
+    if (mem.empty()) return PP_INPUT_ARGS;
+    long off = 0, loff = 0, doff = 0;
+    lam_.clear();
+    dtab_.clear();
+    nz_max_ = 0;
+    for (BatchMember &m : mem) {
+        m.offset = off;
+        off += ((long)m.g.nx * m.g.ny * m.g.nz + 63) / 64 * 64;  // 512-byte aligned fields
+        nz_max_ = std::max(nz_max_, m.g.nz);
+        for (int a = 0; a < 2; a++) {
+            m.lam_off[a] = loff;
+            const std::vector<double> l = eig_factors(m.g, a);
+            lam_.insert(lam_.end(), l.begin(), l.end());
+            loff += (long)l.size();
+        }
+        m.dtab_off = doff;
+        const std::vector<double> d = dirichlet_table(m.g);
+        dtab_.insert(dtab_.end(), d.begin(), d.end());
+        doff += (long)d.size();
+    }
+    arena_elems = off;
+    int rc;
+    if (precision == 1) {
+        if ((rc = plan_lines<float>(0, lx_)) || (rc = plan_lines<float>(1, ly_)) || (rc = plan_mixed())) return rc;
+    } else {
+        if ((rc = plan_lines<double>(0, lx_)) || (rc = plan_lines<double>(1, ly_))) return rc;
+    }
+    plan_z();
+    launches[0] = (int)lx_.size();
+    launches[1] = (int)ly_.size();
+    launches[2] = (int)lz_.size();
+    ls_.assign(mem.size(), LineSolve{});
+    zs_.assign(mem.size(), ZSolve{});
+    // region layout
+    size_t c = 0;
+    o_rhs_ = take(c, (size_t)off * sizeof(double));
+    o_sol_ = take(c, (size_t)off * sizeof(double));
+    if (precision == 1) o_w_ = take(c, (size_t)off * sizeof(float));
+    o_lam_ = take(c, lam_.size() * sizeof(double));
+    o_dtab_ = take(c, dtab_.size() * sizeof(double));
+    o_ls_ = take(c, ls_.size() * sizeof(LineSolve));
+    o_zs_ = take(c, zs_.size() * sizeof(ZSolve));
+    for (auto *v : {&lx_, &ly_})
+        for (HostClass &h : *v) h.off = take(c, h.li.size() * sizeof(LineItem));
+    for (HostClass &h : lz_) h.off = take(c, h.zi.size() * sizeof(ZItem));
+    if (precision == 1) {
+        o_sums_ = take(c, (size_t)nsums_ * sizeof(double));
+        o_plane_ = take(c, (size_t)nplane_ * sizeof(double));
+        o_upart_ = take(c, (size_t)nupart_ * sizeof(double));
+        o_fix_ = take(c, fix_.size() * sizeof(int4));
+        o_m00_ = take(c, m00_.size() * sizeof(Mode00Item));
+        o_res_ = take(c, res_.size() * sizeof(ResArgs));
+        o_ritems_ = take(c, ritems_.size() * sizeof(int4));
+    }
+    region_bytes = (c + 255) / 256 * 256;
+    planned = true;
+    return PP_SUCCESS;
+}
+
+int BatchCore::bind(char *base, TableCache *tc)
+{
+    if (!planned || bound) return PP_CALL_ORDER;
+    d_rhs = (double *)(base + o_rhs_);
+    d_sol = (double *)(base + o_sol_);
+    d_w = precision == 1 ? (float *)(base + o_w_) : nullptr;
+    d_lam_ = (double *)(base + o_lam_);
+    d_dtab_ = (double *)(base + o_dtab_);
+    d_ls_ = (LineSolve *)(base + o_ls_);
+    d_zs_ = (ZSolve *)(base + o_zs_);
+    for (auto *v : {&lx_, &ly_, &lz_})
+        for (HostClass &h : *v) h.d_items = base + h.off;
+    int rc;
+    for (auto *v : {&lx_, &ly_})
+        for (HostClass &h : *v) {
+            AxisDev t;
+            if ((rc = tc->get(h.key, t))) return rc;
+            tabs_[h.key] = t;
+        }
+    if (precision == 1) {
+        d_sums_ = (double *)(base + o_sums_);
+        d_plane_ = (double *)(base + o_plane_);
+        d_upart_ = (double *)(base + o_upart_);
+        d_fix_ = (int4 *)(base + o_fix_);
+        d_m00_ = (Mode00Item *)(base + o_m00_);
+        d_res_ = (ResArgs *)(base + o_res_);
+        d_ritems_ = (int4 *)(base + o_ritems_);
+    }
+    for (size_t i = 0; i < mem.size(); i++) {
+        const BatchMember &m = mem[i];
+        const SolveSetup &g = m.g;
+        LineSolve &l = ls_[i];
+        l.offset = m.offset;
+        l.nx = g.nx;
+        l.ny = g.ny;
+        l.nz = g.nz;
+        l.dmask = g.dmask;
+        l.dz = g.dxyz[2];
+        l.inv_scale = 1. / g.scale;
+        l.dtab = d_dtab_ + m.dtab_off;
+        l.sums_off = m.sums_off;
+        l.plane_off = m.plane_off;
+        ZSolve &z = zs_[i];
+        z.offset = m.offset;
+        z.SX = g.nx;
+        z.SY = g.ny;
+        z.nz = g.nz;
+        z.zlo_dirichlet = g.zlo_d;
+        z.zhi_dirichlet = g.zhi_d;
+        z.Cx = (g.dxyz[2] / g.dxyz[0]) * (g.dxyz[2] / g.dxyz[0]);
+        z.Cy = (g.dxyz[2] / g.dxyz[1]) * (g.dxyz[2] / g.dxyz[1]);
+        z.lamx = d_lam_ + m.lam_off[0];
+        z.lamy = d_lam_ + m.lam_off[1];
+        if (precision == 1) {
+            ResArgs &r = res_[i];
+            r.f = d_rhs + m.offset;
+            r.u = d_sol + m.offset;
+            r.r = d_w + m.offset;
+            r.nx = g.nx;
+            r.ny = g.ny;
+            r.nz = g.nz;
+            r.c3 = g.dxyz[2];
+            r.Cx = z.Cx;
+            r.Cy = z.Cy;
+            for (int f = 0; f < 6; f++) r.bc[f] = g.bc[f];
+            r.dtab = d_dtab_ + m.dtab_off;
+            r.dmask = g.dmask;
+            r.pin_defect = m.pinned ? d_plane_ + m.plane_off + g.nz : nullptr;
+            r.upart = m.pinned ? d_upart_ + m.upart_off : nullptr;
+        }
+    }
+    for (size_t k = 0; k < m00_.size(); k++) {
+        const BatchMember &m = mem[m00_mem_[k]];
+        m00_[k].line_sums = d_sums_ + m.sums_off;
+        m00_[k].plane_add = d_plane_ + m.plane_off;
+        m00_[k].upart = d_upart_ + m.upart_off;
+    }
+    bound = true;
+    return PP_SUCCESS;
+}
+
+int BatchCore::upload(hipStream_t s)
+{
+    if (!bound) return PP_CALL_ORDER;
+    int rc;
+    if ((rc = put(d_lam_, lam_, s)) || (rc = put(d_dtab_, dtab_, s)) || (rc = put(d_ls_, ls_, s)) ||
+        (rc = put(d_zs_, zs_, s)))
+        return rc;
+    for (auto *v : {&lx_, &ly_})
+        for (HostClass &h : *v)
+            if ((rc = put((LineItem *)h.d_items, h.li, s))) return rc;
+    for (HostClass &h : lz_)
+        if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
+    if (precision == 1) {
+        // members without a pinned column add zeros in the inverse-x store
+        PPB_CHECK(hipMemsetAsync(d_plane_, 0, (size_t)nplane_ * sizeof(double), s));
+        if ((rc = put(d_fix_, fix_, s)) || (rc = put(d_m00_, m00_, s)) || (rc = put(d_res_, res_, s)) ||
+            (rc = put(d_ritems_, ritems_, s)))
+            return rc;
+    }
+    return PP_SUCCESS;
+}
+
+// one stage of line transforms over every class: in/out arenas, precision form, flags
+int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
+                              void *out, int bflags, double *sums, const double *plane, int accumulate)
+{
+    for (const HostClass &c : cls) {
+        const AxisDev &t = tabs_.at(c.key);
+        LineArgs a{};
+        a.in = in;
+        a.out = out;
+        a.tw = t.tw;
+        a.A = t.A;
+        a.B = t.B;
+        a.C = t.C;
+        a.D = t.D;
+        a.in_scale = 1.0;
+        a.out_scale = 1.0;
+        a.items = (const LineItem *)c.d_items;
+        a.solves = d_ls_;
+        a.nlines = c.nitems;
+        a.bflags = bflags;
+        a.line_sums = sums;
+        a.plane_add = plane;
+        a.accumulate = accumulate;
+        PPB_CHECK(launch_line(form, c.kind, c.key, axis, fwd, a, s));
+    }
+    return PP_SUCCESS;
+}
+
+// z stage over every rows-per-lane class; FP32 with the mixed form's column rules
+int BatchCore::run_z_stage(hipStream_t s, bool f32)
+{
+    for (const HostClass &c : lz_) {
+        ZArgs z{};
+        z.data = f32 ? (void *)d_w : (void *)d_sol;
+        z.zero_mode = zero_mode;
+        z.items = (const ZItem *)c.d_items;
+        z.solves = d_zs_;
+        z.nitems = c.nitems;
+        z.nz = 16 * c.key;  // selects the launch template; per-member lengths come from solves
+        if (f32) {
+            z.tau = tau;
+            z.pinned_zero = 1;
+            PPB_CHECK(launch_zsolve_pcr_f32(16 * c.key, z, s));
+        } else {
+            PPB_CHECK(launch_zsolve_pcr_f64(16 * c.key, z, s));
+        }
+    }
+    return PP_SUCCESS;
+}
+
+int BatchCore::run_fp64(hipStream_t s, Marks *mk)
+{
+    int rc;
+    if ((rc = run_line_stage(s, lx_, FORM_F64, 0, 1, d_rhs, d_sol, BF_RHS, nullptr, nullptr, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 1, d_sol, d_sol, 0, nullptr, nullptr, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_z_stage(s, false))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 0, d_sol, d_sol, 0, nullptr, nullptr, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_line_stage(s, lx_, FORM_F64, 0, 0, d_sol, d_sol, BF_SCALE, nullptr, nullptr, 0))) return rc;
+    return mark(s, mk);
+}
+
+// y forward, z (FP32 + FP64 near-singular columns), y inverse on the work arena
+int BatchCore::mixed_inner(hipStream_t s, Marks *mk)
+{
+    int rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F32, 1, 1, d_w, d_w, 0, nullptr, nullptr, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_z_stage(s, true))) return rc;
+    if (!fix_.empty()) {
+        ZArgs z{};
+        z.data = d_w;
+        z.solves = d_zs_;
+        PPB_CHECK(launch_zfix64_f32(z, d_fix_, (int)fix_.size(), nz_max_, s));
+    }
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F32, 1, 0, d_w, d_w, 0, nullptr, nullptr, 0))) return rc;
+    return mark(s, mk);
+}
+
+// the mixed form over the batch (same steps as the single-solve handle's)
+int BatchCore::run_mixed(hipStream_t s, Marks *mk)
+{
+    int rc;
+    const int nm00 = (int)m00_.size();
+    double *sums = nm00 ? d_sums_ : nullptr;
+    const double *plane = nm00 ? d_plane_ : nullptr;
+    if ((rc = run_line_stage(s, lx_, FORM_F64_TO_F32, 0, 1, d_rhs, d_w, BF_RHS, sums, nullptr, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    if ((rc = mixed_inner(s, mk))) return rc;
+    if (nm00) PPB_CHECK(launch_mode00_batch(d_m00_, nm00, nz_max_, 0, s));
+    if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 0))) return rc;
+    if ((rc = mark(s, mk))) return rc;
+    for (int it = 0; it < refine; it++) {
+        PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), s));
+        if ((rc = mark(s, mk))) return rc;
+        if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0))) return rc;
+        if ((rc = mark(s, mk))) return rc;
+        if ((rc = mixed_inner(s, mk))) return rc;
+        if (nm00) PPB_CHECK(launch_mode00_batch(d_m00_, nm00, nz_max_, 1, s));
+        if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 1))) return rc;
+        if ((rc = mark(s, mk))) return rc;
+    }
+    return PP_SUCCESS;
+}
+
+int BatchCore::run(hipStream_t s, Marks *mk)
+{
+    if (!bound) return PP_CALL_ORDER;
+    int rc = mark(s, mk);
+    if (rc) return rc;
+    return precision == 1 ? run_mixed(s, mk) : run_fp64(s, mk);
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+struct pp_batch_s {
+    BatchCore core;
+    TableCache tables;
+    char *region = nullptr;
+    bool finalized = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // timing: events between the launches of the last solves
+    bool timing = false;
+    static constexpr int kMarks = 64;
+    std::vector<hipEvent_t> ev;
+    int ev_slots = 0, ev_next = 0, ev_used = 0, marks_last = 0;
+};
 
 extern "C" {
 
@@ -442,7 +543,7 @@ int pp_batch_destroy(pp_batch b)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    free_device(b);
+    if (b->region) (void)hipFree(b->region);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -453,100 +554,22 @@ int pp_batch_destroy(pp_batch b)
 int pp_batch_add(pp_batch b, int nx, int ny, int nz, const char *bc, const double *X, int *id)
 {
     if (!b) return PP_INVALID_HANDLE;
-    if (!bc || !X || !id) return PP_INPUT_ARGS;
     if (b->finalized) return PP_CALL_ORDER;
-    const bool zok = b->precision == 1 ? zsolve_supported_f32(nz) : zsolve_supported_f64(nz);
-    if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
-    if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
-    if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
-    BatchMember m;
-    m.g.nx = nx;
-    m.g.ny = ny;
-    m.g.nz = nz;
-    const int rc = setup_boundary(m.g, bc);
-    if (rc) return rc;
-    for (int i = 0; i < 6; i++) m.g.X[i] = X[i];
-    setup_coords(m.g);
-    *id = (int)b->mem.size();
-    b->mem.push_back(m);
-    return PP_SUCCESS;
+    return b->core.add(nx, ny, nz, bc, X, id);
 }
 
 int pp_batch_finalize(pp_batch b)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (b->finalized) return PP_CALL_ORDER;
-    if (b->mem.empty()) return PP_INPUT_ARGS;
-    // arenas, eigenvalue factors and Dirichlet tables, packed per member
-    long off = 0, loff = 0, doff = 0;
-    std::vector<double> lam, dt;
-    b->nz_max = 0;
-    for (BatchMember &m : b->mem) {
-        m.offset = off;
-        off += ((long)m.g.nx * m.g.ny * m.g.nz + 63) / 64 * 64;  // 512-byte aligned fields
-        b->nz_max = std::max(b->nz_max, m.g.nz);
-        for (int a = 0; a < 2; a++) {
-            m.lam_off[a] = loff;
-            const std::vector<double> l = eig_factors(m.g, a);
-            lam.insert(lam.end(), l.begin(), l.end());
-            loff += (long)l.size();
-        }
-        m.dtab_off = doff;
-        const std::vector<double> d = dirichlet_table(m.g);
-        dt.insert(dt.end(), d.begin(), d.end());
-        doff += (long)d.size();
-    }
-    b->arena_elems = off;
-    int rc;
-    if (hipMalloc((void **)&b->d_rhs, (size_t)off * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&b->d_sol, (size_t)off * sizeof(double)) != hipSuccess) {
-        free_device(b);
+    int rc = b->core.plan();
+    if (rc) return rc;
+    if (hipMalloc((void **)&b->region, b->core.region_bytes) != hipSuccess) {
+        b->region = nullptr;
         return PP_ALLOCATION_FAIL;
     }
-    if ((rc = upload_vec(&b->d_lam, lam)) || (rc = upload_vec(&b->d_dtab, dt))) {
-        free_device(b);
-        return rc;
-    }
-    if (b->precision == 1 && (rc = build_mixed(b))) {
-        free_device(b);
-        return rc;
-    }
-    std::vector<LineSolve> ls(b->mem.size());
-    std::vector<ZSolve> zs(b->mem.size());
-    for (size_t i = 0; i < b->mem.size(); i++) {
-        const BatchMember &m = b->mem[i];
-        LineSolve &l = ls[i];
-        l.offset = m.offset;
-        l.nx = m.g.nx;
-        l.ny = m.g.ny;
-        l.nz = m.g.nz;
-        l.dmask = m.g.dmask;
-        l.dz = m.g.dxyz[2];
-        l.inv_scale = 1. / m.g.scale;
-        l.dtab = b->d_dtab + m.dtab_off;
-        l.sums_off = m.sums_off;
-        l.plane_off = m.plane_off;
-        ZSolve &z = zs[i];
-        z.offset = m.offset;
-        z.SX = m.g.nx;
-        z.SY = m.g.ny;
-        z.nz = m.g.nz;
-        z.zlo_dirichlet = m.g.zlo_d;
-        z.zhi_dirichlet = m.g.zhi_d;
-        z.Cx = (m.g.dxyz[2] / m.g.dxyz[0]) * (m.g.dxyz[2] / m.g.dxyz[0]);
-        z.Cy = (m.g.dxyz[2] / m.g.dxyz[1]) * (m.g.dxyz[2] / m.g.dxyz[1]);
-        z.lamx = b->d_lam + m.lam_off[0];
-        z.lamy = b->d_lam + m.lam_off[1];
-    }
-    if ((rc = upload_vec(&b->d_lsolve, ls)) || (rc = upload_vec(&b->d_zsolve, zs)) || (rc = build_z_classes(b))) {
-        free_device(b);
-        return rc;
-    }
-    if (b->precision == 0 &&
-        ((rc = build_line_classes<double>(b, 0, b->lx)) || (rc = build_line_classes<double>(b, 1, b->ly)))) {
-        free_device(b);
-        return rc;
-    }
+    if ((rc = b->core.bind(b->region, &b->tables)) || (rc = b->core.upload(b->stream))) return rc;
+    PPB_CHECK(hipStreamSynchronize(b->stream));
     b->finalized = true;
     return PP_SUCCESS;
 }
@@ -554,10 +577,8 @@ int pp_batch_finalize(pp_batch b)
 int pp_batch_count(pp_batch b, int *members, long long *points)
 {
     if (!b) return PP_INVALID_HANDLE;
-    long long p = 0;
-    for (const BatchMember &m : b->mem) p += (long long)m.g.nx * m.g.ny * m.g.nz;
-    if (members) *members = (int)b->mem.size();
-    if (points) *points = p;
+    if (members) *members = (int)b->core.mem.size();
+    if (points) *points = b->core.points();
     return PP_SUCCESS;
 }
 
@@ -566,9 +587,7 @@ int pp_batch_launch_count(pp_batch b, int *per_stage)
     if (!b) return PP_INVALID_HANDLE;
     if (!b->finalized) return PP_CALL_ORDER;
     if (!per_stage) return PP_INPUT_ARGS;
-    per_stage[0] = (int)(b->precision ? b->lx32.size() : b->lx.size());
-    per_stage[1] = (int)(b->precision ? b->ly32.size() : b->ly.size());
-    per_stage[2] = (int)b->lz.size();
+    for (int i = 0; i < 3; i++) per_stage[i] = b->core.launches[i];
     return PP_SUCCESS;
 }
 
@@ -576,8 +595,8 @@ int pp_batch_field(pp_batch b, int id, double **d_rhs)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (!b->finalized) return PP_CALL_ORDER;
-    if (id < 0 || id >= (int)b->mem.size() || !d_rhs) return PP_INPUT_ARGS;
-    *d_rhs = b->d_rhs + b->mem[id].offset;
+    if (id < 0 || id >= (int)b->core.mem.size() || !d_rhs) return PP_INPUT_ARGS;
+    *d_rhs = b->core.d_rhs + b->core.mem[id].offset;
     return PP_SUCCESS;
 }
 
@@ -585,9 +604,15 @@ int pp_batch_solution(pp_batch b, int id, double **d_u)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (!b->finalized) return PP_CALL_ORDER;
-    if (id < 0 || id >= (int)b->mem.size() || !d_u) return PP_INPUT_ARGS;
-    *d_u = b->d_sol + b->mem[id].offset;
+    if (id < 0 || id >= (int)b->core.mem.size() || !d_u) return PP_INPUT_ARGS;
+    *d_u = b->core.d_sol + b->core.mem[id].offset;
     return PP_SUCCESS;
+}
+
+static size_t member_bytes(pp_batch b, int id)
+{
+    const SolveSetup &g = b->core.mem[id].g;
+    return (size_t)g.nx * g.ny * g.nz * sizeof(double);
 }
 
 int pp_batch_set_rhs(pp_batch b, int id, const double *rhs)
@@ -596,9 +621,7 @@ int pp_batch_set_rhs(pp_batch b, int id, const double *rhs)
     int rc = pp_batch_field(b, id, &d);
     if (rc) return rc;
     if (!rhs) return PP_INPUT_ARGS;
-    const BatchMember &m = b->mem[id];
-    PPB_CHECK(hipMemcpyAsync(d, rhs, (size_t)m.g.nx * m.g.ny * m.g.nz * sizeof(double), hipMemcpyHostToDevice,
-                             b->stream));
+    PPB_CHECK(hipMemcpyAsync(d, rhs, member_bytes(b, id), hipMemcpyHostToDevice, b->stream));
     PPB_CHECK(hipStreamSynchronize(b->stream));
     return PP_SUCCESS;
 }
@@ -609,9 +632,7 @@ int pp_batch_get_solution(pp_batch b, int id, double *u)
     int rc = pp_batch_solution(b, id, &d);
     if (rc) return rc;
     if (!u) return PP_INPUT_ARGS;
-    const BatchMember &m = b->mem[id];
-    PPB_CHECK(hipMemcpyAsync(u, d, (size_t)m.g.nx * m.g.ny * m.g.nz * sizeof(double), hipMemcpyDeviceToHost,
-                             b->stream));
+    PPB_CHECK(hipMemcpyAsync(u, d, member_bytes(b, id), hipMemcpyDeviceToHost, b->stream));
     PPB_CHECK(hipStreamSynchronize(b->stream));
     return PP_SUCCESS;
 }
@@ -622,9 +643,7 @@ int pp_batch_get_rhs(pp_batch b, int id, double *rhs)
     int rc = pp_batch_field(b, id, &d);
     if (rc) return rc;
     if (!rhs) return PP_INPUT_ARGS;
-    const BatchMember &m = b->mem[id];
-    PPB_CHECK(hipMemcpyAsync(rhs, d, (size_t)m.g.nx * m.g.ny * m.g.nz * sizeof(double), hipMemcpyDeviceToHost,
-                             b->stream));
+    PPB_CHECK(hipMemcpyAsync(rhs, d, member_bytes(b, id), hipMemcpyDeviceToHost, b->stream));
     PPB_CHECK(hipStreamSynchronize(b->stream));
     return PP_SUCCESS;
 }
@@ -633,9 +652,9 @@ int pp_batch_fill_uniform(pp_batch b, unsigned long long seed, int round_f32)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (!b->finalized) return PP_CALL_ORDER;
-    for (size_t i = 0; i < b->mem.size(); i++) {
-        const BatchMember &m = b->mem[i];
-        PPB_CHECK(launch_fill_uniform(b->d_rhs + m.offset, (long)m.g.nx * m.g.ny * m.g.nz, seed + i, round_f32,
+    for (size_t i = 0; i < b->core.mem.size(); i++) {
+        const BatchMember &m = b->core.mem[i];
+        PPB_CHECK(launch_fill_uniform(b->core.d_rhs + m.offset, (long)m.g.nx * m.g.ny * m.g.nz, seed + i, round_f32,
                                       b->stream));
     }
     return PP_SUCCESS;
@@ -658,16 +677,16 @@ int pp_batch_set_option(pp_batch b, int key, long long value)
     case PP_OPT_ZERO_MODE:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         if (b->finalized) return PP_CALL_ORDER;
-        b->zero_mode = (int)value;
+        b->core.zero_mode = (int)value;
         return PP_SUCCESS;
     case PP_OPT_PRECISION:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
-        if (b->finalized || !b->mem.empty()) return PP_CALL_ORDER;
-        b->precision = (int)value;
+        if (b->finalized || !b->core.mem.empty()) return PP_CALL_ORDER;
+        b->core.precision = (int)value;
         return PP_SUCCESS;
     case PP_OPT_REFINE:
         if (value < 0 || value > 16) return PP_INPUT_ARGS;
-        b->refine = (int)value;
+        b->core.refine = (int)value;
         return PP_SUCCESS;
     case PP_OPT_TIMING: {
         if (value < 0 || value > 1024) return PP_INPUT_ARGS;
@@ -691,7 +710,7 @@ int pp_batch_set_option_double(pp_batch b, int key, double value)
     if (!b) return PP_INVALID_HANDLE;
     if (key != PP_OPT_TAU || !(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
     if (b->finalized) return PP_CALL_ORDER;
-    b->tau = value;
+    b->core.tau = value;
     return PP_SUCCESS;
 }
 
@@ -701,18 +720,15 @@ int pp_batch_solve(pp_batch b)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (!b->finalized) return PP_CALL_ORDER;
-    b->ev_cur = nullptr;
+    Marks mk;
     if (b->timing) {
-        b->ev_cur = &b->ev[(size_t)pp_batch_s::kMarks * b->ev_next];
+        mk.ev = &b->ev[(size_t)pp_batch_s::kMarks * b->ev_next];
+        mk.max = pp_batch_s::kMarks;
         b->ev_next = (b->ev_next + 1) % b->ev_slots;
         if (b->ev_used < b->ev_slots) b->ev_used++;
-        b->marks_cur = 0;
     }
-    int rc = mark(b);
-    if (rc) return rc;
-    rc = b->precision == 1 ? run_mixed(b) : run_fp64(b);
-    if (b->ev_cur) b->marks_last = b->marks_cur;
-    b->ev_cur = nullptr;
+    const int rc = b->core.run(b->stream, &mk);
+    if (mk.ev) b->marks_last = mk.n;
     return rc;
 }
 

@@ -1,0 +1,125 @@
+// pp_batch_core.h -- the engine of the batched form, shared by pp_batch (one batch, its own
+// device memory) and pp_server (many batches in flight, memory from one pool).
+//
+// A batch is planned on the host (member offsets, per-member descriptors, per-launch block
+// items), then bound to one contiguous device region, uploaded asynchronously and run on a
+// stream.  Every stage is one launch per shape class: line length and transform kind for
+// the x and y transforms, rows per lane for the z solve.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "pp_line.h"
+#include "pp_mixed.h"
+#include "pp_setup.h"
+#include "pp_zsolve.h"
+
+namespace pp {
+
+struct AxisDev {
+    double *tw = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr;
+};
+
+// FP64 twiddle tables per line length, resident on the device until destruction
+class TableCache {
+  public:
+    ~TableCache();
+    int get(int N, AxisDev &out);
+
+  private:
+    std::map<int, AxisDev> t_;
+};
+
+// events recorded between the launches of one run (null ev: no timing)
+struct Marks {
+    hipEvent_t *ev = nullptr;
+    int n = 0, max = 0;
+};
+
+struct BatchMember {
+    SolveSetup g;
+    long offset = 0;  // element offset of the member's field in the rhs / solution / work arenas
+    long lam_off[2] = {0, 0};
+    long dtab_off = 0;
+    bool pinned = false;  // mixed form: pinned (0,0) column solved in FP64 from plane sums
+    long sums_off = 0, plane_off = 0, upart_off = 0;
+};
+
+class BatchCore {
+  public:
+    // options (set before plan)
+    int precision = 0;  // 0 FP64, 1 FP32 + FP64 refinement
+    int refine = 2;
+    int zero_mode = 1;
+    double tau = 0.05;
+
+    std::vector<BatchMember> mem;
+
+    // validates and appends a member; *id = its index
+    int add(int nx, int ny, int nz, const char *bc, const double *X, int *id);
+    // host plan: offsets, descriptors, launch items; sets region_bytes
+    int plan();
+    // device pointers inside [base, base + region_bytes), table pointers from tc
+    int bind(char *base, TableCache *tc);
+    // asynchronous uploads of the planned tables (host copies stay alive in this object)
+    int upload(hipStream_t s);
+    // the solve of every member (right-hand sides kept, solutions in the solution arena)
+    int run(hipStream_t s, Marks *mk);
+
+    long long points() const;
+    size_t region_bytes = 0;
+    bool planned = false, bound = false;
+
+    // device pointers (valid after bind)
+    double *d_rhs = nullptr, *d_sol = nullptr;
+    float *d_w = nullptr;
+    long arena_elems = 0;
+    int launches[3] = {0, 0, 0};
+
+  private:
+    struct HostClass {
+        int key = 0, kind = 0, nitems = 0;
+        std::vector<LineItem> li;
+        std::vector<ZItem> zi;
+        size_t off = 0;
+        void *d_items = nullptr;
+    };
+    template <typename R>
+    int plan_lines(int axis, std::vector<HostClass> &out);
+    void plan_z();
+    int plan_mixed();
+    int run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
+                       void *out, int bflags, double *sums, const double *plane, int accumulate);
+    int run_z_stage(hipStream_t s, bool f32);
+    int run_fp64(hipStream_t s, Marks *mk);
+    int mixed_inner(hipStream_t s, Marks *mk);
+    int run_mixed(hipStream_t s, Marks *mk);
+
+    // host copies of everything uploaded
+    std::vector<double> lam_, dtab_;
+    std::vector<LineSolve> ls_;
+    std::vector<ZSolve> zs_;
+    std::vector<HostClass> lx_, ly_, lz_;
+    std::vector<int4> fix_;
+    std::vector<Mode00Item> m00_;
+    std::vector<int> m00_mem_;  // member of each m00_ entry
+    std::vector<ResArgs> res_;
+    std::vector<int4> ritems_;
+    int nz_max_ = 0;
+    long nsums_ = 0, nplane_ = 0, nupart_ = 0;
+    // region layout (byte offsets)
+    size_t o_rhs_ = 0, o_sol_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
+           o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0;
+    std::map<int, AxisDev> tabs_;
+    // device pointers of the tables
+    double *d_lam_ = nullptr, *d_dtab_ = nullptr, *d_sums_ = nullptr, *d_plane_ = nullptr, *d_upart_ = nullptr;
+    LineSolve *d_ls_ = nullptr;
+    ZSolve *d_zs_ = nullptr;
+    int4 *d_fix_ = nullptr, *d_ritems_ = nullptr;
+    Mode00Item *d_m00_ = nullptr;
+    ResArgs *d_res_ = nullptr;
+};
+
+}  // namespace pp

@@ -64,6 +64,123 @@ def cpu_baseline(n: int = 256, bc: str = BC):
     }
 
 
+XY_COMBOS = ["PPPP", "NNNN", "DDDD", "NNDD"]  # x/y faces of the batched and serving workloads
+Z_COMBOS = ["DD", "NN"]
+XY_SIDES = [32, 48, 64, 96, 128, 192]
+SERVE_SIDES = [32, 48, 64, 96, 128, 192, 256, 384, 512]  # compiled line lengths in [32, 512]
+
+
+def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8):
+    """4096 independent solves (nx, ny from {32..192}, nz in [16, 256], BC uniform over four
+    x/y combos x z in {DD, NN}, RHS U[-1,1] rounded to FP32 from seed 2), mixed form with
+    `refine` steps; the same batch in FP64 gives the error reference and its own rate."""
+    import numpy as np
+
+    r = np.random.default_rng(2)
+    members = []
+    for _ in range(count):
+        nx, ny = int(r.choice(XY_SIDES)), int(r.choice(XY_SIDES))
+        nz = int(r.integers(16, 257))
+        bc = XY_COMBOS[int(r.integers(4))] + Z_COMBOS[int(r.integers(2))]
+        members.append((nx, ny, nz, bc))
+    picks = list(range(0, count, max(1, count // sample)))[:sample]
+    out = {}
+    ref = {}
+    for prec in (0, 1):
+        B = pa.Batch(precision=prec, refine=refine if prec else None)
+        for nx, ny, nz, bc in members:
+            B.add(nx, ny, nz, bc)
+        t0 = time.perf_counter()
+        B.finalize()
+        plan_s = time.perf_counter() - t0
+        B.fill_uniform(2, round_f32=True)
+        B.solve()
+        B.synchronize()
+        B.set_option(pa.OPT_TIMING, steps)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            B.solve()
+        B.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        kt = B.kernel_times()
+        m, pts = B.count()
+        launches = B.launch_count()
+        key = "fp64" if prec == 0 else "mixed"
+        sols = {i: B.get_solution(i) for i in picks}
+        if prec == 0:
+            ref = sols
+        res = {"ms_per_step": ms, "value": pts / (ms * 1e-3), "launches_per_stage_xyz": launches,
+               "plan_and_upload_s": plan_s, "kernels_ms": kt}
+        if prec == 1:
+            errs = [float(np.abs(sols[i] - ref[i]).max() / np.abs(ref[i]).max()) for i in picks]
+            res["rel_max_diff_vs_fp64_sampled"] = max(errs)
+            res["refine"] = refine
+        out[key] = res
+        out["members"], out["points"] = m, pts
+        B.close()
+    out["workload"] = (f"{count} solves, nx,ny in {XY_SIDES}, nz in [16,256], bc x/y in {XY_COMBOS} x z in "
+                       f"{Z_COMBOS}, RHS U[-1,1] rounded to FP32, seed 2 (+member index)")
+    return out
+
+
+def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int):
+    """Poisson arrivals at `rate`/s; sides log-uniform in [32, 512] per axis (x and y rounded to
+    the nearest compiled line length, z any integer), BC uniform over four x/y combos x z in
+    {DD, NN}, precision FP64 or mixed 50/50, RHS from the device counter generator; every
+    solution is copied back to the host."""
+    import threading
+
+    import numpy as np
+
+    r = np.random.default_rng(5)
+    reqs = []
+    for i in range(n_req):
+        side = np.exp(r.uniform(np.log(32), np.log(512), 3))
+        nx = int(min(SERVE_SIDES, key=lambda v: abs(np.log(v) - np.log(side[0]))))
+        ny = int(min(SERVE_SIDES, key=lambda v: abs(np.log(v) - np.log(side[1]))))
+        nz = int(round(side[2]))
+        bc = XY_COMBOS[int(r.integers(4))] + Z_COMBOS[int(r.integers(2))]
+        reqs.append(((nz, ny, nx), bc, int(r.integers(2)), 7000 + i))
+    gaps = r.exponential(1.0 / rate, n_req)
+    S = pa.Server(hbm_bytes=int(budget_gb * (1 << 30)), staging_bytes=16 << 30, streams=streams)
+    tickets = [None] * n_req
+    ready = [threading.Event() for _ in range(n_req)]
+    lat = [0.0] * n_req
+
+    def collector():
+        for i in range(n_req):
+            ready[i].wait()
+            _, lat[i] = S.wait(tickets[i])
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, (shape, bc, prec, seed) in enumerate(reqs):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[i] = S.submit(shape, bc, precision=prec, seed=seed)
+        ready[i].set()
+    th.join()
+    wall = time.perf_counter() - t0
+    st = S.stats()
+    S.close()
+    pts = sum(s[0] * s[1] * s[2] for s, _, _, _ in reqs)
+    lat_a = np.array(lat)
+    return {
+        "requests": n_req, "rate_per_s": rate, "hbm_budget_gb": budget_gb, "streams": streams,
+        "p50_ms": float(np.percentile(lat_a, 50)), "p99_ms": float(np.percentile(lat_a, 99)),
+        "mean_ms": float(lat_a.mean()), "solves_per_s": n_req / wall, "points_per_s": pts / wall,
+        "batches": st["batches"], "max_concurrent_batches": st["max_concurrent_batches"],
+        "mean_points_per_request": pts / n_req,
+        "workload": ("Poisson arrivals; sides log-uniform in [32,512] (x, y rounded to compiled lengths), "
+                     f"bc x/y in {XY_COMBOS} x z in {Z_COMBOS}, FP64 / mixed 50/50, device-generated RHS, "
+                     "solutions copied to host; latency = submit to solution in pinned host memory"),
+    }
+
+
 def load_traffic(workload_key: str, kernel: str):
     """HBM bytes per launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     passes (scaled by the copy-kernel calibration), or None."""
@@ -86,6 +203,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas")
+    ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -188,6 +308,7 @@ def main():
         torch.neg(f, out=u)
     torch.cuda.synchronize()
     copy_gbs = 5 * 2 * pts * 8 / (time.perf_counter() - c0) / 1e9
+    M = None
 
     out = {
         "metric": "grid points solved/sec, mixed+refined vs FP64, 512^3; serving p50/p99 latency",
@@ -217,6 +338,12 @@ def main():
     }
     if mixed is not None:
         out["mixed"] = mixed
+    del f, u
+    torch.cuda.empty_cache()
+    if args.batch_count > 0:
+        out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine)
+    if args.serve_requests > 0:
+        out["serving"] = bench_serving(pa, args.serve_requests, args.serve_rate, 200.0, 4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

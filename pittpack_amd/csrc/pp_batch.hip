@@ -97,6 +97,7 @@ int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int 
     m.g.nz = nz;
     const int rc = setup_boundary(m.g, bc);
     if (rc) return rc;
+    if (m.g.zper) return PP_UNSUPPORTED_BC;  // periodic z: single-solve handle only
     for (int i = 0; i < 6; i++) m.g.X[i] = X[i];
     setup_coords(m.g);
     *id = (int)mem.size();
@@ -146,10 +147,11 @@ int BatchCore::plan_lines(int axis, std::vector<HostClass> &out)
 void BatchCore::plan_z()
 {
     std::map<int, std::vector<ZItem>> cls;
+    const int cpb = precision == 1 ? zsolve_cpb<float>() : zsolve_cpb<double>();
     for (int m = 0; m < (int)mem.size(); m++) {
         const SolveSetup &g = mem[m].g;
         auto &items = cls[(g.nz + 15) / 16];
-        const int nbx = (g.nx + 15) / 16;
+        const int nbx = (g.nx + cpb - 1) / cpb;
         for (int sy = 0; sy < g.ny; sy++)
             for (int bx = 0; bx < nbx; bx++) items.push_back({m, bx, sy});
     }

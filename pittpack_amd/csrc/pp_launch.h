@@ -18,6 +18,20 @@ bool zsolve_supported_f64(int nz);
 hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f32(int nz);
 
+// periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64; the singular column by
+// the reference's own branch
+struct ZPerArgs {
+    double *data;
+    int SX, SY, nz;
+    const double *lamx, *lamy;
+    double Cx, Cy;
+    const double *tw;
+    const double *c2;
+};
+bool zperiodic_supported(int nz);
+hipError_t launch_zperiodic_f64(const ZPerArgs &a, hipStream_t s);
+hipError_t launch_smt_v1(double *data, int SX, int SY, int nz, int sx, int sy, double diag, hipStream_t s);
+
 // counter-generator right-hand side (pp_solver.hip): out[i] = 2 u - 1,
 // u = (splitmix64(seed * 2^32 + i) >> 11) * 2^-53, optionally rounded to FP32
 hipError_t launch_fill_uniform(double *out, long n, unsigned long long seed, int round_f32, hipStream_t s);

@@ -448,6 +448,7 @@ constexpr int line_lines()
 {
     constexpr int T = line_threads(M, AX);
     constexpr int bytes = line_slots<M>() * 2 * (int)sizeof(R);
+    // y: 16 FP64 columns (128-byte rows) or up to 32 FP32 columns per block
     int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
     if (lines < 1) lines = 1;
     if (lines > 64) lines = 64;

@@ -67,6 +67,7 @@ struct SolveSetup {
     double X[6] = {0, 1, 0, 1, 0, 1};
     int kind[2] = {-1, -1};
     int zlo_d = 0, zhi_d = 0;
+    int zper = 0;  // periodic z (both z faces P)
     int dtag[3] = {0, 0, 0};
     double num[2] = {0, 0}, denum[2] = {0, 0}, freqs[2] = {kPi, kPi};
     double scale = 1.0;
@@ -81,8 +82,9 @@ inline int setup_boundary(SolveSetup &g, const char *bc)
         if (bc[i] != 'P' && bc[i] != 'N' && bc[i] != 'D') return PP_UNSUPPORTED_BC;
     const int kx = axis_kind(bc[0], bc[1]), ky = axis_kind(bc[2], bc[3]);
     if (kx < 0 || ky < 0) return PP_UNSUPPORTED_BC;
-    if (bc[4] == 'P' || bc[5] == 'P') return PP_UNSUPPORTED_BC;  // periodic z: not built yet
+    if ((bc[4] == 'P') != (bc[5] == 'P')) return PP_UNSUPPORTED_BC;  // one periodic z face
     for (int i = 0; i < 6; i++) g.bc[i] = bc[i];
+    g.zper = bc[4] == 'P';
     g.kind[0] = kx;
     g.kind[1] = ky;
     g.zlo_d = bc[4] == 'D';
@@ -162,6 +164,14 @@ inline std::vector<double> dirichlet_table(const SolveSetup &g)
     dt[o++] = exact_value(omega[2], Za, g.dtag[2]);
     dt[o++] = exact_value(omega[2], Za + (g.nz - 1) * c3 + c3, g.dtag[2]);
     return dt;
+}
+
+// periodic z: 2 cos(2 pi k / nz), k < nz (eigenvalues of the circulant z operator minus eig)
+inline std::vector<double> zper_cos2(int nz)
+{
+    std::vector<double> c(nz);
+    for (int k = 0; k < nz; k++) c[k] = 2.0 * std::cos(2.0 * kPi * (double)k / (double)nz);
+    return c;
 }
 
 // eigenvalue of column (sx, sy) with the kernels' rounding (no contraction)

@@ -95,6 +95,9 @@ struct pp_solver {
     bool own_stream = false;
     double *d_field = nullptr;
     double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
+    double *d_ztw = nullptr, *d_zc2 = nullptr;  // periodic z: twiddles and 2 cos(2 pi k / nz)
+    int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any
+    double zsing_eig = -2.0;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
     DevTables tab64[2];
     // kernel-timing ring: slot s holds up to kMaxMarks events recorded between the kernel
@@ -178,7 +181,7 @@ void free_tables(pp_solver *h)
             *p = nullptr;
         }
     }
-    for (double **p : {&h->d_lamx, &h->d_lamy, &h->d_dtab}) {
+    for (double **p : {&h->d_lamx, &h->d_lamy, &h->d_dtab, &h->d_ztw, &h->d_zc2}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -215,6 +218,23 @@ int build_tables(pp_solver *h)
     h->lamx_h = lx;
     h->lamy_h = ly;
     if ((rc = upload((void **)&h->d_dtab, dirichlet_table(h->g)))) return rc;
+    h->zsing_x = h->zsing_y = -1;
+    if (h->g.zper) {
+        AxisTables zt;
+        make_axis_tables(2 * h->g.nz, zt);  // complex FFT of length nz
+        if ((rc = upload((void **)&h->d_ztw, zt.tw))) return rc;
+        if ((rc = upload((void **)&h->d_zc2, zper_cos2(h->g.nz)))) return rc;
+        const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+        const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+        for (int sy = 0; sy < h->g.ny && h->zsing_x < 0; sy++)
+            for (int sx = 0; sx < h->g.nx; sx++)
+                if (std::fabs(column_eig(lx, ly, sx, sy, Cx, Cy) + 2.) < 1.e-10) {
+                    h->zsing_x = sx;
+                    h->zsing_y = sy;
+                    h->zsing_eig = column_eig(lx, ly, sx, sy, Cx, Cy);
+                    break;
+                }
+    }
     h->tables_ready = true;
     return PP_SUCCESS;
 }
@@ -222,8 +242,13 @@ int build_tables(pp_solver *h)
 int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
+    if (h->g.zper) {
+        // periodic z: FP64 form only, z lengths of the compiled FFTs
+        if (h->precision == 1) return PP_UNSUPPORTED_BC;
+        if (!zperiodic_supported(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+    }
     if (h->precision == 1 && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
-    if (h->precision == 0 && !zsolve_supported_f64(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 0 && !h->g.zper && !zsolve_supported_f64(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (!h->tables_ready) {
         h->mixed_ready = false;
         return build_tables(h);
@@ -294,7 +319,24 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         z.zlo_dirichlet = h->g.zlo_d;
         z.zhi_dirichlet = h->g.zhi_d;
         z.zero_mode = h->zero_mode;
-        PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
+        if (h->g.zper) {
+            ZPerArgs zp{};
+            zp.data = field;
+            zp.SX = h->g.nx;
+            zp.SY = h->g.ny;
+            zp.nz = h->g.nz;
+            zp.lamx = h->d_lamx;
+            zp.lamy = h->d_lamy;
+            zp.Cx = z.Cx;
+            zp.Cy = z.Cy;
+            zp.tw = h->d_ztw;
+            zp.c2 = h->d_zc2;
+            PP_HIP_CHECK(launch_zperiodic_f64(zp, s));
+            if (h->zsing_x >= 0)
+                PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
+        } else {
+            PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
+        }
     }
     if ((rc = mark(h))) return rc;
     // 4. inverse y
@@ -525,7 +567,7 @@ int pp_create(int nx, int ny, int nz, pp_handle *out)
     *out = nullptr;
     if (nx < 2 || ny < 2 || nz < 4) return PP_INPUT_ARGS;
     if (!size_supported(nx) || !size_supported(ny)) return PP_UNSUPPORTED_SIZE;
-    if (!zsolve_supported_f64(nz) && !zsolve_supported_f32(nz)) return PP_UNSUPPORTED_SIZE;
+    if (!zsolve_supported_f64(nz) && !zsolve_supported_f32(nz) && !zperiodic_supported(nz)) return PP_UNSUPPORTED_SIZE;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PP_NO_DEVICE;
     pp_solver *h = new (std::nothrow) pp_solver();

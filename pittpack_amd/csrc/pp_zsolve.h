@@ -256,6 +256,10 @@ __device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n
 template <int J>
 constexpr int blocked_lj() { return (J % 2 == 1) ? J : J + 1; }
 
+// columns per block of the PCR z solve: 128-byte tile rows
+template <typename R>
+constexpr int zsolve_cpb() { return (int)(128 / sizeof(R)); }
+
 template <typename R, int J, int CPB>
 constexpr size_t ztile_bytes_cyclic() { return (size_t)16 * J * (CPB + 1) * sizeof(R); }
 template <typename R, int J, int CPB>

@@ -2,32 +2,21 @@
 // per rows-per-lane J = ceil(nz / 16), so the padding below the last row stays under 16
 // rows: the kernel keeps explicit coefficients only for rows j in {0, J-2, J-1}, which
 // covers the bottom 16 real rows exactly when the padding is shorter than one lane row.
-#include <cstdlib>
 
 #include "pp_zsolve.h"
 
 namespace pp {
 
-// columns per block: 8 (64-byte rows, 4 blocks per CU) or 16 (128-byte rows);
-// PP_ZCPB in the environment overrides for measurement
-static int zcpb()
-{
-    static int v = [] {
-        const char *e = getenv("PP_ZCPB");
-        int x = e ? atoi(e) : 16;
-        return (x == 8) ? 8 : 16;
-    }();
-    return v;
-}
-
+// columns per block: 128-byte rows of the column tile (16 FP64 or 32 FP32 columns), so
+// every global row access covers whole cache lines and no line is shared by two blocks
+// (which run on different XCDs, each with its own L2)
 template <typename R, int J, int CPB>
 static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s);
 
 template <typename R, int J>
 static hipError_t launch_pcr(const ZArgs &a, hipStream_t s)
 {
-    if (zcpb() == 8) return launch_pcr_c<R, J, 8>(a, s);
-    return launch_pcr_c<R, J, 16>(a, s);
+    return launch_pcr_c<R, J, zsolve_cpb<R>()>(a, s);
 }
 
 template <typename R, int J, int CPB>

@@ -132,3 +132,40 @@ def test_call_order_and_errors():
         with pytest.raises(pa.PittPackError) as e:
             M.pittPack()
         assert e.value.code == pa.UNSUPPORTED_SIZE
+
+
+# periodic z: circulant columns by z FFT on the GPU, the singular column by the reference's
+# own branch; the oracle follows shermanMorrisonThomas[V1]
+ZPER_COMBOS = ["PPPPPP", "NNNNPP", "DDDDPP", "DDNNPP", "NDDNPP", "NNPPPP"]
+
+
+@pytest.mark.parametrize("nz", [8, 16, 24, 48, 64, 96, 128, 512])
+@pytest.mark.parametrize("bc", ZPER_COMBOS)
+def test_parity_periodic_z(bc, nz):
+    print(f"periodic z {bc} nz={nz}", flush=True)
+    rng = np.random.default_rng(nz * 7 + len(bc))
+    shape = (nz, 16, 32)
+    f = rng.uniform(-1, 1, shape)
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]
+    ref = oracle.solve(f, bc, box)
+    got = pa.solve(f, bc, box)
+    err = rel_maxdiff(got, ref)
+    assert err <= TOL, f"{bc} nz={nz}: rel max diff {err:.3e}"
+
+
+def test_periodic_z_residual_and_limits():
+    """periodic x with non-periodic y and periodic z (residual), unsupported z lengths and
+    the mixed form"""
+    print("periodic z residual", flush=True)
+    rng = np.random.default_rng(17)
+    f = rng.uniform(-1, 1, (32, 16, 32))
+    u = pa.solve(f, "PPDDPP")
+    fe = oracle.effective_rhs(f, "PPDDPP")
+    r = fe - oracle.apply_operator(u, "PPDDPP")
+    assert float(np.abs(r).max()) <= 1e-12 * max(np.abs(fe).max(), 12.0 * np.abs(u).max())
+    with pytest.raises(pa.PittPackError) as e:
+        pa.solve(rng.uniform(-1, 1, (20, 16, 32)), "NNNNPP")
+    assert e.value.code == pa.UNSUPPORTED_SIZE
+    with pytest.raises(pa.PittPackError) as e:
+        pa.solve(f, "NNNNPP", precision=1)
+    assert e.value.code == pa.UNSUPPORTED_BC

@@ -181,6 +181,45 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int):
     }
 
 
+def bench_large_mixed(pa, torch, n: int, refine: int, steps: int):
+    """mixed form at n^3, periodic x/y, Neumann z, FP32 Gaussian RHS (torch generator, seed 1)
+    with the volume mean removed; FP64 discrete residual of the result"""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(1)
+    f = torch.randn((n, n, n), generator=g, device=dev, dtype=torch.float32).double()
+    f -= f.mean()
+    u = torch.empty_like(f)
+    M = pa.PoissonHIP(n, n, n)
+    M.assignBoundary("PPPPNN")
+    M.setBox([0, 1, 0, 1, 0, 1])
+    M.setCoords(2)
+    M.set_option(pa.OPT_PRECISION, 1)
+    M.set_option(pa.OPT_REFINE, refine)
+    M.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    M.solve_device_oop(f.data_ptr(), u.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M.solve_device_oop(f.data_ptr(), u.data_ptr())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    M.close()
+    # FP64 residual f h^2 - L u (P wrap in x and y, N mirror in z), relative to max(|f h^2|, |u|)
+    r = f * (1.0 / n) ** 2
+    scale = max(float(r.abs().max()), float(u.abs().max()))
+    for ax in (1, 2):
+        r -= torch.roll(u, -1, ax) + torch.roll(u, 1, ax) - 2.0 * u
+    up = torch.cat([u[1:], u[-1:]], 0)
+    um = torch.cat([u[:1], u[:-1]], 0)
+    r -= up + um - 2.0 * u
+    res = float(r.abs().max()) / scale
+    del f, u, r, up, um
+    torch.cuda.empty_cache()
+    return {"workload": f"mixed form, {n}^3, bc PPPPNN, FP32 N(0,1) RHS (torch generator seed 1), "
+                        f"mean removed, {refine} refinement steps",
+            "ms_per_step": ms, "value": n ** 3 / (ms * 1e-3), "fp64_residual_rel": res}
+
+
 def load_traffic(workload_key: str, kernel: str):
     """HBM bytes per launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     passes (scaled by the copy-kernel calibration), or None."""
@@ -206,6 +245,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
+    ap.add_argument("--large", type=int, default=1024, help="side of the large mixed-form run (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -340,6 +380,8 @@ def main():
         out["mixed"] = mixed
     del f, u
     torch.cuda.empty_cache()
+    if args.large > 0:
+        out["large_mixed"] = bench_large_mixed(pa, torch, args.large, args.refine, max(1, min(args.steps, 5)))
     if args.batch_count > 0:
         out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine)
     if args.serve_requests > 0:

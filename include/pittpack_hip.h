@@ -55,7 +55,10 @@ enum {
     PP_OPT_PRECISION = 4,
     PP_OPT_REFINE = 5, /* refinement steps of the mixed form (default 2) */
     PP_OPT_TAU = 6,    /* pp_set_option_double: near-singular threshold (default 0.05) */
-    PP_OPT_BATCH_MEMBERS = 7 /* server: most requests per batch (default 256) */
+    PP_OPT_BATCH_MEMBERS = 7, /* server: most requests per batch (default 256) */
+    /* FP64 form: 1 (default) = the y transforms hand the z solve contiguous column tiles
+     * through a second field-sized buffer; 0 = the z solve works on the field in place */
+    PP_OPT_ZLAYOUT = 8
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

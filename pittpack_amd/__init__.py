@@ -35,6 +35,7 @@ OPT_PRECISION = 4
 OPT_REFINE = 5
 OPT_TAU = 6
 OPT_BATCH_MEMBERS = 7
+OPT_ZLAYOUT = 8
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (

@@ -75,7 +75,13 @@ struct LineArgs {
     const LineItem *items;
     const LineSolve *solves;
     int bflags;
+    // y only: z-tiled layout of the output (ZT_OUT, forward) or input (ZT_IN, inverse):
+    // element (x, y, z) at ((x/W * ny + y) * tnz + z) * W + x%W with W = columns per block and
+    // tnz >= nz rows per tile, so each (x/W, y) tile is contiguous for the z solve
+    int ztiled;
+    int tnz;
 };
+enum : int { ZT_OUT = 1, ZT_IN = 2 };
 
 struct c64 {
     double x, y;
@@ -170,6 +176,7 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
     TOut *out = reinterpret_cast<TOut *>(a.out);
 
     long base, stride;
+    long tbase = 0, tstride = 0;  // z-tiled side of a y transform
     bool valid;
     int ly = 0, lz = 0;
     if constexpr (AX == 0) {
@@ -186,12 +193,21 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
         if (!valid) col = 0;
         base = (long)by * N * a.ncols + col;
         stride = a.ncols;
+        if (a.ztiled) {
+            // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
+            tbase = ((long)bx * N * a.tnz + by) * LINES + l;
+            tstride = (long)a.tnz * LINES;
+        }
     }
 
     // ---- issue the coalesced global loads first ----
     double xin[2 * E];
+    {
+        const bool tin = (AX == 1) && (a.ztiled & ZT_IN);
+        const long ib = tin ? tbase : base, is = tin ? tstride : stride;
 #pragma unroll
-    for (int c = 0; c < 2 * E; c++) xin[c] = valid ? (double)in[base + (long)(t + T * c) * stride] : 0.0;
+        for (int c = 0; c < 2 * E; c++) xin[c] = valid ? (double)in[ib + (long)(t + T * c) * is] : 0.0;
+    }
 
     // ---- twiddle seeds (independent of the data, overlap the load latency) ----
     FFTSeeds<R, M, T> fs;
@@ -413,7 +429,8 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
                 if (a.plane_add) val += a.plane_add[lz];
                 if (a.accumulate) val += (double)out[base + (long)n * stride];
             }
-            out[base + (long)n * stride] = (TOut)val;
+            if (AX == 1 && (a.ztiled & ZT_OUT)) out[tbase + (long)n * tstride] = (TOut)val;
+            else out[base + (long)n * stride] = (TOut)val;
         }
     }
 }

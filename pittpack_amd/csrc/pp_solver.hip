@@ -96,6 +96,8 @@ struct pp_solver {
     double *d_field = nullptr;
     double *d_lamx = nullptr, *d_lamy = nullptr, *d_dtab = nullptr;
     double *d_ztw = nullptr, *d_zc2 = nullptr;  // periodic z: twiddles and 2 cos(2 pi k / nz)
+    double *d_zt = nullptr;                     // z tiles of the FP64 form
+    int ztiles = 1;                             // PP_OPT_ZLAYOUT
     int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any
     double zsing_eig = -2.0;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
@@ -285,11 +287,44 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
 }
 
 // the five stages, FP64: rhs -> field (x forward), then in place on field
+// z-tiled layout between the y transforms and the z solve: the y forward writes each
+// (x/16, y) tile of nz rows x 16 columns contiguously, so the z solve reads and writes whole
+// 64-KB-class tiles instead of one 128-byte row per z plane; used when the y transform's
+// block width equals the z solve's (16 FP64 columns) and z is not periodic
+bool use_ztiles(pp_solver *h)
+{
+    // (the partitioned solver rounds its rows per lane up to a coarser set, so it keeps the
+    // in-place layout)
+    return h->ztiles && h->zsolver == 0 && !h->g.zper && line_block_lines<double>(h->g.ny, 1) == zsolve_cpb<double>();
+}
+
+// rows per z tile: whole lane rows of the z solve (16 J), zero padding below nz
+int ztile_rows(int nz) { return (nz + 15) / 16 * 16; }
+
+int ensure_ztiles(pp_solver *h)
+{
+    if (h->d_zt) return PP_SUCCESS;
+    const size_t W = zsolve_cpb<double>();
+    const size_t elems = ((size_t)h->g.nx + W - 1) / W * W * h->g.ny * ztile_rows(h->g.nz);
+    if (hipMalloc((void **)&h->d_zt, elems * sizeof(double)) != hipSuccess) {
+        h->d_zt = nullptr;
+        return PP_ALLOCATION_FAIL;
+    }
+    // the padding (rows from nz, columns from nx) is never written by the transforms; zero
+    // keeps it finite through the z solve, which treats those rows as identity rows
+    PP_HIP_CHECK(hipMemset(h->d_zt, 0, elems * sizeof(double)));
+    return PP_SUCCESS;
+}
+
+// the five stages, FP64: rhs -> field (x forward), then on field (and the z tiles)
 int run_fp64(pp_solver *h, const double *rhs, double *field)
 {
     hipStream_t s = h->stream;
-    int rc = begin_timing(h);
-    if (rc) return rc;
+    const bool zt = use_ztiles(h);
+    int rc;
+    if (zt && (rc = ensure_ztiles(h))) return rc;
+    double *zdata = zt ? h->d_zt : field;
+    if ((rc = begin_timing(h))) return rc;
     // 1. forward x: rhs * dz^2, Dirichlet boundary term, x transform
     {
         LineArgs a = line_args(h, 0, field);
@@ -299,16 +334,20 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[0], h->g.nx, 0, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
-    // 2. forward y
+    // 2. forward y (into the z tiles)
     {
         LineArgs a = line_args(h, 1, field);
+        a.out = zdata;
+        a.ztiled = zt ? ZT_OUT : 0;
+        a.tnz = ztile_rows(h->g.nz);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     // 3. z solve
     {
         ZArgs z{};
-        z.data = field;
+        z.data = zdata;
+        z.tiled = zt ? 1 : 0;
         z.SX = h->g.nx;
         z.SY = h->g.ny;
         z.nz = h->g.nz;
@@ -339,9 +378,12 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         }
     }
     if ((rc = mark(h))) return rc;
-    // 4. inverse y
+    // 4. inverse y (from the z tiles)
     {
         LineArgs a = line_args(h, 1, field);
+        a.in = zdata;
+        a.ztiled = zt ? ZT_IN : 0;
+        a.tnz = ztile_rows(h->g.nz);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
@@ -596,6 +638,7 @@ int pp_destroy(pp_handle h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_tables(h);
     free_mixed(h);
+    if (h->d_zt) (void)hipFree(h->d_zt);
     if (h->d_field) (void)hipFree(h->d_field);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -733,6 +776,10 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_ZSOLVER:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zsolver = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_ZLAYOUT:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->ztiles = (int)value;
         return PP_SUCCESS;
     case PP_OPT_PRECISION:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;

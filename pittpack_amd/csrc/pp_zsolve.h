@@ -37,6 +37,10 @@ struct ZArgs {
     const ZItem *items;
     const ZSolve *solves;
     int nitems;
+    // z-tiled layout (written by the y forward transform): tile (x/CPB, y) holds 16 J rows
+    // (rows from nz on are zero padding) of CPB columns contiguously, element (x, y, z) at
+    // ((x/CPB * SY + y) * 16 J + z) * CPB + x%CPB
+    int tiled;
 };
 
 // one solve of a batched z launch: its field sits at `offset` in the launch's data arena
@@ -220,6 +224,14 @@ __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow 
 {
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
+    if (a.tiled) {
+        // z tiles: 16 J rows (zero padding below nz) x CPB columns, contiguous; row r of the
+        // tile is element r * CPB + c, so thread tid = g * CPB + c reads tid + q * 16 * CPB
+        const R *__restrict__ src = reinterpret_cast<const R *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+#pragma unroll
+        for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = src[q * 16 * CPB + tid];
+        return;
+    }
     const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;
@@ -239,6 +251,12 @@ __device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n
 {
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
+    if (a.tiled) {
+        R *__restrict__ dst = reinterpret_cast<R *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+#pragma unroll
+        for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = tile[lrow(g + 16 * q) * (CPB + 1) + c];
+        return;
+    }
     const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;

@@ -126,7 +126,9 @@ __device__ __forceinline__ R &lr(cpx<R> *L, int p)
 }
 
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
-__global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
+// power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
+// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers
+__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_line(LineArgs a0)
 {
     LineArgs a = a0;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -228,6 +230,25 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
     }
 
     // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
+    // Dirichlet term (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835): per line, the z- and
+    // y-face terms are ex[n] times a line constant and the x-face terms hit elements 0 and
+    // N-1 only, so the line constants are formed once and no table load sits in the element
+    // loop except ex[n] on face lines
+    double cyz = 0.0, cx0 = 0.0, cxn = 0.0;
+    const double *ex = a.dtab;
+    if constexpr (AX == 0 && FWD) {
+        if (a.dmask) {
+            const double *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
+            const double *ef = a.dtab + a.nx + a.ny + a.nz;
+            const double eyl = ey[ly], ezl = ez[lz];
+            if ((a.dmask & 16) && lz == 0) cyz += 2. * eyl * ef[4];
+            if ((a.dmask & 32) && lz == a.nz - 1) cyz += 2. * eyl * ef[5];
+            if ((a.dmask & 4) && ly == 0) cyz += 2. * ef[2] * ezl;
+            if ((a.dmask & 8) && ly == a.ny - 1) cyz += 2. * ef[3] * ezl;
+            if (a.dmask & 1) cx0 = 2. * ef[0] * eyl * ezl;
+            if (a.dmask & 2) cxn = 2. * ef[1] * eyl * ezl;
+        }
+    }
     double lsum = 0.0;
 #pragma unroll
     for (int c = 0; c < 2 * E; c++) {
@@ -236,16 +257,9 @@ __global__ void __launch_bounds__(T *LINES) k_line(LineArgs a0)
         if constexpr (AX == 0 && FWD) {
             v = v * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
             v = v * a.in_scale;
-            if (a.dmask) {
-                const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
-                const double *ef = a.dtab + a.nx + a.ny + a.nz;
-                if ((a.dmask & 16) && lz == 0) v = v - 2. * ex[n] * ey[ly] * ef[4];
-                if ((a.dmask & 32) && lz == a.nz - 1) v = v - 2 * ex[n] * ey[ly] * ef[5];
-                if ((a.dmask & 1) && n == 0) v = v - 2. * ef[0] * ey[ly] * ez[lz];
-                if ((a.dmask & 2) && n == N - 1) v = v - 2. * ef[1] * ey[ly] * ez[lz];
-                if ((a.dmask & 4) && ly == 0) v = v - 2. * ex[n] * ef[2] * ez[lz];
-                if ((a.dmask & 8) && ly == a.ny - 1) v = v - 2. * ex[n] * ef[3] * ez[lz];
-            }
+            if (cyz != 0.0) v = v - ex[n] * cyz;
+            if (n == 0) v = v - cx0;
+            if (n == N - 1) v = v - cxn;
         } else {
             v = v * a.in_scale;
         }

@@ -125,6 +125,17 @@ __device__ __forceinline__ R &lr(cpx<R> *L, int p)
     return reinterpret_cast<R *>(&L[padc(p >> 1)])[p & 1];
 }
 
+// Blocks are dealt to the 8 XCDs round-robin by linear id, and each XCD has its own L2.
+// Renumbering so that each XCD's blocks take consecutive logical indices puts neighbouring
+// tiles -- which share cache lines when a tile row is shorter than 128 bytes -- on one XCD.
+__device__ __forceinline__ int xcd_logical(int lin, int total)
+{
+    constexpr int X = 8;
+    const int q = total / X, r = total % X;
+    const int xcd = lin % X, k = lin / X;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
+}
+
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 // power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
 // VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers
@@ -132,6 +143,14 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
 {
     LineArgs a = a0;
     int bx = blockIdx.x, by = blockIdx.y;
+    if constexpr (AX == 1 && LINES * sizeof(R) < 128) {
+        // y tiles narrower than a cache line: keep the tiles of one plane on one XCD
+        if (!a0.items) {
+            const int lin = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+            bx = lin % gridDim.x;
+            by = lin / gridDim.x;
+        }
+    }
     if (a0.items) {
         const LineItem it = a0.items[blockIdx.x];
         const LineSolve sv = a0.solves[it.solve];

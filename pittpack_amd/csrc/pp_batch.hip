@@ -209,7 +209,8 @@ int BatchCore::plan_mixed()
             m00_.push_back(it);
             m00_mem_.push_back((int)i);
         }
-        const int gx = (g.nx + 63) / 64, gy = (g.ny + 3) / 4, gz = (g.nz + 7) / 8;
+        const int zp = residual_zplanes();
+        const int gx = (g.nx + 63) / 64, gy = (g.ny + 3) / 4, gz = (g.nz + zp - 1) / zp;
         if ((long)gx * gy * gz != residual_blocks(g.nx, g.ny, g.nz)) return PP_INPUT_ARGS;
         for (int bz = 0; bz < gz; bz++)
             for (int by = 0; by < gy; by++)

@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(1024) k_mode00_batch(const Mode00Item *items, 
     mode00_body(it.line_sums, it.ny, it.nz, it.b0, it.bn, it.inv_scale, it.plane_add, it.upart, it.nparts, correct);
 }
 
-constexpr int RES_ZC = 8;  // planes marched per thread
+constexpr int RES_ZC = 32;  // planes marched per thread
 
 // batched launches: block b works on items[b] = {member, bx, by, bz} with solves[member]
 template <typename TOut>
@@ -183,39 +183,68 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     TOut *r = reinterpret_cast<TOut *>(a.r);
     const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
     const double *ef = a.dtab + a.nx + a.ny + a.nz;
-    auto ghost = [](char bc, double inside, double wrap) -> double {
-        return bc == 'P' ? wrap : (bc == 'N' ? inside : -inside);
-    };
-    auto at = [&](int xx, int yy, int zz) -> double { return u[xx + sy * yy + sz * zz]; };
-    double uc = at(x, y, z0);
-    double um = (z0 > 0) ? at(x, y, z0 - 1) : ghost(a.bc[4], uc, at(x, y, a.nz - 1));
-    for (int k = 0; k < RES_ZC; k++) {
-        const int z = z0 + k;
-        if (z >= a.nz) break;
-        const double up = (z + 1 < a.nz) ? at(x, y, z + 1) : ghost(a.bc[5], uc, at(x, y, 0));
-        const double uxm = (x > 0) ? at(x - 1, y, z) : ghost(a.bc[0], uc, at(a.nx - 1, y, z));
-        const double uxp = (x + 1 < a.nx) ? at(x + 1, y, z) : ghost(a.bc[1], uc, at(0, y, z));
-        const double uym = (y > 0) ? at(x, y - 1, z) : ghost(a.bc[2], uc, at(x, a.ny - 1, z));
-        const double uyp = (y + 1 < a.ny) ? at(x, y + 1, z) : ghost(a.bc[3], uc, at(x, 0, z));
-        const double Lu = (uxp - 2.0 * uc + uxm) * a.Cx + (uyp - 2.0 * uc + uym) * a.Cy + (up - 2.0 * uc + um);
-        // effective right-hand side, same formula and order as the forward-x load
-        double v = a.f[x + sy * y + sz * z];
-        v = v * a.c3;
-        v = v * a.c3;
-        if (a.dmask) {
-            if ((a.dmask & 16) && z == 0) v = v - 2. * ex[x] * ey[y] * ef[4];
-            if ((a.dmask & 32) && z == a.nz - 1) v = v - 2 * ex[x] * ey[y] * ef[5];
-            if ((a.dmask & 1) && x == 0) v = v - 2. * ef[0] * ey[y] * ez[z];
-            if ((a.dmask & 2) && x == a.nx - 1) v = v - 2. * ef[1] * ey[y] * ez[z];
-            if ((a.dmask & 4) && y == 0) v = v - 2. * ex[x] * ef[2] * ez[z];
-            if ((a.dmask & 8) && y == a.ny - 1) v = v - 2. * ex[x] * ef[3] * ez[z];
+    // neighbour of a boundary cell through the ghost closure the spectral solve
+    // diagonalises: P wraps, N mirrors (ghost = inside), D is antisymmetric (ghost = -inside);
+    // as an index and a sign per direction, fixed for this thread's (x, y)
+    auto nb = [](char bc, int i, int wrap, long step, long &off, double &sg) {
+        if (bc == 'P') {
+            off = (long)(wrap - i) * step;
+            sg = 1.0;
+        } else {
+            off = 0;
+            sg = (bc == 'N') ? 1.0 : -1.0;
         }
+    };
+    long oxm = -1, oxp = 1, oym = -sy, oyp = sy;
+    double gxm = 1.0, gxp = 1.0, gym = 1.0, gyp = 1.0;
+    if (x == 0) nb(a.bc[0], x, a.nx - 1, 1, oxm, gxm);
+    if (x == a.nx - 1) nb(a.bc[1], x, 0, 1, oxp, gxp);
+    if (y == 0) nb(a.bc[2], y, a.ny - 1, sy, oym, gym);
+    if (y == a.ny - 1) nb(a.bc[3], y, 0, sy, oyp, gyp);
+    // Dirichlet terms of the effective right-hand side: x and y faces are a per-(x, y)
+    // constant times ez[z], z faces a constant on the first and last plane
+    double cxy = 0.0, czlo = 0.0, czhi = 0.0;
+    if (a.dmask) {
+        if ((a.dmask & 1) && x == 0) cxy += 2. * ef[0] * ey[y];
+        if ((a.dmask & 2) && x == a.nx - 1) cxy += 2. * ef[1] * ey[y];
+        if ((a.dmask & 4) && y == 0) cxy += 2. * ex[x] * ef[2];
+        if ((a.dmask & 8) && y == a.ny - 1) cxy += 2. * ex[x] * ef[3];
+        if (a.dmask & 16) czlo = 2. * ex[x] * ey[y] * ef[4];
+        if (a.dmask & 32) czhi = 2 * ex[x] * ey[y] * ef[5];
+    }
+    const double defect = a.pin_defect ? a.pin_defect[0] : 0.0;
+    const double *uc_p = u + x + sy * y;
+    const double *f_p = a.f + x + sy * y;
+    TOut *r_p = r + x + sy * y;
+    double uc = uc_p[sz * z0];
+    double um;
+    if (z0 > 0) um = uc_p[sz * (z0 - 1)];
+    else um = a.bc[4] == 'P' ? uc_p[sz * (a.nz - 1)] : (a.bc[4] == 'N' ? uc : -uc);
+    auto plane = [&](int z) {
+        const long o = sz * z;
+        double up;
+        if (z + 1 < a.nz) up = uc_p[o + sz];
+        else up = a.bc[5] == 'P' ? uc_p[0] : (a.bc[5] == 'N' ? uc : -uc);
+        const double uxm = gxm * uc_p[o + oxm], uxp = gxp * uc_p[o + oxp];
+        const double uym = gym * uc_p[o + oym], uyp = gyp * uc_p[o + oyp];
+        const double Lu = (uxp - 2.0 * uc + uxm) * a.Cx + (uyp - 2.0 * uc + uym) * a.Cy + (up - 2.0 * uc + um);
+        // effective right-hand side: rhs * dz^2 minus the Dirichlet boundary term
+        double v = f_p[o];
+        v = v * a.c3;
+        v = v * a.c3;
+        if (cxy != 0.0) v = v - cxy * ez[z];
+        if (z == 0) v = v - czlo;
+        if (z == a.nz - 1) v = v - czhi;
         double res = v - Lu;
-        if (a.pin_defect && z == 0) res += a.pin_defect[0];
-        r[x + sy * y + sz * z] = (TOut)res;
+        if (z == 0) res += defect;
+        r_p[o] = (TOut)res;
         um = uc;
         uc = up;
-    }
+    };
+    // a plain loop keeps the kernel at ~52 VGPRs (8 waves per SIMD), which hides the load
+    // latency better than unrolling the march
+    const int zend = min(z0 + RES_ZC, a.nz);
+    for (int z = z0; z < zend; z++) plane(z);
 }
 
 hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s)
@@ -244,6 +273,8 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
 }
 
 int residual_parts(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
+
+int residual_zplanes() { return RES_ZC; }
 
 int residual_blocks(int nx, int ny, int nz) { return residual_parts(nx, ny) * ((nz + RES_ZC - 1) / RES_ZC); }
 

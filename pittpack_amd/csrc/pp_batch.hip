@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "pittpack_hip.h"
@@ -219,6 +220,26 @@ int BatchCore::plan_mixed()
     nsums_ = so;
     nplane_ = po;
     nupart_ = uo;
+    // the same columns as 16-column tiles, one launch class per rows-per-lane J (the PCR
+    // kernel's explicit-row scheme needs the exact J of each member)
+    lfix_.clear();
+    {
+        std::map<int, std::vector<ZItem>> cls;
+        std::map<std::tuple<int, int, int>, bool> seen;
+        for (const int4 &c : fix_) {
+            const auto key = std::make_tuple(c.z, c.x / 16, c.y);
+            if (seen.count(key)) continue;
+            seen[key] = true;
+            cls[(mem[c.z].g.nz + 15) / 16].push_back({c.z, c.x / 16, c.y});
+        }
+        for (auto &kv : cls) {
+            HostClass h;
+            h.key = kv.first;
+            h.nitems = (int)kv.second.size();
+            h.zi = std::move(kv.second);
+            lfix_.push_back(std::move(h));
+        }
+    }
     res_.assign(mem.size(), ResArgs{});
     return PP_SUCCESS;
 }
@@ -276,6 +297,7 @@ int BatchCore::plan()
         o_plane_ = take(c, (size_t)nplane_ * sizeof(double));
         o_upart_ = take(c, (size_t)nupart_ * sizeof(double));
         o_fix_ = take(c, fix_.size() * sizeof(int4));
+        for (HostClass &h : lfix_) h.off = take(c, h.zi.size() * sizeof(ZItem));
         o_m00_ = take(c, m00_.size() * sizeof(Mode00Item));
         o_res_ = take(c, res_.size() * sizeof(ResArgs));
         o_ritems_ = take(c, ritems_.size() * sizeof(int4));
@@ -309,6 +331,7 @@ int BatchCore::bind(char *base, TableCache *tc)
         d_plane_ = (double *)(base + o_plane_);
         d_upart_ = (double *)(base + o_upart_);
         d_fix_ = (int4 *)(base + o_fix_);
+        for (HostClass &h : lfix_) h.d_items = base + h.off;
         d_m00_ = (Mode00Item *)(base + o_m00_);
         d_res_ = (ResArgs *)(base + o_res_);
         d_ritems_ = (int4 *)(base + o_ritems_);
@@ -381,6 +404,8 @@ int BatchCore::upload(hipStream_t s)
     if (precision == 1) {
         // members without a pinned column add zeros in the inverse-x store
         PPB_CHECK(hipMemsetAsync(d_plane_, 0, (size_t)nplane_ * sizeof(double), s));
+        for (HostClass &h : lfix_)
+            if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
         if ((rc = put(d_fix_, fix_, s)) || (rc = put(d_m00_, m00_, s)) || (rc = put(d_res_, res_, s)) ||
             (rc = put(d_ritems_, ritems_, s)))
             return rc;
@@ -464,7 +489,18 @@ int BatchCore::mixed_inner(hipStream_t s, Marks *mk)
         ZArgs z{};
         z.data = d_w;
         z.solves = d_zs_;
-        PPB_CHECK(launch_zfix64_f32(z, d_fix_, (int)fix_.size(), nz_max_, s));
+        z.zero_mode = zero_mode;
+        z.tau = tau;
+        z.pinned_zero = 1;
+        if (zfix_tiles_supported(nz_max_)) {
+            for (const HostClass &h : lfix_) {
+                z.items = (const ZItem *)h.d_items;
+                z.nitems = h.nitems;
+                PPB_CHECK(launch_zfix_tiles_f32(16 * h.key, z, s));
+            }
+        } else {
+            PPB_CHECK(launch_zfix64_f32(z, d_fix_, (int)fix_.size(), nz_max_, s));
+        }
     }
     if ((rc = mark(s, mk))) return rc;
     if ((rc = run_line_stage(s, ly_, FORM_F32, 1, 0, d_w, d_w, 0, nullptr, nullptr, 0))) return rc;

@@ -103,6 +103,7 @@ class BatchCore {
     std::vector<ZSolve> zs_;
     std::vector<HostClass> lx_, ly_, lz_;
     std::vector<int4> fix_;
+    std::vector<HostClass> lfix_;  // 16-column tiles holding near-singular columns, per J
     std::vector<Mode00Item> m00_;
     std::vector<int> m00_mem_;  // member of each m00_ entry
     std::vector<ResArgs> res_;
@@ -118,6 +119,7 @@ class BatchCore {
     LineSolve *d_ls_ = nullptr;
     ZSolve *d_zs_ = nullptr;
     int4 *d_fix_ = nullptr, *d_ritems_ = nullptr;
+
     Mode00Item *d_m00_ = nullptr;
     ResArgs *d_res_ = nullptr;
 };

@@ -17,6 +17,10 @@ hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f64(int nz);
 hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f32(int nz);
+// FP64 re-solve of the near-singular columns of FP32 data, one block per listed 16-column
+// tile (a.items over a.solves); columns up to 512 rows
+bool zfix_tiles_supported(int nz);
+hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s);
 
 // periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64; the singular column by
 // the reference's own branch

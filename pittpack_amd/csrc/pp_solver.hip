@@ -121,6 +121,9 @@ struct pp_solver {
     double *d_upart = nullptr;     // residual kernel's plane-0 partial sums of u
     int4 *d_fix = nullptr;         // near-singular columns solved in FP64: {sx, sy, 0, 0}
     int nfix = 0;
+    ZItem *d_fixtiles = nullptr;   // the 16-column tiles holding them
+    int nfixtiles = 0;
+    ZSolve *d_zself = nullptr;     // this handle as a one-member batch (tile items read it)
     bool pinned00 = false;
     bool mixed_ready = false;
 };
@@ -166,6 +169,11 @@ void free_mixed(pp_solver *h)
     if (h->d_plane_add) (void)hipFree(h->d_plane_add);
     if (h->d_upart) (void)hipFree(h->d_upart);
     if (h->d_fix) (void)hipFree(h->d_fix);
+    if (h->d_fixtiles) (void)hipFree(h->d_fixtiles);
+    if (h->d_zself) (void)hipFree(h->d_zself);
+    h->d_fixtiles = nullptr;
+    h->d_zself = nullptr;
+    h->nfixtiles = 0;
     h->d_w = nullptr;
     h->d_aux = nullptr;
     h->d_line_sums = h->d_plane_add = h->d_upart = nullptr;
@@ -463,6 +471,41 @@ int ensure_mixed(pp_solver *h)
         if (hipMalloc((void **)&h->d_fix, fix.size() * sizeof(int4)) != hipSuccess) return PP_ALLOCATION_FAIL;
         PP_HIP_CHECK(hipMemcpy(h->d_fix, fix.data(), fix.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
+    // the same columns as 16-column tiles for the FP64 tile re-solve
+    if (h->d_fixtiles) {
+        PP_HIP_CHECK(hipFree(h->d_fixtiles));
+        h->d_fixtiles = nullptr;
+    }
+    std::vector<ZItem> tiles;
+    for (const int4 &c : fix) {
+        const ZItem it = {0, c.x / 16, c.y};
+        if (tiles.empty() || tiles.back().bx != it.bx || tiles.back().by != it.by) {
+            bool seen = false;
+            for (const ZItem &o : tiles) seen = seen || (o.bx == it.bx && o.by == it.by);
+            if (!seen) tiles.push_back(it);
+        }
+    }
+    h->nfixtiles = (int)tiles.size();
+    if (h->nfixtiles) {
+        if (hipMalloc((void **)&h->d_fixtiles, tiles.size() * sizeof(ZItem)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_fixtiles, tiles.data(), tiles.size() * sizeof(ZItem), hipMemcpyHostToDevice));
+    }
+    {
+        // rewritten whenever the tables change (it points at the eigenvalue factors)
+        ZSolve zs{};
+        zs.offset = 0;
+        zs.SX = h->g.nx;
+        zs.SY = h->g.ny;
+        zs.nz = h->g.nz;
+        zs.zlo_dirichlet = h->g.zlo_d;
+        zs.zhi_dirichlet = h->g.zhi_d;
+        zs.Cx = Cx;
+        zs.Cy = Cy;
+        zs.lamx = h->d_lamx;
+        zs.lamy = h->d_lamy;
+        if (!h->d_zself && hipMalloc((void **)&h->d_zself, sizeof(ZSolve)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_zself, &zs, sizeof(ZSolve), hipMemcpyHostToDevice));
+    }
     h->mixed_ready = true;
     return PP_SUCCESS;
 }
@@ -486,7 +529,15 @@ int mixed_inner(pp_solver *h)
         z.pinned_zero = 1;
         PP_HIP_CHECK(launch_zsolve_pcr_f32(h->g.nz, z, s));
         if ((rc = mark(h))) return rc;
-        PP_HIP_CHECK(launch_zfix64_f32(z, h->d_fix, h->nfix, h->g.nz, s));
+        if (zfix_tiles_supported(h->g.nz)) {
+            ZArgs zf = z;
+            zf.items = h->d_fixtiles;
+            zf.solves = h->d_zself;
+            zf.nitems = h->nfixtiles;
+            PP_HIP_CHECK(launch_zfix_tiles_f32(h->g.nz, zf, s));
+        } else {
+            PP_HIP_CHECK(launch_zfix64_f32(z, h->d_fix, h->nfix, h->g.nz, s));
+        }
     }
     if ((rc = mark(h))) return rc;
     {

@@ -219,7 +219,7 @@ __device__ __forceinline__ void pcr_step(R *d, Row3<R> *ex, R &ta, R &tb, int t)
 // block (16: 128-byte rows; 8: 64-byte rows, half the LDS per block).
 // ---------------------------------------------------------------------------------------
 
-template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
+template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow, typename S = R>
 __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow lrow, int bx, int by)
 {
     const int tid = threadIdx.x;
@@ -227,45 +227,45 @@ __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow 
     if (a.tiled) {
         // z tiles: 16 J rows (zero padding below nz) x CPB columns, contiguous; row r of the
         // tile is element r * CPB + c, so thread tid = g * CPB + c reads tid + q * 16 * CPB
-        const R *__restrict__ src = reinterpret_cast<const R *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+        const S *__restrict__ src = reinterpret_cast<const S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
 #pragma unroll
-        for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = src[q * 16 * CPB + tid];
+        for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[q * 16 * CPB + tid];
         return;
     }
     const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;
     const long lane_elem = (long)by * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
-    const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
+    const ColPtr<S, OFF32> p(reinterpret_cast<S *>(a.data), 16 * zs, lane_elem);
 #pragma unroll
     for (int q = 0; q < J; q++) {
         const int r = g + 16 * q;
         R v = (R)0;
-        if (cvalid && (FULL || r < n)) v = *p.at(q);
+        if (cvalid && (FULL || r < n)) v = (R)*p.at(q);
         tile[lrow(r) * (CPB + 1) + c] = v;
     }
 }
 
-template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow>
+template <typename R, int J, bool FULL, bool OFF32, int CPB, typename LRow, typename S = R>
 __device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n, LRow lrow, int bx, int by)
 {
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
     if (a.tiled) {
-        R *__restrict__ dst = reinterpret_cast<R *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+        S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
 #pragma unroll
-        for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = tile[lrow(g + 16 * q) * (CPB + 1) + c];
+        for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];
         return;
     }
     const int col0 = bx * CPB;
     const bool cvalid = col0 + c < a.SX;
     const long zs = (long)a.SX * a.SY;
     const long lane_elem = (long)by * a.SX + (cvalid ? col0 + c : 0) + (long)g * zs;
-    const ColPtr<R, OFF32> p(reinterpret_cast<R *>(a.data), 16 * zs, lane_elem);
+    const ColPtr<S, OFF32> p(reinterpret_cast<S *>(a.data), 16 * zs, lane_elem);
 #pragma unroll
     for (int q = 0; q < J; q++) {
         const int r = g + 16 * q;
-        if (cvalid && (FULL || r < n)) *p.at(q) = tile[lrow(r) * (CPB + 1) + c];
+        if (cvalid && (FULL || r < n)) *p.at(q) = (S)tile[lrow(r) * (CPB + 1) + c];
     }
 }
 
@@ -283,8 +283,11 @@ constexpr size_t ztile_bytes_cyclic() { return (size_t)16 * J * (CPB + 1) * size
 template <typename R, int J, int CPB>
 constexpr size_t ztile_bytes_blocked() { return (size_t)16 * blocked_lj<J>() * (CPB + 1) * sizeof(R); }
 
-// FULL: nz == 16 * J (no padding rows, no per-row bounds checks)
-template <typename R, int J, bool FULL, bool OFF32, int CPB = 16>
+// FULL: nz == 16 * J (no padding rows, no per-row bounds checks).  S: element type in
+// memory (FP32 data solved in FP64 when R = double, S = float).  FIX: solve and write back
+// only the near-singular columns (|eig+2| < tau, not pinned) that the FP32 solve left
+// untouched; the tile's other columns are stored back unchanged.
+template <typename R, int J, bool FULL, bool OFF32, int CPB = 16, typename S = R, bool FIX = false>
 __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
 {
     constexpr bool ALL = (J <= 3);
@@ -294,7 +297,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
     if (a0.items) {
         const ZItem it = a0.items[blockIdx.x];
         const ZSolve sv = a0.solves[it.solve];
-        a.data = reinterpret_cast<R *>(a0.data) + sv.offset;
+        a.data = reinterpret_cast<S *>(a0.data) + sv.offset;
         a.SX = sv.SX;
         a.SY = sv.SY;
         a.nz = sv.nz;
@@ -337,7 +340,7 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
     extern __shared__ __align__(16) unsigned char zsm[];
     R *tile = reinterpret_cast<R *>(zsm);
     auto lrow = [](int r) { return r; };
-    ztile_load<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, bx, by);
+    ztile_load<R, J, FULL, OFF32, CPB, decltype(lrow), S>(tile, a, n, lrow, bx, by);
     __syncthreads();
     R d[J];
 #pragma unroll
@@ -385,14 +388,15 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
 #pragma unroll
     for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
 
-    const bool skip = !pin && fabs(eig + 2.) < a.tau;
+    const bool near = !pin && fabs(eig + 2.) < a.tau;
+    const bool skip = FIX ? !near : near;
     const bool zero = pin && a.pinned_zero;
     if (!skip) {
 #pragma unroll
         for (int j = 0; j < J; j++) tile[(t + 16 * j) * (CPB + 1) + c] = zero ? (R)0 : d[j];
     }
     __syncthreads();
-    ztile_store<R, J, FULL, OFF32, CPB>(tile, a, n, lrow, bx, by);
+    ztile_store<R, J, FULL, OFF32, CPB, decltype(lrow), S>(tile, a, n, lrow, bx, by);
     (void)valid;
     (void)data;
 }

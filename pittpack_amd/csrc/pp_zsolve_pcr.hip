@@ -73,6 +73,42 @@ hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s)
 
 bool zsolve_supported_f32(int nz) { return nz >= 4 && nz <= 16 * 64; }
 
+// FP64 re-solve of the near-singular columns of FP32 data: one block per listed 16-column
+// tile (a.items / a.solves / a.nitems), FP64 PCR -> Thomas, only those columns written
+template <int J>
+static hipError_t launch_fix(const ZArgs &a, hipStream_t s)
+{
+    constexpr int CPB = 16;
+    constexpr size_t lds = ztile_bytes_cyclic<double, J, CPB>();
+    auto kern = k_zsolve<double, J, false, true, CPB, float, true>;
+    if (lds > 64 * 1024) {
+        static bool attr = false;
+        if (!attr) {
+            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            attr = true;
+        }
+    }
+    if (a.nitems == 0) return hipSuccess;
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.nitems), dim3(16 * CPB), lds, s, a);
+    return hipGetLastError();
+}
+
+bool zfix_tiles_supported(int nz) { return nz >= 4 && nz <= 16 * 32; }
+
+hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s)
+{
+    switch ((nz_max + 15) / 16) {
+#define PPF(v) \
+    case v: return launch_fix<v>(a, s);
+        PPF(1) PPF(2) PPF(3) PPF(4) PPF(5) PPF(6) PPF(7) PPF(8) PPF(9) PPF(10) PPF(11) PPF(12) PPF(13) PPF(14) PPF(15)
+        PPF(16) PPF(17) PPF(18) PPF(19) PPF(20) PPF(21) PPF(22) PPF(23) PPF(24) PPF(25) PPF(26) PPF(27) PPF(28)
+        PPF(29) PPF(30) PPF(31) PPF(32)
+#undef PPF
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s)
 {
     switch ((nz + 15) / 16) {

@@ -92,6 +92,9 @@ int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int 
     if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
     if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
+    // FP64 members hand the z solve 16-column tiles, which the y transform forms for y lines
+    // of up to 512 cells
+    if (precision == 0 && line_block_lines<double>(ny, 1) != zsolve_cpb<double>()) return PP_UNSUPPORTED_SIZE;
     BatchMember m;
     m.g.nx = nx;
     m.g.ny = ny;
@@ -268,6 +271,15 @@ int BatchCore::plan()
         doff += (long)d.size();
     }
     arena_elems = off;
+    // FP64: z tiles (16 columns x 16 J rows, zero padding) between the y transforms and the
+    // z solve, as in the single-solve handle
+    ntiles_ = 0;
+    if (precision == 0) {
+        for (BatchMember &m : mem) {
+            m.toff = ntiles_;
+            ntiles_ += (long)((m.g.nx + 15) / 16) * 16 * m.g.ny * ((m.g.nz + 15) / 16) * 16;
+        }
+    }
     int rc;
     if (precision == 1) {
         if ((rc = plan_lines<float>(0, lx_)) || (rc = plan_lines<float>(1, ly_)) || (rc = plan_mixed())) return rc;
@@ -284,6 +296,7 @@ int BatchCore::plan()
     size_t c = 0;
     o_rhs_ = take(c, (size_t)off * sizeof(double));
     o_sol_ = take(c, (size_t)off * sizeof(double));
+    if (precision == 0) o_tiles_ = take(c, (size_t)ntiles_ * sizeof(double));
     if (precision == 1) o_w_ = take(c, (size_t)off * sizeof(float));
     o_lam_ = take(c, lam_.size() * sizeof(double));
     o_dtab_ = take(c, dtab_.size() * sizeof(double));
@@ -312,6 +325,7 @@ int BatchCore::bind(char *base, TableCache *tc)
     if (!planned || bound) return PP_CALL_ORDER;
     d_rhs = (double *)(base + o_rhs_);
     d_sol = (double *)(base + o_sol_);
+    d_tiles = precision == 0 ? (double *)(base + o_tiles_) : nullptr;
     d_w = precision == 1 ? (float *)(base + o_w_) : nullptr;
     d_lam_ = (double *)(base + o_lam_);
     d_dtab_ = (double *)(base + o_dtab_);
@@ -350,8 +364,10 @@ int BatchCore::bind(char *base, TableCache *tc)
         l.dtab = d_dtab_ + m.dtab_off;
         l.sums_off = m.sums_off;
         l.plane_off = m.plane_off;
+        l.toffset = m.toff;
+        l.tnz = (g.nz + 15) / 16 * 16;
         ZSolve &z = zs_[i];
-        z.offset = m.offset;
+        z.offset = precision == 0 ? m.toff : m.offset;  // FP64 solves the z tiles
         z.SX = g.nx;
         z.SY = g.ny;
         z.nz = g.nz;
@@ -393,6 +409,8 @@ int BatchCore::upload(hipStream_t s)
 {
     if (!bound) return PP_CALL_ORDER;
     int rc;
+    // tile padding (rows from nz, columns from nx) is never written by the transforms
+    if (precision == 0) PPB_CHECK(hipMemsetAsync(d_tiles, 0, (size_t)ntiles_ * sizeof(double), s));
     if ((rc = put(d_lam_, lam_, s)) || (rc = put(d_dtab_, dtab_, s)) || (rc = put(d_ls_, ls_, s)) ||
         (rc = put(d_zs_, zs_, s)))
         return rc;
@@ -415,7 +433,7 @@ int BatchCore::upload(hipStream_t s)
 
 // one stage of line transforms over every class: in/out arenas, precision form, flags
 int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
-                              void *out, int bflags, double *sums, const double *plane, int accumulate)
+                              void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled)
 {
     for (const HostClass &c : cls) {
         const AxisDev &t = tabs_.at(c.key);
@@ -436,6 +454,7 @@ int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int fo
         a.line_sums = sums;
         a.plane_add = plane;
         a.accumulate = accumulate;
+        a.ztiled = ztiled;
         PPB_CHECK(launch_line(form, c.kind, c.key, axis, fwd, a, s));
     }
     return PP_SUCCESS;
@@ -446,7 +465,8 @@ int BatchCore::run_z_stage(hipStream_t s, bool f32)
 {
     for (const HostClass &c : lz_) {
         ZArgs z{};
-        z.data = f32 ? (void *)d_w : (void *)d_sol;
+        z.data = f32 ? (void *)d_w : (void *)d_tiles;
+        z.tiled = f32 ? 0 : 1;
         z.zero_mode = zero_mode;
         z.items = (const ZItem *)c.d_items;
         z.solves = d_zs_;
@@ -468,11 +488,11 @@ int BatchCore::run_fp64(hipStream_t s, Marks *mk)
     int rc;
     if ((rc = run_line_stage(s, lx_, FORM_F64, 0, 1, d_rhs, d_sol, BF_RHS, nullptr, nullptr, 0))) return rc;
     if ((rc = mark(s, mk))) return rc;
-    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 1, d_sol, d_sol, 0, nullptr, nullptr, 0))) return rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 1, d_sol, d_tiles, 0, nullptr, nullptr, 0, ZT_OUT))) return rc;
     if ((rc = mark(s, mk))) return rc;
     if ((rc = run_z_stage(s, false))) return rc;
     if ((rc = mark(s, mk))) return rc;
-    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 0, d_sol, d_sol, 0, nullptr, nullptr, 0))) return rc;
+    if ((rc = run_line_stage(s, ly_, FORM_F64, 1, 0, d_tiles, d_sol, 0, nullptr, nullptr, 0, ZT_IN))) return rc;
     if ((rc = mark(s, mk))) return rc;
     if ((rc = run_line_stage(s, lx_, FORM_F64, 0, 0, d_sol, d_sol, BF_SCALE, nullptr, nullptr, 0))) return rc;
     return mark(s, mk);

@@ -45,6 +45,7 @@ struct BatchMember {
     long dtab_off = 0;
     bool pinned = false;  // mixed form: pinned (0,0) column solved in FP64 from plane sums
     long sums_off = 0, plane_off = 0, upart_off = 0;
+    long toff = 0;  // FP64: offset of the member's z tiles in the tile arena
 };
 
 class BatchCore {
@@ -74,6 +75,7 @@ class BatchCore {
 
     // device pointers (valid after bind)
     double *d_rhs = nullptr, *d_sol = nullptr;
+    double *d_tiles = nullptr;  // FP64: z tiles between the y transforms and the z solve
     float *d_w = nullptr;
     long arena_elems = 0;
     int launches[3] = {0, 0, 0};
@@ -91,7 +93,7 @@ class BatchCore {
     void plan_z();
     int plan_mixed();
     int run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
-                       void *out, int bflags, double *sums, const double *plane, int accumulate);
+                       void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled = 0);
     int run_z_stage(hipStream_t s, bool f32);
     int run_fp64(hipStream_t s, Marks *mk);
     int mixed_inner(hipStream_t s, Marks *mk);
@@ -109,9 +111,9 @@ class BatchCore {
     std::vector<ResArgs> res_;
     std::vector<int4> ritems_;
     int nz_max_ = 0;
-    long nsums_ = 0, nplane_ = 0, nupart_ = 0;
+    long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)
-    size_t o_rhs_ = 0, o_sol_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
+    size_t o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
            o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0;
     std::map<int, AxisDev> tabs_;
     // device pointers of the tables

@@ -41,6 +41,8 @@ struct LineSolve {
     const double *dtab;
     long sums_off;   // offset of the solve's ny*nz line sums in line_sums
     long plane_off;  // offset of the solve's nz (+2) plane values in plane_add
+    long toffset;    // z-tiled launches: offset of the solve's z tiles on the tiled side
+    int tnz;         // rows per z tile
 };
 struct LineItem {
     int solve, bx, by;
@@ -154,8 +156,9 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
     if (a0.items) {
         const LineItem it = a0.items[blockIdx.x];
         const LineSolve sv = a0.solves[it.solve];
-        a.in = reinterpret_cast<const TIn *>(a0.in) + sv.offset;
-        a.out = reinterpret_cast<TOut *>(a0.out) + sv.offset;
+        a.in = reinterpret_cast<const TIn *>(a0.in) + ((a0.ztiled & ZT_IN) ? sv.toffset : sv.offset);
+        a.out = reinterpret_cast<TOut *>(a0.out) + ((a0.ztiled & ZT_OUT) ? sv.toffset : sv.offset);
+        a.tnz = sv.tnz;
         a.nx = sv.nx;
         a.ny = sv.ny;
         a.nz = sv.nz;

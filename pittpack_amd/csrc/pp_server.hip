@@ -155,7 +155,7 @@ namespace {
 size_t estimate_bytes(const Request *r)
 {
     const size_t pts = (size_t)r->points();
-    return pts * (r->precision ? 20 : 16) + 65536;
+    return pts * (r->precision ? 20 : 24) + 65536;
 }
 
 void fail_request(pp_server_s *s, Request *r, int rc)

@@ -241,7 +241,7 @@ def main():
     ap.add_argument("--refine", type=int, default=2, help="refinement steps of the mixed form")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas")
+    ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas, 2 multigrid")
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")

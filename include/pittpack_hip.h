@@ -46,7 +46,10 @@ enum {
      * 0 switches timing off */
     PP_OPT_TIMING = 2,
     /* z solver: 0 = PCR across 16 lanes then Thomas per lane (cyclic rows);
-     *           1 = partitioned Thomas, contiguous rows per lane, cross-lane scans */
+     *           1 = partitioned Thomas, contiguous rows per lane, cross-lane scans;
+     *           2 = multigrid V-cycles (the reference's MultiGrid alternative) to a
+     *               max-norm residual of PP_OPT_MG_TOL * max|rhs| (default 1e-16), or until
+     *               a V-cycle reduces it by less than 10% (its rounding floor) */
     PP_OPT_ZSOLVER = 3,
     /* 0 (default): FP64 throughout.  1: mixed form -- FP32 transforms and z solves with
      * FP64 twiddles and eigenvalues, FP64 solves of the near-singular columns
@@ -58,7 +61,8 @@ enum {
     PP_OPT_BATCH_MEMBERS = 7, /* server: most requests per batch (default 256) */
     /* FP64 form: 1 (default) = the y transforms hand the z solve contiguous column tiles
      * through a second field-sized buffer; 0 = the z solve works on the field in place */
-    PP_OPT_ZLAYOUT = 8
+    PP_OPT_ZLAYOUT = 8,
+    PP_OPT_MG_TOL = 9 /* pp_set_option_double: multigrid residual tolerance */
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

@@ -36,6 +36,7 @@ OPT_REFINE = 5
 OPT_TAU = 6
 OPT_BATCH_MEMBERS = 7
 OPT_ZLAYOUT = 8
+OPT_MG_TOL = 9
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (

@@ -98,6 +98,8 @@ struct pp_solver {
     double *d_ztw = nullptr, *d_zc2 = nullptr;  // periodic z: twiddles and 2 cos(2 pi k / nz)
     double *d_zt = nullptr;                     // z tiles of the FP64 form
     int ztiles = 1;                             // PP_OPT_ZLAYOUT
+    int mg_cycles = 200;                        // multigrid z solver: V-cycle limit
+    double mg_tol = 1e-16;                      //   and relative max-norm residual tolerance
     int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any
     double zsing_eig = -2.0;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
@@ -382,7 +384,8 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
             if (h->zsing_x >= 0)
                 PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
         } else {
-            PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
+            if (h->zsolver == 2) PP_HIP_CHECK(launch_zsolve_mg_f64(z, h->mg_cycles, h->mg_tol, s));
+            else PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
         }
     }
     if ((rc = mark(h))) return rc;
@@ -825,7 +828,7 @@ int pp_set_option(pp_handle h, int key, long long value)
         h->mixed_ready = false;
         return PP_SUCCESS;
     case PP_OPT_ZSOLVER:
-        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        if (value < 0 || value > 2) return PP_INPUT_ARGS;
         h->zsolver = (int)value;
         return PP_SUCCESS;
     case PP_OPT_ZLAYOUT:
@@ -896,6 +899,11 @@ int pp_get_stage_times(pp_handle h, float *ms5)
 int pp_set_option_double(pp_handle h, int key, double value)
 {
     if (!h) return PP_INVALID_HANDLE;
+    if (key == PP_OPT_MG_TOL) {
+        if (!(value > 0.0 && value < 1.0)) return PP_INPUT_ARGS;
+        h->mg_tol = value;
+        return PP_SUCCESS;
+    }
     if (key == PP_OPT_TAU) {
         if (!(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
         h->tau = value;

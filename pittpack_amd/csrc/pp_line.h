@@ -93,6 +93,12 @@ __device__ __forceinline__ c64 ld64(const double *t, int k) { return {t[2 * k], 
 template <typename R>
 __device__ __forceinline__ cpx<R> to_c(c64 a) { return {(R)a.x, (R)a.y}; }
 
+// load through the global address space (pointers taken from descriptor tables are generic)
+__device__ __forceinline__ double gld(const double *p, long i)
+{
+    return ((const __attribute__((address_space(1))) double *)p)[i];
+}
+
 // position in the real view of a line slice where input element n is placed
 template <int N, int KIND, int FWD>
 __device__ __forceinline__ int load_pos(int n, bool &neg)
@@ -224,13 +230,28 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         }
     }
 
+    // output element n of this thread's line, with the fused output scaling
+    auto put = [&](int n, double val) {
+        val = a.out_scale * val;
+        if constexpr (AX == 0 && !FWD) {
+            if (a.plane_add) val += a.plane_add[lz];
+            if (a.accumulate) val += (double)out[base + (long)n * stride];
+        }
+        if (AX == 1 && (a.ztiled & ZT_OUT)) out[tbase + (long)n * tstride] = (TOut)val;
+        else out[base + (long)n * stride] = (TOut)val;
+    };
+
     // ---- issue the coalesced global loads first ----
     double xin[2 * E];
     {
         const bool tin = (AX == 1) && (a.ztiled & ZT_IN);
         const long ib = tin ? tbase : base, is = tin ? tstride : stride;
 #pragma unroll
-        for (int c = 0; c < 2 * E; c++) xin[c] = valid ? (double)in[ib + (long)(t + T * c) * is] : 0.0;
+        for (int c = 0; c < 2 * E; c++) {
+            // invalid lanes read line 0 (in bounds): lines are transformed independently and
+            // their results are not stored, so no branch or select per load
+            xin[c] = (double)in[ib + (long)(t + T * c) * is];
+        }
     }
 
     // ---- twiddle seeds (independent of the data, overlap the load latency) ----
@@ -262,29 +283,38 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         if (a.dmask) {
             const double *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
             const double *ef = a.dtab + a.nx + a.ny + a.nz;
-            const double eyl = ey[ly], ezl = ez[lz];
-            if ((a.dmask & 16) && lz == 0) cyz += 2. * eyl * ef[4];
-            if ((a.dmask & 32) && lz == a.nz - 1) cyz += 2. * eyl * ef[5];
-            if ((a.dmask & 4) && ly == 0) cyz += 2. * ef[2] * ezl;
-            if ((a.dmask & 8) && ly == a.ny - 1) cyz += 2. * ef[3] * ezl;
-            if (a.dmask & 1) cx0 = 2. * ef[0] * eyl * ezl;
-            if (a.dmask & 2) cxn = 2. * ef[1] * eyl * ezl;
+            const double eyl = gld(ey, ly), ezl = gld(ez, lz);
+            if ((a.dmask & 16) && lz == 0) cyz += 2. * eyl * gld(ef, 4);
+            if ((a.dmask & 32) && lz == a.nz - 1) cyz += 2. * eyl * gld(ef, 5);
+            if ((a.dmask & 4) && ly == 0) cyz += 2. * gld(ef, 2) * ezl;
+            if ((a.dmask & 8) && ly == a.ny - 1) cyz += 2. * gld(ef, 3) * ezl;
+            if (a.dmask & 1) cx0 = 2. * gld(ef, 0) * eyl * ezl;
+            if (a.dmask & 2) cxn = 2. * gld(ef, 1) * eyl * ezl;
         }
+    }
+    if constexpr (AX == 0 && FWD) {
+#pragma unroll
+        for (int c = 0; c < 2 * E; c++) {
+            xin[c] = xin[c] * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
+            xin[c] = xin[c] * a.in_scale;
+        }
+        // one branch per thread, not per element: y/z-face lines, then the x-face elements
+        // n = 0 (thread 0, c = 0) and n = N - 1 (thread T - 1, c = 2E - 1)
+        if (cyz != 0.0) {
+#pragma unroll
+            for (int c = 0; c < 2 * E; c++) xin[c] = xin[c] - gld(ex, t + T * c) * cyz;
+        }
+        if (t == 0) xin[0] = xin[0] - cx0;
+        if (t == T - 1) xin[2 * E - 1] = xin[2 * E - 1] - cxn;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 2 * E; c++) xin[c] = xin[c] * a.in_scale;
     }
     double lsum = 0.0;
 #pragma unroll
     for (int c = 0; c < 2 * E; c++) {
         const int n = t + T * c;
-        double v = xin[c];
-        if constexpr (AX == 0 && FWD) {
-            v = v * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
-            v = v * a.in_scale;
-            if (cyz != 0.0) v = v - ex[n] * cyz;
-            if (n == 0) v = v - cx0;
-            if (n == N - 1) v = v - cxn;
-        } else {
-            v = v * a.in_scale;
-        }
+        const double v = xin[c];
         lsum += v;
         bool neg;
         const int pos = load_pos<N, KIND, FWD>(n, neg);
@@ -427,22 +457,43 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
                 }
             }
         }
-        __syncthreads();
+        // outputs k and N - k (N - 1 - k and k - 1 for DD) are contiguous across lanes:
+        // stored from registers, no LDS round trip
+        if (valid) {
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-            lr(L, p1[m]) = o1[m];
-            lr(L, p2[m]) = o2[m];
+            for (int m = 0; m < E; m++) {
+                put(p1[m], (double)o1[m]);
+                put(p2[m], (double)o2[m]);
+            }
         }
+        return;
     } else if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
         const R f = FWD ? (R)2 : (R)1;
         c64 dw = s2;
+        R o1[E], o2[E];
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int k = t + T * m;
             cpx<R> y = c_mul(v[m], to_c<R>(dw));
             dw = m64(dw, st2);
-            lr(L, 2 * k) = f * y.x;
-            lr(L, N - 1 - 2 * k) = (KIND == KIND_DN) ? f * y.y : -f * y.y;
+            o1[m] = f * y.x;
+            o2[m] = (KIND == KIND_DN) ? f * y.y : -f * y.y;
+            if constexpr (AX == 0) {
+                lr(L, 2 * k) = o1[m];
+                lr(L, N - 1 - 2 * k) = o2[m];
+            }
+        }
+        if constexpr (AX == 1) {
+            // lanes are columns: every output row is a coalesced store from registers
+            if (valid) {
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int k = t + T * m;
+                    put(2 * k, (double)o1[m]);
+                    put(N - 1 - 2 * k, (double)o2[m]);
+                }
+            }
+            return;
         }
     } else {
         // inverse PP / NN / DD: z_m holds (x_{2m}, x_{2m+1}) or the Makhoul-ordered v
@@ -460,13 +511,7 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
             const int pos = store_pos<N, KIND, FWD>(n, neg);
             double val = (double)lr(L, pos);
             if (neg) val = -val;
-            val = a.out_scale * val;
-            if constexpr (AX == 0 && !FWD) {
-                if (a.plane_add) val += a.plane_add[lz];
-                if (a.accumulate) val += (double)out[base + (long)n * stride];
-            }
-            if (AX == 1 && (a.ztiled & ZT_OUT)) out[tbase + (long)n * tstride] = (TOut)val;
-            else out[base + (long)n * stride] = (TOut)val;
+            put(n, val);
         }
     }
 }

@@ -115,6 +115,25 @@ __device__ __forceinline__ int load_pos(int n, bool &neg)
     }
 }
 
+// element held at real-view position p: the inverse of load_pos for forward transforms
+// and of store_pos for inverse ones (NN / DD share the Makhoul order both ways)
+template <int N, int KIND>
+__device__ __forceinline__ int pos_elem(int p, bool &neg)
+{
+    neg = false;
+    if constexpr (KIND == KIND_NN || KIND == KIND_DD) {
+        const int n = (p < N / 2) ? 2 * p : 2 * (N - 1 - p) + 1;
+        if constexpr (KIND == KIND_DD) neg = (n & 1);
+        return n;
+    } else if constexpr (KIND == KIND_ND) {
+        return (p & 1) ? N - p : p;
+    } else if constexpr (KIND == KIND_DN) {
+        return (p & 1) ? p - 1 : N - 1 - p;
+    } else {
+        return p;
+    }
+}
+
 template <int N, int KIND, int FWD>
 __device__ __forceinline__ int store_pos(int n, bool &neg)
 {
@@ -242,6 +261,11 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
     };
 
     // ---- issue the coalesced global loads first ----
+    // y forward and y quarter-wave (either direction): lanes are columns, so every row is a
+    // coalesced load and each thread reads the rows of its own FFT inputs directly (real view
+    // position p = 2 (t + T m) + c % 2), no LDS scatter; otherwise element n = t + T c,
+    // scattered through LDS below
+    constexpr bool DIRECT_IN = (AX == 1) && (FWD || KIND == KIND_ND || KIND == KIND_DN);
     double xin[2 * E];
     {
         const bool tin = (AX == 1) && (a.ztiled & ZT_IN);
@@ -250,7 +274,12 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         for (int c = 0; c < 2 * E; c++) {
             // invalid lanes read line 0 (in bounds): lines are transformed independently and
             // their results are not stored, so no branch or select per load
-            xin[c] = (double)in[ib + (long)(t + T * c) * is];
+            int n = t + T * c;
+            if constexpr (DIRECT_IN) {
+                bool ng;
+                n = pos_elem<N, KIND>(2 * (t + T * (c >> 1)) + (c & 1), ng);
+            }
+            xin[c] = (double)in[ib + (long)n * is];
         }
     }
 
@@ -311,14 +340,16 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         for (int c = 0; c < 2 * E; c++) xin[c] = xin[c] * a.in_scale;
     }
     double lsum = 0.0;
+    if constexpr (!DIRECT_IN) {
 #pragma unroll
-    for (int c = 0; c < 2 * E; c++) {
-        const int n = t + T * c;
-        const double v = xin[c];
-        lsum += v;
-        bool neg;
-        const int pos = load_pos<N, KIND, FWD>(n, neg);
-        lr(L, pos) = neg ? (R)(-v) : (R)v;
+        for (int c = 0; c < 2 * E; c++) {
+            const int n = t + T * c;
+            const double v = xin[c];
+            lsum += v;
+            bool neg;
+            const int pos = load_pos<N, KIND, FWD>(n, neg);
+            lr(L, pos) = neg ? (R)(-v) : (R)v;
+        }
     }
     if constexpr (AX == 0 && FWD) {
         if (a.line_sums) {
@@ -329,11 +360,27 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         }
     }
     (void)lsum;
-    __syncthreads();
+    if constexpr (!DIRECT_IN) __syncthreads();
 
     // ---- pre-processing: FFT input in registers, layout k = t + T*m ----
     cpx<R> v[E];
-    if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+    if constexpr (DIRECT_IN) {
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            bool n0, n1;
+            pos_elem<N, KIND>(2 * (t + T * m), n0);
+            pos_elem<N, KIND>(2 * (t + T * m) + 1, n1);
+            v[m] = {n0 ? (R)(-xin[2 * m]) : (R)xin[2 * m], n1 ? (R)(-xin[2 * m + 1]) : (R)xin[2 * m + 1]};
+        }
+        if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
+            c64 cw = s1;
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                v[m] = c_mul(v[m], to_c<R>(cw));
+                cw = m64(cw, st1);
+            }
+        }
+    } else if constexpr (KIND == KIND_ND || KIND == KIND_DN) {
         c64 cw = s1;
 #pragma unroll
         for (int m = 0; m < E; m++) {
@@ -388,7 +435,7 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
             v[m] = {s.x - q.y, s.y + q.x};
         }
     }
-    __syncthreads();
+    if constexpr (!DIRECT_IN) __syncthreads();
 
     // ---- complex FFT of length M ----
     constexpr bool INV_FFT = (!FWD) && (KIND == KIND_PP || KIND == KIND_NN || KIND == KIND_DD);
@@ -497,6 +544,21 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
         }
     } else {
         // inverse PP / NN / DD: z_m holds (x_{2m}, x_{2m+1}) or the Makhoul-ordered v
+        if constexpr (AX == 1) {
+            // lanes are columns: each real-view position p goes to its output row directly
+            if (valid) {
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    bool n0, n1;
+                    const int r0 = pos_elem<N, KIND>(2 * (t + T * m), n0);
+                    const int r1 = pos_elem<N, KIND>(2 * (t + T * m) + 1, n1);
+                    const double a0 = (double)v[m].x, a1 = (double)v[m].y;
+                    put(r0, n0 ? -a0 : a0);
+                    put(r1, n1 ? -a1 : a1);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int m = 0; m < E; m++) L[padc(t + T * m)] = v[m];
     }

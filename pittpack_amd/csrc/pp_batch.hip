@@ -214,7 +214,8 @@ int BatchCore::plan_mixed()
             m00_mem_.push_back((int)i);
         }
         const int zp = residual_zplanes();
-        const int gx = (g.nx + 63) / 64, gy = (g.ny + 3) / 4, gz = (g.nz + zp - 1) / zp;
+        const int tx = residual_tile_x(), ty = residual_tile_y();
+        const int gx = (g.nx + tx - 1) / tx, gy = (g.ny + ty - 1) / ty, gz = (g.nz + zp - 1) / zp;
         if ((long)gx * gy * gz != residual_blocks(g.nx, g.ny, g.nz)) return PP_INPUT_ARGS;
         for (int bz = 0; bz < gz; bz++)
             for (int by = 0; by < gy; by++)

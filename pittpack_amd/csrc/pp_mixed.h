@@ -28,10 +28,12 @@ struct ResArgs {
 
 // per-block partial sums of u over plane 0 written by the residual kernel
 int residual_parts(int nx, int ny);
-// blocks of one residual launch over an nx x ny x nz grid (64 x 4 cells, residual_zplanes()
-// planes per block)
+// blocks of one residual launch over an nx x ny x nz grid (residual_tile_x() x
+// residual_tile_y() cells, residual_zplanes() planes per block)
 int residual_blocks(int nx, int ny, int nz);
 int residual_zplanes();
+int residual_tile_x();
+int residual_tile_y();
 
 // one pinned member of a batched (0,0)-column launch
 struct Mode00Item {

@@ -147,104 +147,148 @@ __global__ void __launch_bounds__(1024) k_mode00_batch(const Mode00Item *items, 
 }
 
 constexpr int RES_ZC = 32;  // planes marched per thread
+constexpr int RES_TX = 64;  // block tile: one 64-lane wave per row ...
+constexpr int RES_TY = 4;   // ... four rows
 
-// batched launches: block b works on items[b] = {member, bx, by, bz} with solves[member]
+// Residual of one block tile (64 columns x 4 rows), marching RES_ZC planes with u(z-1), u(z),
+// u(z+1) in registers.  Each wave reads one 512-byte row per field and plane (the access
+// shape that streams fastest here); the four rows of u(z) are shared through a
+// double-buffered LDS tile, so in-plane neighbours inside the tile are LDS reads and only the
+// tile's edge lanes (x) and edge waves (y) need a halo or ghost value from memory: four
+// loads per cell -- u(z+1), f and one halo per direction -- instead of seven.
+// Batched launches: block b works on items[b] = {member, bx, by, bz} with solves[member].
 template <typename TOut>
 __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
     ResArgs a = a0;
-    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z, gx = gridDim.x;
+    // Blocks are dealt to the 8 XCDs round-robin and each XCD has its own L2: numbered so
+    // that each XCD works on a contiguous run of tiles, the halo lines a tile shares with its
+    // neighbours are fetched once per XCD instead of once per tile
+    const int lin = xcd_logical(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                gridDim.x * gridDim.y * gridDim.z);
+    int bx = lin % gridDim.x, by = (lin / gridDim.x) % gridDim.y, bz = lin / (gridDim.x * gridDim.y);
+    int gx = gridDim.x;
     if (items) {
-        const int4 it = items[blockIdx.x];
+        const int4 it = items[lin];
         a = solves[it.x];
         bx = it.y;
         by = it.z;
         bz = it.w;
-        gx = (a.nx + 63) / 64;
+        gx = (a.nx + RES_TX - 1) / RES_TX;
     }
-    const int x = bx * 64 + (threadIdx.x & 63);
-    const int y = by * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = bx * RES_TX + lane;
+    const int y = by * RES_TY + w;
     const int z0 = bz * RES_ZC;
     const bool valid = x < a.nx && y < a.ny;
     const long sy = a.nx, sz = (long)a.nx * a.ny;
-    const double *u = a.u;
+    // global address space: generic (flat) accesses force a full wait on every load
+    using gdouble = const __attribute__((address_space(1))) double;
+    using gout = __attribute__((address_space(1))) TOut;
+    // lanes outside the grid work on the clamped cell (in bounds) and store nothing
+    const int xc = min(x, a.nx - 1), yc = min(y, a.ny - 1);
+    const long cell = xc + sy * yc;
+    gdouble *uc_p = (gdouble *)(a.u + cell);
+    gdouble *f_p = (gdouble *)(a.f + cell);
+    __shared__ double red[256];
     if (a.upart && bz == 0) {
-        // plane-0 sum of u, one partial per (x, y) block
-        __shared__ double red[256];
-        red[threadIdx.x] = valid ? u[x + sy * y] : 0.0;
+        // plane-0 sum of u, one partial per (x, y) tile
+        red[threadIdx.x] = valid ? uc_p[0] : 0.0;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
             __syncthreads();
         }
         if (threadIdx.x == 0) a.upart[by * gx + bx] = red[0];
     }
-    if (!valid) return;
-    TOut *r = reinterpret_cast<TOut *>(a.r);
     const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
     const double *ef = a.dtab + a.nx + a.ny + a.nz;
-    // neighbour of a boundary cell through the ghost closure the spectral solve
-    // diagonalises: P wraps, N mirrors (ghost = inside), D is antisymmetric (ghost = -inside);
-    // as an index and a sign per direction, fixed for this thread's (x, y)
-    auto nb = [](char bc, int i, int wrap, long step, long &off, double &sg) {
-        if (bc == 'P') {
-            off = (long)(wrap - i) * step;
-            sg = 1.0;
-        } else {
-            off = 0;
-            sg = (bc == 'N') ? 1.0 : -1.0;
-        }
+    // Halo lanes and waves: the neighbour is outside the tile, or outside the grid where the
+    // ghost closure the spectral solve diagonalises applies (P wraps, N mirrors, D is
+    // antisymmetric).  With nx and ny even no cell needs both neighbours of one direction
+    // from memory, so one halo load per direction serves every lane (a lane with no halo in
+    // a direction reads its own cell): unconditional loads keep the compiler's count of
+    // outstanding loads exact, which the pipelined march depends on.
+    auto ghost = [](char bc, long wrap, long &off, double &sg) {
+        off = bc == 'P' ? wrap : 0;
+        sg = bc == 'D' ? -1.0 : 1.0;
     };
+    const bool hxm = lane == 0, hxp = lane == 63 || xc == a.nx - 1;
+    const bool hym = w == 0, hyp = w == RES_TY - 1 || yc == a.ny - 1;
     long oxm = -1, oxp = 1, oym = -sy, oyp = sy;
     double gxm = 1.0, gxp = 1.0, gym = 1.0, gyp = 1.0;
-    if (x == 0) nb(a.bc[0], x, a.nx - 1, 1, oxm, gxm);
-    if (x == a.nx - 1) nb(a.bc[1], x, 0, 1, oxp, gxp);
-    if (y == 0) nb(a.bc[2], y, a.ny - 1, sy, oym, gym);
-    if (y == a.ny - 1) nb(a.bc[3], y, 0, sy, oyp, gyp);
+    if (xc == 0) ghost(a.bc[0], a.nx - 1, oxm, gxm);
+    if (xc == a.nx - 1) ghost(a.bc[1], -(long)(a.nx - 1), oxp, gxp);
+    if (yc == 0) ghost(a.bc[2], sy * (a.ny - 1), oym, gym);
+    if (yc == a.ny - 1) ghost(a.bc[3], -sy * (a.ny - 1), oyp, gyp);
+    const long ohx = hxm ? oxm : (hxp ? oxp : 0), ohy = hym ? oym : (hyp ? oyp : 0);
+    // z-profile of the x/y-face Dirichlet terms for this block's planes, and the u(z) tile
+    __shared__ double ezs[RES_ZC];
+    __shared__ double tile[2][RES_TY][RES_TX];
+    if (threadIdx.x < RES_ZC) ezs[threadIdx.x] = z0 + (int)threadIdx.x < a.nz ? gld(ez, z0 + threadIdx.x) : 0.0;
     // Dirichlet terms of the effective right-hand side: x and y faces are a per-(x, y)
     // constant times ez[z], z faces a constant on the first and last plane
     double cxy = 0.0, czlo = 0.0, czhi = 0.0;
     if (a.dmask) {
-        if ((a.dmask & 1) && x == 0) cxy += 2. * ef[0] * ey[y];
-        if ((a.dmask & 2) && x == a.nx - 1) cxy += 2. * ef[1] * ey[y];
-        if ((a.dmask & 4) && y == 0) cxy += 2. * ex[x] * ef[2];
-        if ((a.dmask & 8) && y == a.ny - 1) cxy += 2. * ex[x] * ef[3];
-        if (a.dmask & 16) czlo = 2. * ex[x] * ey[y] * ef[4];
-        if (a.dmask & 32) czhi = 2 * ex[x] * ey[y] * ef[5];
+        if ((a.dmask & 1) && xc == 0) cxy += 2. * ef[0] * ey[yc];
+        if ((a.dmask & 2) && xc == a.nx - 1) cxy += 2. * ef[1] * ey[yc];
+        if ((a.dmask & 4) && yc == 0) cxy += 2. * ex[xc] * ef[2];
+        if ((a.dmask & 8) && yc == a.ny - 1) cxy += 2. * ex[xc] * ef[3];
+        if (a.dmask & 16) czlo = 2. * ex[xc] * ey[yc] * ef[4];
+        if (a.dmask & 32) czhi = 2 * ex[xc] * ey[yc] * ef[5];
     }
     const double defect = a.pin_defect ? a.pin_defect[0] : 0.0;
-    const double *uc_p = u + x + sy * y;
-    const double *f_p = a.f + x + sy * y;
-    TOut *r_p = r + x + sy * y;
+    gout *r_p = (gout *)(reinterpret_cast<TOut *>(a.r) + cell);
+
     double uc = uc_p[sz * z0];
     double um;
     if (z0 > 0) um = uc_p[sz * (z0 - 1)];
     else um = a.bc[4] == 'P' ? uc_p[sz * (a.nz - 1)] : (a.bc[4] == 'N' ? uc : -uc);
-    auto plane = [&](int z) {
+    // loads of one plane: u(z+1) (at the top face: plane 0 for P, else this plane, replaced
+    // by the ghost below), the right-hand side and the halo value per direction
+    struct Plane {
+        double up, f, hx, hy;
+    };
+    const bool ztop_wrap = a.bc[5] == 'P';
+    auto fetch = [&](int z, Plane &q) {
         const long o = sz * z;
-        double up;
-        if (z + 1 < a.nz) up = uc_p[o + sz];
-        else up = a.bc[5] == 'P' ? uc_p[0] : (a.bc[5] == 'N' ? uc : -uc);
-        const double uxm = gxm * uc_p[o + oxm], uxp = gxp * uc_p[o + oxp];
-        const double uym = gym * uc_p[o + oym], uyp = gyp * uc_p[o + oyp];
+        q.up = uc_p[z + 1 < a.nz ? o + sz : (ztop_wrap ? 0 : o)];
+        q.f = f_p[o];
+        q.hx = uc_p[o + ohx];
+        q.hy = uc_p[o + ohy];
+    };
+    // software pipelined march: the loads of plane z + 1 are issued before plane z is
+    // computed and stored, so their latency overlaps this plane's arithmetic
+    const int zend = min(z0 + RES_ZC, a.nz);
+    Plane cur;
+    fetch(z0, cur);
+    for (int z = z0; z < zend; z++) {
+        Plane nxt;
+        fetch(min(z + 1, zend - 1), nxt);
+        const int b = (z - z0) & 1;
+        tile[b][w][lane] = uc;
+        __syncthreads();  // one barrier per plane: the other buffer was last read before it
+        const long o = sz * z;
+        const double up = (z + 1 < a.nz || ztop_wrap) ? cur.up : (a.bc[5] == 'N' ? uc : -uc);
+        const double uxm = hxm ? gxm * cur.hx : tile[b][w][lane > 0 ? lane - 1 : 0];
+        const double uxp = hxp ? gxp * cur.hx : tile[b][w][lane < 63 ? lane + 1 : 63];
+        const double uym = hym ? gym * cur.hy : tile[b][w > 0 ? w - 1 : 0][lane];
+        const double uyp = hyp ? gyp * cur.hy : tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
         const double Lu = (uxp - 2.0 * uc + uxm) * a.Cx + (uyp - 2.0 * uc + uym) * a.Cy + (up - 2.0 * uc + um);
         // effective right-hand side: rhs * dz^2 minus the Dirichlet boundary term
-        double v = f_p[o];
+        double v = cur.f;
         v = v * a.c3;
         v = v * a.c3;
-        if (cxy != 0.0) v = v - cxy * ez[z];
+        if (cxy != 0.0) v = v - cxy * ezs[z - z0];
         if (z == 0) v = v - czlo;
         if (z == a.nz - 1) v = v - czhi;
         double res = v - Lu;
         if (z == 0) res += defect;
-        r_p[o] = (TOut)res;
+        if (valid) r_p[o] = (TOut)res;
         um = uc;
         uc = up;
-    };
-    // a plain loop keeps the kernel at ~52 VGPRs (8 waves per SIMD), which hides the load
-    // latency better than unrolling the march
-    const int zend = min(z0 + RES_ZC, a.nz);
-    for (int z = z0; z < zend; z++) plane(z);
+        cur = nxt;
+    }
 }
 
 hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s)
@@ -272,7 +316,11 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
     return hipGetLastError();
 }
 
-int residual_parts(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
+int residual_parts(int nx, int ny) { return ((nx + RES_TX - 1) / RES_TX) * ((ny + RES_TY - 1) / RES_TY); }
+
+int residual_tile_x() { return RES_TX; }
+
+int residual_tile_y() { return RES_TY; }
 
 int residual_zplanes() { return RES_ZC; }
 
@@ -280,7 +328,8 @@ int residual_blocks(int nx, int ny, int nz) { return residual_parts(nx, ny) * ((
 
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
 {
-    dim3 grid((unsigned)((a.nx + 63) / 64), (unsigned)((a.ny + 3) / 4), (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
+    dim3 grid((unsigned)((a.nx + RES_TX - 1) / RES_TX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
+              (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
     hipLaunchKernelGGL(k_residual<float>, grid, dim3(256), 0, s, a, nullptr, nullptr);
     return hipGetLastError();
 }

@@ -49,7 +49,10 @@ enum {
      *           1 = partitioned Thomas, contiguous rows per lane, cross-lane scans;
      *           2 = multigrid V-cycles (the reference's MultiGrid alternative) to a
      *               max-norm residual of PP_OPT_MG_TOL * max|rhs| (default 1e-16), or until
-     *               a V-cycle reduces it by less than 10% (its rounding floor) */
+     *               a V-cycle reduces it by less than 10% (its rounding floor);
+     *           3 = full parallel cyclic reduction (TriDiag::pcr, src/triDiag.cpp:1442-1542);
+     *           4 = cyclic reduction (TriDiag::crp, src/triDiag.cpp:1285-1437).
+     *           2-4: FP64 form (nz <= 512, as the default), non-periodic z */
     PP_OPT_ZSOLVER = 3,
     /* 0 (default): FP64 throughout.  1: mixed form -- FP32 transforms and z solves with
      * FP64 twiddles and eigenvalues, FP64 solves of the near-singular columns

@@ -17,6 +17,11 @@ hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f64(int nz);
 // multigrid alternative (pp_zmg.hip): V-cycles until max|residual| <= tol * max|rhs|
 hipError_t launch_zsolve_mg_f64(const ZArgs &a, int max_cycles, double tol, hipStream_t s);
+// the reference's alternatives (pp_zalt.hip): full parallel cyclic reduction and cyclic
+// reduction, one wave per column, nz <= 1024
+bool zsolve_alt_supported(int nz);
+hipError_t launch_zsolve_pcr_full_f64(const ZArgs &a, hipStream_t s);
+hipError_t launch_zsolve_cr_f64(const ZArgs &a, hipStream_t s);
 hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f32(int nz);
 // FP64 re-solve of the near-singular columns of FP32 data, one block per listed 16-column

@@ -384,8 +384,13 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
             if (h->zsing_x >= 0)
                 PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
         } else {
-            if (h->zsolver == 2) PP_HIP_CHECK(launch_zsolve_mg_f64(z, h->mg_cycles, h->mg_tol, s));
-            else PP_HIP_CHECK(h->zsolver == 1 ? launch_zsolve_scan_f64(h->g.nz, z, s) : launch_zsolve_pcr_f64(h->g.nz, z, s));
+            switch (h->zsolver) {
+            case 1: PP_HIP_CHECK(launch_zsolve_scan_f64(h->g.nz, z, s)); break;
+            case 2: PP_HIP_CHECK(launch_zsolve_mg_f64(z, h->mg_cycles, h->mg_tol, s)); break;
+            case 3: PP_HIP_CHECK(launch_zsolve_pcr_full_f64(z, s)); break;
+            case 4: PP_HIP_CHECK(launch_zsolve_cr_f64(z, s)); break;
+            default: PP_HIP_CHECK(launch_zsolve_pcr_f64(h->g.nz, z, s)); break;
+            }
         }
     }
     if ((rc = mark(h))) return rc;
@@ -828,7 +833,7 @@ int pp_set_option(pp_handle h, int key, long long value)
         h->mixed_ready = false;
         return PP_SUCCESS;
     case PP_OPT_ZSOLVER:
-        if (value < 0 || value > 2) return PP_INPUT_ARGS;
+        if (value < 0 || value > 4) return PP_INPUT_ARGS;
         h->zsolver = (int)value;
         return PP_SUCCESS;
     case PP_OPT_ZLAYOUT:

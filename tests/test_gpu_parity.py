@@ -31,7 +31,7 @@ def test_selftest_lane_rotation():
     assert pa.selftest() == 0
 
 
-@pytest.mark.parametrize("zsolver", [0, 1, 2], ids=["pcr", "scan", "mg"])
+@pytest.mark.parametrize("zsolver", [0, 1, 2, 3, 4], ids=["pcr", "scan", "mg", "pcrfull", "cr"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("bc", COMBOS)
 def test_parity_fp64(bc, shape, zsolver):
@@ -46,7 +46,7 @@ def test_parity_fp64(bc, shape, zsolver):
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
-@pytest.mark.parametrize("zsolver", [0, 1, 2], ids=["pcr", "scan", "mg"])
+@pytest.mark.parametrize("zsolver", [0, 1, 2, 3, 4], ids=["pcr", "scan", "mg", "pcrfull", "cr"])
 @pytest.mark.parametrize("nz", [4, 5, 15, 16, 17, 31, 33, 48, 49, 64, 100, 128, 255, 256, 500, 511, 512])
 def test_parity_z_lengths(nz, zsolver):
     """every z length class: J = 1..64 rows per lane, padded and full columns; uniform

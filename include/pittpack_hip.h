@@ -65,7 +65,11 @@ enum {
     /* FP64 form: 1 (default) = the y transforms hand the z solve contiguous column tiles
      * through a second field-sized buffer; 0 = the z solve works on the field in place */
     PP_OPT_ZLAYOUT = 8,
-    PP_OPT_MG_TOL = 9 /* pp_set_option_double: multigrid residual tolerance */
+    PP_OPT_MG_TOL = 9, /* pp_set_option_double: multigrid residual tolerance */
+    /* batched form: 1 (default) = the second pp_batch_solve captures the solve's kernel
+     * launches into a HIP graph and later solves replay it (one submission per solve);
+     * 0 = every solve launches its kernels.  Timing (PP_OPT_TIMING) runs eagerly. */
+    PP_OPT_GRAPH = 10
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

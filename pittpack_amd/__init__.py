@@ -37,6 +37,7 @@ OPT_TAU = 6
 OPT_BATCH_MEMBERS = 7
 OPT_ZLAYOUT = 8
 OPT_MG_TOL = 9
+OPT_GRAPH = 10
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -265,11 +266,13 @@ class Batch:
     """
 
     def __init__(self, precision: int = 0, refine: int | None = None, tau: float | None = None,
-                 zero_mode: int | None = None):
+                 zero_mode: int | None = None, graph: int | None = None):
         self._L = load_library()
         self._h = ctypes.c_void_p()
         _check(self._L.pp_batch_create(ctypes.byref(self._h)), "pp_batch_create")
         self.shapes = []
+        if graph is not None:
+            self.set_option(OPT_GRAPH, graph)
         if precision:
             self.set_option(OPT_PRECISION, precision)
         if refine is not None:

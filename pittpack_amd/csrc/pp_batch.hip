@@ -578,6 +578,16 @@ struct pp_batch_s {
     static constexpr int kMarks = 64;
     std::vector<hipEvent_t> ev;
     int ev_slots = 0, ev_next = 0, ev_used = 0, marks_last = 0;
+    // PP_OPT_GRAPH: the solve's launches captured once into a HIP graph and replayed (the
+    // first solve runs eagerly, so one-time launch set-up happens outside the capture)
+    int graph_mode = 1;
+    int solves = 0;
+    hipGraphExec_t graph = nullptr;
+    void drop_graph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
 };
 
 extern "C" {
@@ -603,6 +613,7 @@ int pp_batch_destroy(pp_batch b)
 {
     if (!b) return PP_INVALID_HANDLE;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->drop_graph();
     if (b->region) (void)hipFree(b->region);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -724,6 +735,7 @@ int pp_batch_set_stream(pp_batch b, void *stream)
 {
     if (!b) return PP_INVALID_HANDLE;
     PPB_CHECK(hipStreamSynchronize(b->stream));
+    b->drop_graph();
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     b->own_stream = false;
     b->stream = (hipStream_t)stream;
@@ -746,7 +758,17 @@ int pp_batch_set_option(pp_batch b, int key, long long value)
         return PP_SUCCESS;
     case PP_OPT_REFINE:
         if (value < 0 || value > 16) return PP_INPUT_ARGS;
+        if (b->core.refine != (int)value) {
+            PPB_CHECK(hipStreamSynchronize(b->stream));
+            b->drop_graph();
+        }
         b->core.refine = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_GRAPH:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        PPB_CHECK(hipStreamSynchronize(b->stream));
+        b->drop_graph();
+        b->graph_mode = (int)value;
         return PP_SUCCESS;
     case PP_OPT_TIMING: {
         if (value < 0 || value > 1024) return PP_INPUT_ARGS;
@@ -787,8 +809,36 @@ int pp_batch_solve(pp_batch b)
         b->ev_next = (b->ev_next + 1) % b->ev_slots;
         if (b->ev_used < b->ev_slots) b->ev_used++;
     }
+    const bool use_graph = b->graph_mode && !b->timing && b->stream != nullptr;
+    if (use_graph && b->graph) {
+        PPB_CHECK(hipGraphLaunch(b->graph, b->stream));
+        b->solves++;
+        return PP_SUCCESS;
+    }
+    if (use_graph && b->solves > 0) {
+        // capture this solve's launches (kernels only: no memset or copy is recorded) and
+        // replay the graph from now on
+        hipGraph_t g = nullptr;
+        PPB_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        const int rc = b->core.run(b->stream, nullptr);
+        const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : PP_HIP_ERROR;
+        }
+        const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b->graph = nullptr;
+            return PP_HIP_ERROR;
+        }
+        PPB_CHECK(hipGraphLaunch(b->graph, b->stream));
+        b->solves++;
+        return PP_SUCCESS;
+    }
     const int rc = b->core.run(b->stream, &mk);
     if (mk.ev) b->marks_last = mk.n;
+    if (!rc) b->solves++;
     return rc;
 }
 

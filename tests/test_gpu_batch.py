@@ -125,3 +125,30 @@ def test_batch_mixed_precision(zero_mode):
             assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
             single = pa.solve(rhs[i], bc, box, zero_mode=zero_mode, precision=1)
             assert rel_maxdiff(got, single) <= 1e-14, f"member {i}: batched vs single mixed"
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_graph_replays_match_eager(precision):
+    """PP_OPT_GRAPH: the second solve captures the batch's launches into a HIP graph and later
+    solves replay it.  Every replay, with eager work (new right-hand sides from the device
+    generator, copies and synchronisations) between them, must equal the eager batch bit for
+    bit."""
+    print(f"batch graph replays precision={precision}", flush=True)
+    E = pa.Batch(precision=precision, graph=0)
+    G = pa.Batch(precision=precision, graph=1)
+    try:
+        for B in (E, G):
+            for nx, ny, nz, bc, box in MEMBERS:
+                B.add(nx, ny, nz, bc, box)
+            B.finalize()
+        for rep, seed in enumerate((11, 12, 11, 13, 12)):
+            for B in (E, G):
+                B.fill_uniform(seed, round_f32=bool(precision))
+                B.solve()
+                B.synchronize()
+            for i in range(len(MEMBERS)):
+                a, b = G.get_solution(i), E.get_solution(i)
+                assert np.array_equal(a, b), f"solve {rep} (seed {seed}) member {i}: graph replay != eager"
+    finally:
+        E.close()
+        G.close()

@@ -246,7 +246,9 @@ def main():
     ap.add_argument("--refine", type=int, default=2, help="refinement steps of the mixed form")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--zsolver", type=int, default=None, help="0 PCR->Thomas, 1 partitioned Thomas, 2 multigrid")
+    ap.add_argument("--zsolver", type=int, default=None,
+                    help="0 PCR->Thomas, 1 partitioned Thomas, 2 multigrid, 3 full PCR, 4 cyclic reduction")
+    ap.add_argument("--no-autotune", action="store_true", help="FP64 form in the fixed default configuration")
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
@@ -279,6 +281,8 @@ def main():
         M.setCoords(2)
         if args.zsolver is not None:
             M.set_option(pa.OPT_ZSOLVER, args.zsolver)
+        if precision == 0 and not args.no_autotune:
+            M.set_option(pa.OPT_AUTOTUNE, 1)  # tuned on the first (warm-up) solve
         if precision:
             M.set_option(pa.OPT_PRECISION, precision)
             M.set_option(pa.OPT_REFINE, args.refine)
@@ -322,6 +326,7 @@ def main():
     ms_per_step, kt, M = timed_form(0)
     value = pts / (ms_per_step * 1e-3)
     roof, kms, gbs = roofline(0, kt)
+    tuning = M.tuning()
     u64 = u.clone()
     M.close()
 
@@ -377,6 +382,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": roof,
+        "tuning": tuning,
         "kernels_ms": kms,
         "kernels_gbs": gbs,
         "copy_gbs": copy_gbs,

@@ -69,7 +69,15 @@ enum {
     /* batched form: 1 (default) = the second pp_batch_solve captures the solve's kernel
      * launches into a HIP graph and later solves replay it (one submission per solve);
      * 0 = every solve launches its kernels.  Timing (PP_OPT_TIMING) runs eagerly. */
-    PP_OPT_GRAPH = 10
+    PP_OPT_GRAPH = 10,
+    /* FP64 form: 1 = before the next solve, time the candidate configurations on this shape
+     * (z: PCR->Thomas on z tiles, the same in place, or partitioned Thomas; each line stage:
+     * default or alternative radix plan / block shape) and keep the fastest; the choice is
+     * cached per process and shape.  0 (default) = fixed default configuration. */
+    PP_OPT_AUTOTUNE = 11,
+    /* testing: force every FP64 line stage to the default (0) or alternative (1) launch
+     * shape where it has one; -1 (default) = the autotuner's choice */
+    PP_OPT_LINE_VARIANT = 12
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
@@ -132,6 +140,11 @@ int pp_get_stage_times(pp_handle h, float *ms5);
 /* Mean durations (ms) between consecutive kernel launches of the recorded solves (any
  * form), in launch order; *count receives the number of intervals.  Resets the record. */
 int pp_get_kernel_times(pp_handle h, float *ms, int max, int *count);
+
+/* Configuration in use: out[0] = 1 when tuned, out[1] = z solver, out[2] = 1 when z tiles
+ * are used, out[3..6] = launch shape (0 default, 1 alternative) of the x forward, y forward,
+ * y inverse and x inverse stages.  n >= 7.  No counterpart in the reference. */
+int pp_get_tuning(pp_handle h, int *out, int n);
 
 /* Device pointer of the handle's field (nx*ny*nz doubles). */
 int pp_get_device_field(pp_handle h, double **d_field);

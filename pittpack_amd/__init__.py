@@ -38,6 +38,8 @@ OPT_BATCH_MEMBERS = 7
 OPT_ZLAYOUT = 8
 OPT_MG_TOL = 9
 OPT_GRAPH = 10
+OPT_AUTOTUNE = 11
+OPT_LINE_VARIANT = 12
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -45,7 +47,7 @@ EXPORTED = (
     "pp_assign_rhs", "pp_assign_rhs_device", "pp_pittpack", "pp_get_solution",
     "pp_get_solution_device", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
     "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
-    "pp_selftest", "pp_set_option_double", "pp_get_kernel_times",
+    "pp_selftest", "pp_set_option_double", "pp_get_kernel_times", "pp_get_tuning",
     "pp_batch_create", "pp_batch_destroy", "pp_batch_add", "pp_batch_finalize", "pp_batch_count",
     "pp_batch_launch_count", "pp_batch_field", "pp_batch_set_rhs", "pp_batch_get_solution",
     "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
@@ -99,6 +101,7 @@ def load_library(path: str | None = None):
     L.pp_get_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.pp_set_option_double.argtypes = [vp, i, ctypes.c_double]
     L.pp_get_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, ctypes.POINTER(i)]
+    L.pp_get_tuning.argtypes = [vp, ctypes.POINTER(i), i]
     L.pp_get_device_field.argtypes = [vp, ctypes.POINTER(vp)]
     L.pp_get_error_enum.argtypes = [i]
     L.pp_get_error_enum.restype = ctypes.c_char_p
@@ -237,6 +240,13 @@ class PoissonHIP:
         n = ctypes.c_int(0)
         _check(self._L.pp_get_kernel_times(self._h, ms, 64, ctypes.byref(n)), "kernel_times")
         return list(ms)[: n.value]
+
+    def tuning(self) -> dict:
+        """Configuration in use (after PP_OPT_AUTOTUNE: the tuner's choice)."""
+        out = (ctypes.c_int * 7)()
+        _check(self._L.pp_get_tuning(self._h, out, 7), "get_tuning")
+        return {"tuned": bool(out[0]), "zsolver": out[1], "ztiles": out[2],
+                "line_variant": {"x_forward": out[3], "y_forward": out[4], "y_inverse": out[5], "x_inverse": out[6]}}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value is not None:
@@ -439,13 +449,18 @@ class Server:
 
 
 def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None,
-          precision: int = 0, refine: int | None = None, tau: float | None = None) -> np.ndarray:
+          precision: int = 0, refine: int | None = None, tau: float | None = None,
+          line_variant: int | None = None, autotune: int | None = None) -> np.ndarray:
     """One-shot solve of an (nz, ny, nx) FP64 RHS (precision 0: FP64; 1: mixed + refinement)."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     with PoissonHIP(nx, ny, nz) as M:
         if zero_mode is not None:
             M.set_option(OPT_ZERO_MODE, zero_mode)
+        if line_variant is not None:
+            M.set_option(OPT_LINE_VARIANT, line_variant)
+        if autotune is not None:
+            M.set_option(OPT_AUTOTUNE, autotune)
         if zsolver is not None:
             M.set_option(OPT_ZSOLVER, zsolver)
         if precision:

@@ -82,6 +82,8 @@ struct LineArgs {
     // tnz >= nz rows per tile, so each (x/W, y) tile is contiguous for the z solve
     int ztiled;
     int tnz;
+    // host side, FP64 form: 1 = the alternative launch shape (line_threads_alt), 0 = default
+    int variant;
 };
 enum : int { ZT_OUT = 1, ZT_IN = 2 };
 
@@ -603,10 +605,20 @@ constexpr int line_threads(int M, int AX)
     }
 }
 
-template <typename R, int M, int AX>
+// Alternative launch shape of the FP64 transforms (chosen per stage by the autotuner): twice
+// the threads per line, half the elements per thread -- fewer registers and more waves in
+// flight for one more radix pass (e.g. M = 256: 4 x 4 x 4 x 4 instead of 8 x 8 x 4); offered
+// where the plan grows by at most one pass, 0 elsewhere
+constexpr int line_threads_alt(int M)
+{
+    const int T = line_threads(M, 0);
+    if (T == 0 || M / T < 8 || !plan_ok(M, M / (2 * T))) return 0;
+    return make_plan(M, M / (2 * T)).n <= make_plan(M, M / T).n + 1 ? 2 * T : 0;
+}
+
+template <typename R, int M, int AX, int T = line_threads(M, AX)>
 constexpr int line_lines()
 {
-    constexpr int T = line_threads(M, AX);
     constexpr int bytes = line_slots<M>() * 2 * (int)sizeof(R);
     // y: 16 FP64 columns (128-byte rows) or up to 32 FP32 columns per block
     int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
@@ -618,11 +630,15 @@ constexpr int line_lines()
     return lines;
 }
 
-template <typename TIn, typename TOut, typename R, int M, int AX, int FWD, int KIND>
+template <typename TIn, typename TOut, typename R, int M, int AX, int FWD, int KIND, int V = 0>
 hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
 {
-    constexpr int T = line_threads(M, AX);
-    constexpr int LINES = line_lines<R, M, AX>();
+    constexpr int T = V ? line_threads_alt(M) : line_threads(M, AX);
+    if constexpr (T == 0) {
+        return hipErrorInvalidValue;
+    } else {
+    // y keeps the default's columns per block in either shape (the z-tile width)
+    constexpr int LINES = (V && AX == 1) ? line_lines<R, M, AX>() : line_lines<R, M, AX, T>();
     const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>);
     auto kern = k_line<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>;
     if (lds > 64 * 1024) {
@@ -640,14 +656,15 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
     else grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES), (unsigned)a.nplanes);
     hipLaunchKernelGGL(kern, grid, block, lds, s, a);
     return hipGetLastError();
+    }
 }
 
-template <typename TIn, typename TOut, typename R, int AX, int FWD, int KIND>
+template <typename TIn, typename TOut, typename R, int AX, int FWD, int KIND, int V = 0>
 hipError_t launch_line_sz(int N, const LineArgs &a, hipStream_t s)
 {
     switch (N / 2) {
 #define PPM(m) \
-    case m: return launch_line_cfg<TIn, TOut, R, m, AX, FWD, KIND>(a, s);
+    case m: return launch_line_cfg<TIn, TOut, R, m, AX, FWD, KIND, V>(a, s);
         PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256)
         PPM(384) PPM(512)
 #undef PPM
@@ -667,6 +684,12 @@ hipError_t launch_line_kind(int form, int N, int AX, int FWD, const LineArgs &a,
 {
     switch (form) {
     case FORM_F64:
+        if (a.variant) {
+            if (AX == 0) return FWD ? launch_line_sz<double, double, double, 0, 1, KIND, 1>(N, a, s)
+                                    : launch_line_sz<double, double, double, 0, 0, KIND, 1>(N, a, s);
+            return FWD ? launch_line_sz<double, double, double, 1, 1, KIND, 1>(N, a, s)
+                       : launch_line_sz<double, double, double, 1, 0, KIND, 1>(N, a, s);
+        }
         if (AX == 0) return FWD ? launch_line_sz<double, double, double, 0, 1, KIND>(N, a, s)
                                 : launch_line_sz<double, double, double, 0, 0, KIND>(N, a, s);
         return FWD ? launch_line_sz<double, double, double, 1, 1, KIND>(N, a, s)
@@ -687,6 +710,9 @@ hipError_t launch_line_kind(int form, int N, int AX, int FWD, const LineArgs &a,
 }
 
 inline bool line_size_supported(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }
+
+// the FP64 transforms of length N have an alternative launch shape
+inline bool line_has_variant(int n) { return line_size_supported(n) && line_threads_alt(n / 2) > 0; }
 
 // lines (x) or columns (y) per block of the launch for line length N, precision R
 template <typename R>

@@ -4,7 +4,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <array>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -98,6 +101,13 @@ struct pp_solver {
     double *d_ztw = nullptr, *d_zc2 = nullptr;  // periodic z: twiddles and 2 cos(2 pi k / nz)
     double *d_zt = nullptr;                     // z tiles of the FP64 form
     int ztiles = 1;                             // PP_OPT_ZLAYOUT
+    // autotuner (PP_OPT_AUTOTUNE): launch shape per FP64 line stage (x fwd, y fwd, y inv,
+    // x inv; 1 = line_threads_alt) and the z configuration, chosen by timing
+    int autotune = 0;
+    bool tuned = false;
+    int line_var[4] = {0, 0, 0, 0};
+    int force_var = -1;                         // PP_OPT_LINE_VARIANT (every stage; -1: tuner's)
+    hipEvent_t *ev_force = nullptr;             // tuner's own events for the launch marks
     int mg_cycles = 200;                        // multigrid z solver: V-cycle limit
     double mg_tol = 1e-16;                      //   and relative max-norm residual tolerance
     int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any
@@ -149,6 +159,11 @@ int mark(pp_solver *h)
 int begin_timing(pp_solver *h)
 {
     h->ev_cur = nullptr;
+    if (h->ev_force) {
+        h->ev_cur = h->ev_force;
+        h->marks_cur = 0;
+        return mark(h);
+    }
     if (!h->timing) return PP_SUCCESS;
     h->ev_cur = &h->ev[(size_t)pp_solver::kMaxMarks * h->ev_next];
     h->ev_next = (h->ev_next + 1) % h->ev_slots;
@@ -159,8 +174,15 @@ int begin_timing(pp_solver *h)
 
 void end_timing(pp_solver *h)
 {
-    if (h->ev_cur) h->marks_last = h->marks_cur;
+    if (h->ev_cur && !h->ev_force) h->marks_last = h->marks_cur;
     h->ev_cur = nullptr;
+}
+
+// launch shape of FP64 line stage i (0 x fwd, 1 y fwd, 2 y inv, 3 x inv) of length N
+int stage_var(const pp_solver *h, int i, int N)
+{
+    const int v = h->force_var >= 0 ? h->force_var : h->line_var[i];
+    return v && line_has_variant(N) ? 1 : 0;
 }
 
 void free_mixed(pp_solver *h)
@@ -341,6 +363,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         a.in = rhs;
         a.in_scale = h->g.dxyz[2];
         a.dmask = h->g.dmask;
+        a.variant = stage_var(h, 0, h->g.nx);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[0], h->g.nx, 0, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
@@ -349,6 +372,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         LineArgs a = line_args(h, 1, field);
         a.out = zdata;
         a.ztiled = zt ? ZT_OUT : 0;
+        a.variant = stage_var(h, 1, h->g.ny);
         a.tnz = ztile_rows(h->g.nz);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
@@ -399,6 +423,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         LineArgs a = line_args(h, 1, field);
         a.in = zdata;
         a.ztiled = zt ? ZT_IN : 0;
+        a.variant = stage_var(h, 2, h->g.ny);
         a.tnz = ztile_rows(h->g.nz);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
@@ -407,10 +432,125 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
     {
         LineArgs a = line_args(h, 0, field);
         a.out_scale = 1. / (h->g.scale);
+        a.variant = stage_var(h, 3, h->g.nx);
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
     end_timing(h);
+    return PP_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------
+// Autotuner of the FP64 form.  The choices, per grid shape and boundary kinds:
+//   z configuration: PCR across 16 lanes then Thomas per lane on z tiles, the same in place,
+//     or partitioned Thomas with cross-lane scans (where the PCR / Thomas split sits);
+//   per line stage: the default launch shape or the alternative one (line_threads_alt:
+//     radix plan and block shape).
+// They are timed on the handle's shape with a scratch field (HIP events between the
+// launches, best of a few solves), kept per process and reused by every handle of the same
+// shape.  Results do not depend on the choice beyond rounding (each configuration is checked
+// against the oracle by the tests).
+// ---------------------------------------------------------------------------------------
+struct TuneChoice {
+    int zsolver, ztiles;
+    int var[4];
+};
+std::mutex g_tune_mu;
+std::map<std::array<int, 7>, TuneChoice> g_tune;
+
+// best-of-`reps` time of each stage and of the whole solve on `buf` (in place)
+int tune_time(pp_solver *h, double *buf, int reps, double stage[5], double &total)
+{
+    constexpr int K = 8;
+    hipEvent_t ev[K];
+    for (int i = 0; i < K; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess) return PP_HIP_ERROR;
+    for (int i = 0; i < 5; i++) stage[i] = 1e30;
+    total = 1e30;
+    int rc = run_fp64(h, buf, buf);  // warm-up (first-use set-up, caches)
+    for (int r = 0; r < reps && !rc; r++) {
+        h->ev_force = ev;
+        rc = run_fp64(h, buf, buf);
+        h->ev_force = nullptr;
+        if (rc) break;
+        if (hipEventSynchronize(ev[h->marks_cur - 1]) != hipSuccess) {
+            rc = PP_HIP_ERROR;
+            break;
+        }
+        for (int i = 0; i + 1 < h->marks_cur && i < 5; i++) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess) rc = PP_HIP_ERROR;
+            stage[i] = std::min(stage[i], (double)t);
+        }
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, ev[0], ev[h->marks_cur - 1]) != hipSuccess) rc = PP_HIP_ERROR;
+        total = std::min(total, (double)t);
+    }
+    h->ev_force = nullptr;
+    for (int i = 0; i < K; i++) (void)hipEventDestroy(ev[i]);
+    return rc;
+}
+
+int autotune(pp_solver *h)
+{
+    const std::array<int, 7> key = {h->g.nx, h->g.ny, h->g.nz, h->g.kind[0], h->g.kind[1], h->g.zlo_d, h->g.zhi_d};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            const TuneChoice &c = it->second;
+            if (h->zsolver <= 1) {
+                h->zsolver = c.zsolver;
+                h->ztiles = c.ztiles;
+            }
+            for (int i = 0; i < 4; i++) h->line_var[i] = c.var[i];
+            h->tuned = true;
+            return PP_SUCCESS;
+        }
+    }
+    const size_t n = (size_t)h->g.nx * h->g.ny * h->g.nz;
+    double *buf = nullptr;
+    if (hipMalloc((void **)&buf, n * sizeof(double)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    int rc = launch_fill_uniform(buf, (long)n, 0x7475ull, 0, h->stream) == hipSuccess ? PP_SUCCESS : PP_HIP_ERROR;
+    const int saved_force = h->force_var;
+    h->force_var = -1;
+    for (int i = 0; i < 4; i++) h->line_var[i] = 0;
+    double st[5], tot, best = 1e30;
+    int bz = h->zsolver, bt = h->ztiles;
+    if (!rc && h->zsolver <= 1) {
+        const int cand[3][2] = {{0, 1}, {0, 0}, {1, 0}};
+        for (const auto &c : cand) {
+            h->zsolver = c[0];
+            h->ztiles = c[1];
+            if (c[1] && !use_ztiles(h)) continue;
+            if ((rc = tune_time(h, buf, 3, st, tot))) break;
+            if (tot < best) {
+                best = tot;
+                bz = c[0];
+                bt = c[1];
+            }
+        }
+        h->zsolver = bz;
+        h->ztiles = bt;
+    }
+    double st0[5], st1[5];
+    if (!rc) rc = tune_time(h, buf, 3, st0, tot);
+    if (!rc) {
+        for (int i = 0; i < 4; i++) h->line_var[i] = 1;
+        rc = tune_time(h, buf, 3, st1, tot);
+        // stages in launch order: x fwd, y fwd, z, y inv, x inv
+        const int si[4] = {0, 1, 3, 4};
+        const int len[4] = {h->g.nx, h->g.ny, h->g.ny, h->g.nx};
+        for (int i = 0; i < 4; i++) h->line_var[i] = (line_has_variant(len[i]) && st1[si[i]] < st0[si[i]]) ? 1 : 0;
+    }
+    h->force_var = saved_force;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    h->tuned = true;
+    TuneChoice c{h->zsolver, h->ztiles, {h->line_var[0], h->line_var[1], h->line_var[2], h->line_var[3]}};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = c;
     return PP_SUCCESS;
 }
 
@@ -648,7 +788,13 @@ int run_mixed(pp_solver *h, const double *f, double *u)
 // dispatch on the precision form; rhs and out may alias (the mixed form then works on a copy)
 int run_solve(pp_solver *h, const double *rhs, double *out)
 {
-    if (h->precision == 0) return run_fp64(h, rhs, out);
+    if (h->precision == 0) {
+        if (h->autotune && !h->tuned && !h->g.zper) {
+            const int rc = autotune(h);
+            if (rc) return rc;
+        }
+        return run_fp64(h, rhs, out);
+    }
     if (rhs == out) {
         const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
         if (!h->d_aux && hipMalloc((void **)&h->d_aux, bytes) != hipSuccess) return PP_ALLOCATION_FAIL;
@@ -840,6 +986,17 @@ int pp_set_option(pp_handle h, int key, long long value)
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->ztiles = (int)value;
         return PP_SUCCESS;
+    case PP_OPT_AUTOTUNE:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->autotune = (int)value;
+        h->tuned = false;
+        if (!value)
+            for (int i = 0; i < 4; i++) h->line_var[i] = 0;
+        return PP_SUCCESS;
+    case PP_OPT_LINE_VARIANT:
+        if (value < -1 || value > 1) return PP_INPUT_ARGS;
+        h->force_var = (int)value;
+        return PP_SUCCESS;
     case PP_OPT_PRECISION:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->precision = (int)value;
@@ -864,6 +1021,17 @@ int pp_set_option(pp_handle h, int key, long long value)
     }
     default: return PP_INPUT_ARGS;
     }
+}
+
+int pp_get_tuning(pp_handle h, int *out, int n)
+{
+    if (!h) return PP_INVALID_HANDLE;
+    if (!out || n < 7) return PP_INPUT_ARGS;
+    out[0] = h->tuned ? 1 : 0;
+    out[1] = h->zsolver;
+    out[2] = use_ztiles(h) ? 1 : 0;
+    for (int i = 0; i < 4; i++) out[3 + i] = stage_var(h, i, i == 0 || i == 3 ? h->g.nx : h->g.ny);
+    return PP_SUCCESS;
 }
 
 int pp_get_kernel_times(pp_handle h, float *ms, int max, int *count)

@@ -169,3 +169,46 @@ def test_periodic_z_residual_and_limits():
     with pytest.raises(pa.PittPackError) as e:
         pa.solve(f, "NNNNPP", precision=1)
     assert e.value.code == pa.UNSUPPORTED_BC
+
+
+# line lengths with an alternative launch shape (radix plan / block) from 16 to 512
+VARIANT_SHAPES = [(10, 16, 32), (20, 24, 48), (6, 128, 256), (4, 64, 512), (5, 256, 128)]
+
+
+@pytest.mark.parametrize("shape", VARIANT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", ["PPPPNN", "NNNNDD", "DDNDDD", "DDDDNN", "NDDNDN"])
+def test_parity_line_variant(bc, shape):
+    """every FP64 line stage in its alternative launch shape (the autotuner's other choice)"""
+    print(f"line variant {bc} {shape}", flush=True)
+    rng = np.random.default_rng(sum(map(ord, bc)) + shape[2])
+    f = rng.uniform(-1, 1, shape)
+    box = [0, 1, 0, 1, 0, 1]
+    ref = oracle.solve(f, bc, box, zero_mode=1)
+    got = pa.solve(f, bc, box, zero_mode=1, line_variant=1)
+    err = rel_maxdiff(got, ref)
+    assert err <= TOL, f"{bc} {shape}: rel max diff {err:.3e}"
+
+
+def test_autotune_choice_and_parity():
+    """the autotuner times the candidates, keeps a configuration, and the solve still meets
+    the FP64 bar; a second handle of the same shape reuses the cached choice"""
+    print("autotune 64x96x128", flush=True)
+    rng = np.random.default_rng(7)
+    shape = (64, 96, 128)
+    f = rng.uniform(-1, 1, shape)
+    box = [0, 1, 0, 1, 0, 1]
+    ref = oracle.solve(f, "DDNDDD", box, zero_mode=1)
+    for _ in range(2):
+        with pa.PoissonHIP(shape[2], shape[1], shape[0]) as M:
+            M.set_option(pa.OPT_AUTOTUNE, 1)
+            M.assignBoundary("DDNDDD")
+            M.setBox(box)
+            M.setCoords(2)
+            M.assignRhs(f)
+            M.pittPack()
+            got = M.getSolution()
+            t = M.tuning()
+        assert t["tuned"], t
+        assert t["zsolver"] in (0, 1)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"autotuned {t}: rel max diff {err:.3e}"

@@ -70,7 +70,7 @@ XY_SIDES = [32, 48, 64, 96, 128, 192]
 SERVE_SIDES = [32, 48, 64, 96, 128, 192, 256, 384, 512]  # compiled line lengths in [32, 512]
 
 
-def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8):
+def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, autotune: bool = True):
     """4096 independent solves (nx, ny from {32..192}, nz in [16, 256], BC uniform over four
     x/y combos x z in {DD, NN}, RHS U[-1,1] rounded to FP32 from seed 2), mixed form with
     `refine` steps; the same batch in FP64 gives the error reference and its own rate."""
@@ -87,7 +87,8 @@ def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8):
     out = {}
     ref = {}
     for prec in (0, 1):
-        B = pa.Batch(precision=prec, refine=refine if prec else None)
+        # FP64: line classes tuned at finalize (before the right-hand sides are written)
+        B = pa.Batch(precision=prec, refine=refine if prec else None, autotune=int(autotune and prec == 0))
         for nx, ny, nz, bc in members:
             B.add(nx, ny, nz, bc)
         t0 = time.perf_counter()
@@ -116,6 +117,8 @@ def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8):
             ref = sols
         res = {"ms_per_step": ms, "value": pts / (ms * 1e-3), "launches_per_stage_xyz": launches,
                "plan_and_upload_s": plan_s, "hip_graph": True, "kernels_ms": kt}
+        if prec == 0:
+            res["alternative_shapes"] = B.tuning()
         if prec == 1:
             errs = [float(np.abs(sols[i] - ref[i]).max() / np.abs(ref[i]).max()) for i in picks]
             res["rel_max_diff_vs_fp64_sampled"] = max(errs)
@@ -394,7 +397,8 @@ def main():
     if args.large > 0:
         out["large_mixed"] = bench_large_mixed(pa, torch, args.large, args.refine, max(1, min(args.steps, 5)))
     if args.batch_count > 0:
-        out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine)
+        out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine,
+                                       autotune=not args.no_autotune)
     if args.serve_requests > 0:
         out["serving"] = bench_serving(pa, args.serve_requests, args.serve_rate, 200.0, 4)
     if not args.no_cpu_baseline:

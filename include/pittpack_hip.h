@@ -70,7 +70,9 @@ enum {
      * launches into a HIP graph and later solves replay it (one submission per solve);
      * 0 = every solve launches its kernels.  Timing (PP_OPT_TIMING) runs eagerly. */
     PP_OPT_GRAPH = 10,
-    /* FP64 form: 1 = before the next solve, time the candidate configurations on this shape
+    /* FP64 form: 1 = before the next solve (batched form: at pp_batch_finalize, per line
+     * class; the batch's right-hand sides are set afterwards), time the candidate
+     * configurations on this shape
      * (z: PCR->Thomas on z tiles, the same in place, or partitioned Thomas; each line stage:
      * default or alternative radix plan / block shape) and keep the fastest; the choice is
      * cached per process and shape.  0 (default) = fixed default configuration. */
@@ -191,6 +193,10 @@ int pp_batch_get_rhs(pp_batch b, int id, double *rhs);
  * optionally rounded to FP32 */
 int pp_batch_fill_uniform(pp_batch b, unsigned long long seed, int round_f32);
 int pp_batch_set_stream(pp_batch b, void *stream);
+/* Number of (line class, direction) pairs of an FP64 batch running the alternative launch
+ * shape after PP_OPT_AUTOTUNE.  No counterpart in the reference. */
+int pp_batch_tuning(pp_batch b, int *alt_shapes);
+
 /* PP_OPT_ZERO_MODE, PP_OPT_PRECISION (before pp_batch_add), PP_OPT_REFINE, PP_OPT_TIMING */
 int pp_batch_set_option(pp_batch b, int key, long long value);
 int pp_batch_set_option_double(pp_batch b, int key, double value);

@@ -53,7 +53,7 @@ EXPORTED = (
     "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
     "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
     "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
-    "pp_server_stats",
+    "pp_server_stats", "pp_batch_tuning",
 )
 
 
@@ -102,6 +102,7 @@ def load_library(path: str | None = None):
     L.pp_set_option_double.argtypes = [vp, i, ctypes.c_double]
     L.pp_get_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, ctypes.POINTER(i)]
     L.pp_get_tuning.argtypes = [vp, ctypes.POINTER(i), i]
+    L.pp_batch_tuning.argtypes = [vp, ctypes.POINTER(i)]
     L.pp_get_device_field.argtypes = [vp, ctypes.POINTER(vp)]
     L.pp_get_error_enum.argtypes = [i]
     L.pp_get_error_enum.restype = ctypes.c_char_p
@@ -276,13 +277,18 @@ class Batch:
     """
 
     def __init__(self, precision: int = 0, refine: int | None = None, tau: float | None = None,
-                 zero_mode: int | None = None, graph: int | None = None):
+                 zero_mode: int | None = None, graph: int | None = None, autotune: int | None = None,
+                 line_variant: int | None = None):
         self._L = load_library()
         self._h = ctypes.c_void_p()
         _check(self._L.pp_batch_create(ctypes.byref(self._h)), "pp_batch_create")
         self.shapes = []
         if graph is not None:
             self.set_option(OPT_GRAPH, graph)
+        if autotune is not None:
+            self.set_option(OPT_AUTOTUNE, autotune)
+        if line_variant is not None:
+            self.set_option(OPT_LINE_VARIANT, line_variant)
         if precision:
             self.set_option(OPT_PRECISION, precision)
         if refine is not None:
@@ -351,6 +357,12 @@ class Batch:
 
     def solve(self) -> None:
         _check(self._L.pp_batch_solve(self._h), "batch_solve")
+
+    def tuning(self) -> int:
+        """(line class, direction) pairs running the alternative launch shape (autotuned)."""
+        n = ctypes.c_int(0)
+        _check(self._L.pp_batch_tuning(self._h, ctypes.byref(n)), "batch_tuning")
+        return n.value
 
     def synchronize(self) -> None:
         _check(self._L.pp_batch_synchronize(self._h), "batch_synchronize")

@@ -143,6 +143,16 @@ int BatchCore::plan_lines(int axis, std::vector<HostClass> &out)
         c.kind = kv.first.second;
         c.nitems = (int)kv.second.size();
         c.li = std::move(kv.second);
+        if (sizeof(R) == 8 && axis == 0 && line_has_variant(c.key)) {
+            // x blocks of the alternative shape hold a different number of lines
+            const int la = line_block_lines_alt(c.key, 0);
+            for (int m = 0; m < (int)mem.size(); m++) {
+                const SolveSetup &g = mem[m].g;
+                if (g.nx != c.key || g.kind[0] != c.kind) continue;
+                const long nl = (long)g.ny * g.nz;
+                for (long bx = 0; bx < (nl + la - 1) / la; bx++) c.li_alt.push_back({m, (int)bx, 0});
+            }
+        }
         out.push_back(std::move(c));
     }
     return PP_SUCCESS;
@@ -304,7 +314,10 @@ int BatchCore::plan()
     o_ls_ = take(c, ls_.size() * sizeof(LineSolve));
     o_zs_ = take(c, zs_.size() * sizeof(ZSolve));
     for (auto *v : {&lx_, &ly_})
-        for (HostClass &h : *v) h.off = take(c, h.li.size() * sizeof(LineItem));
+        for (HostClass &h : *v) {
+            h.off = take(c, h.li.size() * sizeof(LineItem));
+            if (!h.li_alt.empty()) h.off_alt = take(c, h.li_alt.size() * sizeof(LineItem));
+        }
     for (HostClass &h : lz_) h.off = take(c, h.zi.size() * sizeof(ZItem));
     if (precision == 1) {
         o_sums_ = take(c, (size_t)nsums_ * sizeof(double));
@@ -333,7 +346,10 @@ int BatchCore::bind(char *base, TableCache *tc)
     d_ls_ = (LineSolve *)(base + o_ls_);
     d_zs_ = (ZSolve *)(base + o_zs_);
     for (auto *v : {&lx_, &ly_, &lz_})
-        for (HostClass &h : *v) h.d_items = base + h.off;
+        for (HostClass &h : *v) {
+            h.d_items = base + h.off;
+            h.d_items_alt = h.li_alt.empty() ? nullptr : base + h.off_alt;
+        }
     int rc;
     for (auto *v : {&lx_, &ly_})
         for (HostClass &h : *v) {
@@ -417,7 +433,9 @@ int BatchCore::upload(hipStream_t s)
         return rc;
     for (auto *v : {&lx_, &ly_})
         for (HostClass &h : *v)
-            if ((rc = put((LineItem *)h.d_items, h.li, s))) return rc;
+            if ((rc = put((LineItem *)h.d_items, h.li, s)) ||
+                (!h.li_alt.empty() && (rc = put((LineItem *)h.d_items_alt, h.li_alt, s))))
+                return rc;
     for (HostClass &h : lz_)
         if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
     if (precision == 1) {
@@ -436,9 +454,21 @@ int BatchCore::upload(hipStream_t s)
 int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
                               void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled)
 {
-    for (const HostClass &c : cls) {
+    int rc;
+    for (const HostClass &c : cls)
+        if ((rc = launch_class(s, c, form, axis, fwd, in, out, bflags, sums, plane, accumulate, ztiled))) return rc;
+    return PP_SUCCESS;
+}
+
+int BatchCore::launch_class(hipStream_t s, const HostClass &c, int form, int axis, int fwd, const void *in, void *out,
+                            int bflags, double *sums, const double *plane, int accumulate, int ztiled)
+{
+    {
+        const int v = (form == FORM_F64 && line_has_variant(c.key)) ? (force_var >= 0 ? force_var : c.var[fwd ? 0 : 1]) : 0;
+        const bool alt_items = v && axis == 0;
         const AxisDev &t = tabs_.at(c.key);
         LineArgs a{};
+        a.variant = v;
         a.in = in;
         a.out = out;
         a.tw = t.tw;
@@ -448,9 +478,9 @@ int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int fo
         a.D = t.D;
         a.in_scale = 1.0;
         a.out_scale = 1.0;
-        a.items = (const LineItem *)c.d_items;
+        a.items = (const LineItem *)(alt_items ? c.d_items_alt : c.d_items);
         a.solves = d_ls_;
-        a.nlines = c.nitems;
+        a.nlines = alt_items ? (long)c.li_alt.size() : c.nitems;
         a.bflags = bflags;
         a.line_sums = sums;
         a.plane_add = plane;
@@ -459,6 +489,61 @@ int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int fo
         PPB_CHECK(launch_line(form, c.kind, c.key, axis, fwd, a, s));
     }
     return PP_SUCCESS;
+}
+
+int BatchCore::alt_classes() const
+{
+    int n = 0;
+    for (const auto *v : {&lx_, &ly_})
+        for (const HostClass &c : *v) n += c.var[0] + c.var[1];
+    return n;
+}
+
+int BatchCore::tune_lines(hipStream_t s)
+{
+    if (!bound || precision != 0) return PP_CALL_ORDER;
+    int rc = PP_SUCCESS;
+    for (const BatchMember &m : mem)
+        PPB_CHECK(launch_fill_uniform(d_rhs + m.offset, (long)m.g.nx * m.g.ny * m.g.nz, 0x7475ull, 0, s));
+    if ((rc = run(s, nullptr))) return rc;  // warm-up, and every stage's input is finite
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_HIP_ERROR;
+    // the four FP64 line stages in run_fp64's order and arguments
+    struct Stage {
+        std::vector<HostClass> *cls;
+        int axis, fwd;
+        const void *in;
+        void *out;
+        int bflags, ztiled;
+    };
+    const Stage st[4] = {{&lx_, 0, 1, d_rhs, d_sol, BF_RHS, 0},
+                         {&ly_, 1, 1, d_sol, d_tiles, 0, ZT_OUT},
+                         {&ly_, 1, 0, d_tiles, d_sol, 0, ZT_IN},
+                         {&lx_, 0, 0, d_sol, d_sol, BF_SCALE, 0}};
+    const int saved_force = force_var;
+    force_var = -1;
+    for (const Stage &g : st) {
+        for (HostClass &c : *g.cls) {
+            if (!line_has_variant(c.key)) continue;
+            float best[2] = {1e30f, 1e30f};
+            for (int v = 0; v < 2 && !rc; v++) {
+                c.var[g.fwd ? 0 : 1] = v;
+                for (int r = 0; r < 3 && !rc; r++) {
+                    if (hipEventRecord(e0, s) != hipSuccess) rc = PP_HIP_ERROR;
+                    if (!rc) rc = launch_class(s, c, FORM_F64, g.axis, g.fwd, g.in, g.out, g.bflags, nullptr, nullptr, 0, g.ztiled);
+                    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = PP_HIP_ERROR;
+                    float ms = 0.f;
+                    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PP_HIP_ERROR;
+                    if (!rc) best[v] = std::min(best[v], ms);
+                }
+            }
+            c.var[g.fwd ? 0 : 1] = best[1] < best[0] ? 1 : 0;
+        }
+    }
+    force_var = saved_force;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 // z stage over every rows-per-lane class; FP32 with the mixed form's column rules
@@ -581,6 +666,7 @@ struct pp_batch_s {
     // PP_OPT_GRAPH: the solve's launches captured once into a HIP graph and replayed (the
     // first solve runs eagerly, so one-time launch set-up happens outside the capture)
     int graph_mode = 1;
+    int autotune = 0;  // PP_OPT_AUTOTUNE: FP64 line classes tuned at finalize
     int solves = 0;
     hipGraphExec_t graph = nullptr;
     void drop_graph()
@@ -641,7 +727,20 @@ int pp_batch_finalize(pp_batch b)
     }
     if ((rc = b->core.bind(b->region, &b->tables)) || (rc = b->core.upload(b->stream))) return rc;
     PPB_CHECK(hipStreamSynchronize(b->stream));
+    if (b->autotune && b->core.precision == 0) {
+        if ((rc = b->core.tune_lines(b->stream))) return rc;
+        PPB_CHECK(hipStreamSynchronize(b->stream));
+    }
     b->finalized = true;
+    return PP_SUCCESS;
+}
+
+int pp_batch_tuning(pp_batch b, int *alt_shapes)
+{
+    if (!b) return PP_INVALID_HANDLE;
+    if (!b->finalized) return PP_CALL_ORDER;
+    if (!alt_shapes) return PP_INPUT_ARGS;
+    *alt_shapes = b->core.alt_classes();
     return PP_SUCCESS;
 }
 
@@ -763,6 +862,17 @@ int pp_batch_set_option(pp_batch b, int key, long long value)
             b->drop_graph();
         }
         b->core.refine = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_AUTOTUNE:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        if (b->finalized) return PP_CALL_ORDER;
+        b->autotune = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_LINE_VARIANT:
+        if (value < -1 || value > 1) return PP_INPUT_ARGS;
+        PPB_CHECK(hipStreamSynchronize(b->stream));
+        b->drop_graph();
+        b->core.force_var = (int)value;
         return PP_SUCCESS;
     case PP_OPT_GRAPH:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;

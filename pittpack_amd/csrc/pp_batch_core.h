@@ -55,6 +55,7 @@ class BatchCore {
     int refine = 2;
     int zero_mode = 1;
     double tau = 0.05;
+    int force_var = -1;  // FP64 line classes: -1 tuned / default shapes, 0 default, 1 alternative
 
     std::vector<BatchMember> mem;
 
@@ -68,6 +69,10 @@ class BatchCore {
     int upload(hipStream_t s);
     // the solve of every member (right-hand sides kept, solutions in the solution arena)
     int run(hipStream_t s, Marks *mk);
+    // FP64: time each line class in both launch shapes on this batch (bound, uploaded) and
+    // keep the faster per class and direction; the right-hand-side arena is overwritten
+    int tune_lines(hipStream_t s);
+    int alt_classes() const;
 
     long long points() const;
     size_t region_bytes = 0;
@@ -87,7 +92,15 @@ class BatchCore {
         std::vector<ZItem> zi;
         size_t off = 0;
         void *d_items = nullptr;
+        // FP64 line classes: launch shape per direction (forward, inverse) and, for x, the
+        // block items of the alternative shape (its lines per block differ)
+        int var[2] = {0, 0};
+        std::vector<LineItem> li_alt;
+        size_t off_alt = 0;
+        void *d_items_alt = nullptr;
     };
+    int launch_class(hipStream_t s, const HostClass &c, int form, int axis, int fwd, const void *in, void *out,
+                     int bflags, double *sums, const double *plane, int accumulate, int ztiled);
     template <typename R>
     int plan_lines(int axis, std::vector<HostClass> &out);
     void plan_z();

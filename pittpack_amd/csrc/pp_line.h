@@ -728,4 +728,20 @@ inline int line_block_lines(int N, int AX)
     }
 }
 
+// lines (x) or columns (y) per block of the alternative FP64 shape (y keeps the default's)
+inline int line_block_lines_alt(int N, int AX)
+{
+    if (AX == 1) return line_block_lines<double>(N, 1);
+    switch (N / 2) {
+#define PPM(m)                                                                                  \
+    case m:                                                                                     \
+        if constexpr (line_threads_alt(m) > 0) return line_lines<double, m, 0, line_threads_alt(m)>(); \
+        else return 0;
+        PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256)
+        PPM(384) PPM(512)
+#undef PPM
+    default: return 0;
+    }
+}
+
 }  // namespace pp

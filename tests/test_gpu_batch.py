@@ -152,3 +152,36 @@ def test_graph_replays_match_eager(precision):
     finally:
         E.close()
         G.close()
+
+
+def _batch_vs_oracle(B, tag):
+    ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in MEMBERS]
+    B.finalize()
+    rhs = []
+    for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+        f = prng.uniform_pm1((nz, ny, nx), seed=300 + i)
+        rhs.append(f)
+        B.set_rhs(i, f)
+    B.solve()
+    for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+        ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+        err = rel_maxdiff(B.get_solution(i), ref)
+        assert err <= TOL, f"{tag} member {i} {bc} ({nx},{ny},{nz}): rel max diff {err:.3e}"
+
+
+def test_batch_alternative_shapes_vs_oracle():
+    """every FP64 line class of the batch in its alternative launch shape (x classes then use
+    their own block items)"""
+    print("batch alternative shapes", flush=True)
+    with pa.Batch(line_variant=1) as B:
+        _batch_vs_oracle(B, "alternative shapes")
+
+
+def test_batch_autotune_vs_oracle():
+    """PP_OPT_AUTOTUNE on a batch: each line class and direction timed in both shapes at
+    finalize, the faster kept; the tuned batch still meets the FP64 bar"""
+    print("batch autotune", flush=True)
+    with pa.Batch(autotune=1) as B:
+        _batch_vs_oracle(B, "autotuned")
+        n = B.tuning()
+        assert 0 <= n <= 2 * (B.launch_count()[0] + B.launch_count()[1])

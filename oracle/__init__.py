@@ -48,9 +48,16 @@ def lib():
         L.ppo_apply_operator.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, dp, dp]
         L.ppo_apply_operator.restype = ctypes.c_int
         L.ppo_eigenvalue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, ctypes.c_int, ctypes.c_int]
+        L.ppo_threads.argtypes = []
+        L.ppo_threads.restype = ctypes.c_int
         L.ppo_eigenvalue.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+def threads() -> int:
+    """OpenMP threads the oracle's line, plane and column loops use."""
+    return int(lib().ppo_threads())
 
 
 def _dp(a: np.ndarray):

@@ -22,6 +22,9 @@
  * row-0 rule under z-NN beyond "rows 1..n-1 of the original system hold and u0=r0", and
  * the singular periodic-z averaging of shermanMorrisonThomasV1.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -627,8 +630,13 @@ static void ppo_lines(double *P, long nlines, int size, int kind, int inverse)
     ppo_fft_plan pl, pl2;
     ppo_fft_plan_init(&pl, size);
     ppo_fft_plan_init(&pl2, 2 * size);
+    /* lines are independent: OpenMP threads split them, each with its own work buffers (the
+     * plans are read-only), so the result is bitwise the serial one */
+#pragma omp parallel
+    {
     double *W = (double *)malloc(sizeof(double) * 4 * (size_t)size);
     double *Z = (double *)malloc(sizeof(double) * 4 * (size_t)size);
+#pragma omp for schedule(static)
     for (long l = 0; l < nlines; l++) {
         double *L = P + 2 * (size_t)size * l;
         if (kind == 3 || kind == 4) {
@@ -654,13 +662,19 @@ static void ppo_lines(double *P, long nlines, int size, int kind, int inverse)
     }
     free(W);
     free(Z);
+    }
     ppo_fft_plan_free(&pl);
     ppo_fft_plan_free(&pl2);
 }
 
 /* rearrangeX2Y (2279-2380): per plane [k][j][i] -> [k][i][j] (complex), via scratch */
-static void ppo_x2y(double *P, double *R, int nx, int ny, int nz, int inverse)
+static void ppo_x2y(double *P, double *R0, int nx, int ny, int nz, int inverse)
 {
+    (void)R0;
+#pragma omp parallel
+    {
+    double *R = (double *)malloc(sizeof(double) * 2 * (size_t)nx * ny);
+#pragma omp for schedule(static)
     for (int k = 0; k < nz; k++) {
         double *pl = P + 2 * (size_t)nx * ny * k;
         for (int j = 0; j < ny; j++)
@@ -671,6 +685,8 @@ static void ppo_x2y(double *P, double *R, int nx, int ny, int nz, int inverse)
                 R[2 * d + 1] = pl[2 * s + 1];
             }
         memcpy(pl, R, sizeof(double) * 2 * (size_t)nx * ny);
+    }
+    free(R);
     }
 }
 
@@ -694,6 +710,7 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
     if (!P || !R) { free(P); free(R); return 15; }
     /* assignRhs (5591-5647): P = rhs * c3 * c3, Im = 0 */
     double c3 = S.dxyz[2];
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; q++) { P[2 * q] = rhs[q] * c3 * c3; P[2 * q + 1] = 0.0; }
     /* modifyRhsDirichlet (5890) */
     ppo_dirichlet(&S, P);
@@ -705,11 +722,14 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
     ppo_lines(P, (long)nx * nz, ny, S.kind[1], 0);
     /* z solve (6105-6115): per (i = y-mode, j = x-mode), y fastest layout */
     {
+        int nidx = (bc[0] == 'P' || bc[2] == 'P') ? 2 : 1;
+#pragma omp parallel
+        {
         double *line = (double *)malloc(sizeof(double) * (size_t)nz);
         double *tmp = (double *)calloc((size_t)nz + 1, sizeof(double));
         double *rh1 = (double *)calloc((size_t)nz + 1, sizeof(double));
-        int nidx = (bc[0] == 'P' || bc[2] == 'P') ? 2 : 1;
         for (int index = 0; index < nidx; index++) {
+#pragma omp for schedule(static)
             for (int j = 0; j < nx; j++)
                 for (int i = 0; i < ny; i++) {
                     double eig = ppo_eig(&S, i, j);
@@ -724,6 +744,7 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
         free(line);
         free(tmp);
         free(rh1);
+        }
     }
     /* inverse y (6208-6215) */
     ppo_lines(P, (long)nx * nz, ny, S.kind[1], 1);
@@ -732,6 +753,7 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
     /* inverse x (6287-6294) */
     ppo_lines(P, (long)ny * nz, nx, S.kind[0], 1);
     /* rescale (3503-3531) */
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; q++) {
         P[2 * q] = 1. / (S.scale) * P[2 * q];
         P[2 * q + 1] = 1. / (S.scale) * P[2 * q + 1];
@@ -802,4 +824,14 @@ int ppo_apply_operator(int nx, int ny, int nz, const char *bc, const double *X, 
                 out[q] = acc;
             }
     return 0;
+}
+
+/* threads the oracle's line, plane and column loops use (1 without OpenMP) */
+int ppo_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

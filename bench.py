@@ -254,6 +254,7 @@ def main():
     ap.add_argument("--zsolver", type=int, default=None,
                     help="0 PCR->Thomas, 1 partitioned Thomas, 2 multigrid, 3 full PCR, 4 cyclic reduction")
     ap.add_argument("--no-autotune", action="store_true", help="FP64 form in the fixed default configuration")
+    ap.add_argument("--no-graph", action="store_true", help="launch every kernel of every solve (no HIP graph)")
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
@@ -271,6 +272,8 @@ def main():
 
     n = args.n
     dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)  # default priority; graphs need a non-null stream
+    torch.cuda.set_stream(stream)
     f_host = rng.uniform_pm1((n, n, n), seed=0)
     f = torch.from_numpy(f_host).to(dev)
     u = torch.empty_like(f)
@@ -291,18 +294,23 @@ def main():
         if precision:
             M.set_option(pa.OPT_PRECISION, precision)
             M.set_option(pa.OPT_REFINE, args.refine)
-        stream = torch.cuda.current_stream(dev)
         M.set_stream(stream.cuda_stream)
-        for _ in range(args.warmup):
+        # repeated solves on the same pointers replay a captured HIP graph (the first warm-up
+        # solve runs eagerly -- and tunes the FP64 form -- the second is captured)
+        M.set_option(pa.OPT_GRAPH, 0 if args.no_graph else 1)
+        for _ in range(max(2, args.warmup)):
             M.solve_device_oop(f.data_ptr(), u.data_ptr())
-        torch.cuda.synchronize()
-        M.set_option(pa.OPT_TIMING, max(1, args.steps))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             M.solve_device_oop(f.data_ptr(), u.data_ptr())
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        # per-kernel durations: HIP events between the launches of `steps` more (eager) solves
+        M.set_option(pa.OPT_TIMING, max(1, args.steps))
+        for _ in range(args.steps):
+            M.solve_device_oop(f.data_ptr(), u.data_ptr())
+        torch.cuda.synchronize()
         return (t1 - t0) * 1e3 / args.steps, M.kernel_times(), M
 
     def roofline(precision: int, kt):

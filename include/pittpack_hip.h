@@ -68,7 +68,12 @@ enum {
     PP_OPT_MG_TOL = 9, /* pp_set_option_double: multigrid residual tolerance */
     /* batched form: 1 (default) = the second pp_batch_solve captures the solve's kernel
      * launches into a HIP graph and later solves replay it (one submission per solve);
-     * 0 = every solve launches its kernels.  Timing (PP_OPT_TIMING) runs eagerly. */
+     * 0 = every solve launches its kernels.  Timing (PP_OPT_TIMING) runs eagerly.
+     * Single-solve handle (default 0): 1 = repeated pp_solve_device / pp_solve_device_oop
+     * calls on the same device pointers (the flow-solver pattern) run the first call
+     * eagerly, capture the second into a graph and replay it from then on; any option, box
+     * or boundary change, or a new stream, drops it.  Needs a non-null stream; an in-place
+     * mixed-form solve runs eagerly. */
     PP_OPT_GRAPH = 10,
     /* FP64 form: 1 = before the next solve (batched form: at pp_batch_finalize, per line
      * class; the batch's right-hand sides are set afterwards), time the candidate

@@ -107,6 +107,13 @@ struct pp_solver {
     bool tuned = false;
     int line_var[4] = {0, 0, 0, 0};
     int force_var = -1;                         // PP_OPT_LINE_VARIANT (every stage; -1: tuner's)
+    // PP_OPT_GRAPH: repeated solves on the same device pointers replay a HIP graph of the
+    // solve's launches (captured on the second such solve; any configuration change drops it)
+    int graph_mode = 0;
+    hipGraphExec_t graph = nullptr;
+    const void *g_rhs = nullptr;
+    const void *g_out = nullptr;
+    int g_eager = 0;
     hipEvent_t *ev_force = nullptr;             // tuner's own events for the launch marks
     int mg_cycles = 200;                        // multigrid z solver: V-cycle limit
     double mg_tol = 1e-16;                      //   and relative max-norm residual tolerance
@@ -176,6 +183,18 @@ void end_timing(pp_solver *h)
 {
     if (h->ev_cur && !h->ev_force) h->marks_last = h->marks_cur;
     h->ev_cur = nullptr;
+}
+
+// drop the captured solve (after the stream has drained: a replay may still be running)
+void drop_graph(pp_solver *h)
+{
+    if (h->graph) {
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->graph);
+    }
+    h->graph = nullptr;
+    h->g_rhs = h->g_out = nullptr;
+    h->g_eager = 0;
 }
 
 // launch shape of FP64 line stage i (0 x fwd, 1 y fwd, 2 y inv, 3 x inv) of length N
@@ -786,6 +805,51 @@ int run_mixed(pp_solver *h, const double *f, double *u)
 }
 
 // dispatch on the precision form; rhs and out may alias (the mixed form then works on a copy)
+int run_solve(pp_solver *h, const double *rhs, double *out);
+
+// a solve through the graph path when enabled: the first solve on a pointer pair runs
+// eagerly (tables, buffers, the tuner), the second is captured -- kernel launches only, no
+// memset or copy is recorded -- and later ones replay it
+int solve_graphed(pp_solver *h, const double *rhs, double *out)
+{
+    const bool graphable = h->graph_mode && !h->timing && h->stream != nullptr &&
+                           !(h->precision == 0 && h->autotune && !h->tuned) && !(h->precision == 1 && rhs == out);
+    if (!graphable) {
+        if (h->graph) drop_graph(h);
+        return run_solve(h, rhs, out);
+    }
+    if (h->g_rhs != rhs || h->g_out != out) {
+        drop_graph(h);
+        h->g_rhs = rhs;
+        h->g_out = out;
+    }
+    if (h->graph) {
+        PP_HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+        return PP_SUCCESS;
+    }
+    if (h->g_eager == 0) {
+        const int rc = run_solve(h, rhs, out);
+        if (!rc) h->g_eager = 1;
+        return rc;
+    }
+    hipGraph_t g = nullptr;
+    PP_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    const int rc = run_solve(h, rhs, out);
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : PP_HIP_ERROR;
+    }
+    const hipError_t ei = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        h->graph = nullptr;
+        return PP_HIP_ERROR;
+    }
+    PP_HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+    return PP_SUCCESS;
+}
+
 int run_solve(pp_solver *h, const double *rhs, double *out)
 {
     if (h->precision == 0) {
@@ -841,6 +905,7 @@ int pp_destroy(pp_handle h)
 {
     if (!h) return PP_INVALID_HANDLE;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
     free_tables(h);
     free_mixed(h);
     if (h->d_zt) (void)hipFree(h->d_zt);
@@ -856,6 +921,7 @@ int pp_destroy(pp_handle h)
 int pp_assign_boundary(pp_handle h, const char *bc)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     if (!bc) return PP_INPUT_ARGS;
     const int rc = setup_boundary(h->g, bc);
     if (rc) return rc;
@@ -867,6 +933,7 @@ int pp_assign_boundary(pp_handle h, const char *bc)
 int pp_set_box(pp_handle h, const double *X)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     if (!X) return PP_INPUT_ARGS;
     for (int i = 0; i < 6; i++) h->g.X[i] = X[i];
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
@@ -879,6 +946,7 @@ int pp_set_box(pp_handle h, const double *X)
 int pp_set_coords(pp_handle h, int dir)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     if (dir < 0 || dir > 2) return PP_INPUT_ARGS;
     setup_coords(h->g);
     h->coords_set = true;
@@ -926,7 +994,7 @@ int pp_solve_device(pp_handle h, double *d_field)
     if (!d_field) return PP_INPUT_ARGS;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    return run_solve(h, d_field, d_field);
+    return solve_graphed(h, d_field, d_field);
 }
 
 int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u)
@@ -935,7 +1003,7 @@ int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u)
     if (!d_rhs || !d_u) return PP_INPUT_ARGS;
     int rc = ensure_ready(h);
     if (rc) return rc;
-    return run_solve(h, d_rhs, d_u);
+    return solve_graphed(h, d_rhs, d_u);
 }
 
 int pp_get_solution(pp_handle h, double *u)
@@ -960,6 +1028,7 @@ int pp_get_solution_device(pp_handle h, double *d_u)
 int pp_set_stream(pp_handle h, void *stream)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (h->own_stream) {
         (void)hipStreamDestroy(h->stream);
@@ -972,6 +1041,7 @@ int pp_set_stream(pp_handle h, void *stream)
 int pp_set_option(pp_handle h, int key, long long value)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     switch (key) {
     case PP_OPT_ZERO_MODE:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
@@ -996,6 +1066,10 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_LINE_VARIANT:
         if (value < -1 || value > 1) return PP_INPUT_ARGS;
         h->force_var = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_GRAPH:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->graph_mode = (int)value;
         return PP_SUCCESS;
     case PP_OPT_PRECISION:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
@@ -1072,6 +1146,7 @@ int pp_get_stage_times(pp_handle h, float *ms5)
 int pp_set_option_double(pp_handle h, int key, double value)
 {
     if (!h) return PP_INVALID_HANDLE;
+    drop_graph(h);
     if (key == PP_OPT_MG_TOL) {
         if (!(value > 0.0 && value < 1.0)) return PP_INPUT_ARGS;
         h->mg_tol = value;

@@ -100,3 +100,44 @@ def test_mixed_device_forms():
         M.solve_device(fd.data_ptr())
         torch.cuda.synchronize()
         assert rel_maxdiff(fd.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_handle_graph_replays_match_eager(precision):
+    """PP_OPT_GRAPH on a single-solve handle (the flow-solver pattern): repeated device solves
+    on the same pointers replay a captured graph.  Every replay, with torch work (new
+    right-hand sides) and synchronisations between them, must equal an eager handle's result
+    bit for bit."""
+    import torch
+
+    print(f"handle graph replays precision={precision}", flush=True)
+    dev = torch.device("cuda:0")
+    nz, ny, nx = 40, 48, 64
+    stream = torch.cuda.Stream(device=dev)
+    handles = []
+    for graph in (0, 1):
+        M = pa.PoissonHIP(nx, ny, nz)
+        M.assignBoundary("DDNDDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        if precision:
+            M.set_option(pa.OPT_PRECISION, 1)
+        M.set_option(pa.OPT_GRAPH, graph)
+        M.set_stream(stream.cuda_stream)
+        handles.append(M)
+    E, G = handles
+    try:
+        with torch.cuda.stream(stream):
+            f = torch.empty((nz, ny, nx), dtype=torch.float64, device=dev)
+            ue = torch.empty_like(f)
+            ug = torch.empty_like(f)
+            for rep, seed in enumerate((1, 2, 1, 3, 2)):
+                gen = torch.Generator(device=dev).manual_seed(seed)
+                f.uniform_(-1, 1, generator=gen)
+                E.solve_device_oop(f.data_ptr(), ue.data_ptr())
+                G.solve_device_oop(f.data_ptr(), ug.data_ptr())
+                stream.synchronize()
+                assert torch.equal(ue, ug), f"solve {rep} (seed {seed}): graph replay != eager"
+    finally:
+        E.close()
+        G.close()

@@ -284,6 +284,18 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
             xin[c] = (double)in[ib + (long)n * is];
         }
     }
+    // x inverse: the solution being accumulated into and the plane term are read here, with
+    // the input, so their latency overlaps the transform instead of sitting before each store
+    constexpr bool XINV = (AX == 0) && !FWD;
+    double accv[XINV ? 2 * E : 1];
+    double padd = 0.0;
+    if constexpr (XINV) {
+        if (a.accumulate) {
+#pragma unroll
+            for (int c = 0; c < 2 * E; c++) accv[c] = (double)out[base + (long)(t + T * c) * stride];
+        }
+        if (a.plane_add) padd = a.plane_add[lz];
+    }
 
     // ---- twiddle seeds (independent of the data, overlap the load latency) ----
     FFTSeeds<R, M, T> fs;
@@ -575,7 +587,14 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
             const int pos = store_pos<N, KIND, FWD>(n, neg);
             double val = (double)lr(L, pos);
             if (neg) val = -val;
-            put(n, val);
+            if constexpr (XINV) {
+                val = a.out_scale * val;
+                if (a.plane_add) val += padd;
+                if (a.accumulate) val += accv[c];
+                out[base + (long)n * stride] = (TOut)val;
+            } else {
+                put(n, val);
+            }
         }
     }
 }

@@ -529,7 +529,13 @@ int autotune(pp_solver *h)
     }
     const size_t n = (size_t)h->g.nx * h->g.ny * h->g.nz;
     double *buf = nullptr;
-    if (hipMalloc((void **)&buf, n * sizeof(double)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    if (hipMalloc((void **)&buf, n * sizeof(double)) != hipSuccess) {
+        // no room for a scratch field: keep the default configuration (tuning is an
+        // optimisation, never a reason for the solve to fail)
+        (void)hipGetLastError();
+        h->tuned = true;
+        return PP_SUCCESS;
+    }
     int rc = launch_fill_uniform(buf, (long)n, 0x7475ull, 0, h->stream) == hipSuccess ? PP_SUCCESS : PP_HIP_ERROR;
     const int saved_force = h->force_var;
     h->force_var = -1;

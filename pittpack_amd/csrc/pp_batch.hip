@@ -526,9 +526,9 @@ int BatchCore::tune_lines(hipStream_t s)
         for (HostClass &c : *g.cls) {
             if (!line_has_variant(c.key)) continue;
             float best[2] = {1e30f, 1e30f};
-            for (int v = 0; v < 2 && !rc; v++) {
-                c.var[g.fwd ? 0 : 1] = v;
-                for (int r = 0; r < 3 && !rc; r++) {
+            for (int r = 0; r < 3 && !rc; r++) {  // shapes in alternation, best of three each
+                for (int v = 0; v < 2 && !rc; v++) {
+                    c.var[g.fwd ? 0 : 1] = v;
                     if (hipEventRecord(e0, s) != hipSuccess) rc = PP_HIP_ERROR;
                     if (!rc) rc = launch_class(s, c, FORM_F64, g.axis, g.fwd, g.in, g.out, g.bflags, nullptr, nullptr, 0, g.ztiled);
                     if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = PP_HIP_ERROR;
@@ -537,7 +537,7 @@ int BatchCore::tune_lines(hipStream_t s)
                     if (!rc) best[v] = std::min(best[v], ms);
                 }
             }
-            c.var[g.fwd ? 0 : 1] = best[1] < best[0] ? 1 : 0;
+            c.var[g.fwd ? 0 : 1] = best[1] < 0.98f * best[0] ? 1 : 0;  // alternative only when clearly faster
         }
     }
     force_var = saved_force;

@@ -23,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pp_fft.h"
 
 namespace pp {
@@ -167,8 +169,12 @@ __device__ __forceinline__ int xcd_logical(int lin, int total)
 
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 // power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
-// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers
-__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_line(LineArgs a0)
+// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers, and so does
+// the FP32 -> FP64 x inverse of 1024-point lines (its prefetched solution would spill)
+__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
+                                            !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512))
+                                               ? 4
+                                               : 1) k_line(LineArgs a0)
 {
     LineArgs a = a0;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -286,7 +292,9 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256) ? 4 : 1) k_
     }
     // x inverse: the solution being accumulated into and the plane term are read here, with
     // the input, so their latency overlaps the transform instead of sitting before each store
-    constexpr bool XINV = (AX == 0) && !FWD;
+    // (FP32 -> FP64 form only: the FP64 form never accumulates, and holding 2E more doubles
+    // there would exceed its register bound)
+    constexpr bool XINV = (AX == 0) && !FWD && !std::is_same<TIn, TOut>::value;
     double accv[XINV ? 2 * E : 1];
     double padd = 0.0;
     if constexpr (XINV) {

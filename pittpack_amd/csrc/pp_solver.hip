@@ -558,16 +558,30 @@ int autotune(pp_solver *h)
         h->zsolver = bz;
         h->ztiles = bt;
     }
-    double st0[5], st1[5];
-    if (!rc) rc = tune_time(h, buf, 3, st0, tot);
-    if (!rc) {
-        for (int i = 0; i < 4; i++) h->line_var[i] = 1;
-        rc = tune_time(h, buf, 3, st1, tot);
-        // stages in launch order: x fwd, y fwd, z, y inv, x inv
-        const int si[4] = {0, 1, 3, 4};
-        const int len[4] = {h->g.nx, h->g.ny, h->g.ny, h->g.nx};
-        for (int i = 0; i < 4; i++) h->line_var[i] = (line_has_variant(len[i]) && st1[si[i]] < st0[si[i]]) ? 1 : 0;
+    // line shapes: every combination of the stages' two shapes (at most 16), timed round-robin
+    // over three rounds (best whole-solve time each, so drift spreads over all of them and
+    // the effect of a stage's output order on the next stage counts); the default
+    // combination is kept unless another is more than 0.5% faster
+    const int len[4] = {h->g.nx, h->g.ny, h->g.ny, h->g.nx};
+    int stages[4], nv = 0;
+    for (int i = 0; i < 4; i++)
+        if (line_has_variant(len[i])) stages[nv++] = i;
+    const int ncomb = 1 << nv;
+    std::vector<double> tc((size_t)ncomb, 1e30);
+    for (int round = 0; round < 3 && !rc; round++) {
+        for (int comb = 0; comb < ncomb && !rc; comb++) {
+            for (int i = 0; i < 4; i++) h->line_var[i] = 0;
+            for (int k = 0; k < nv; k++) h->line_var[stages[k]] = (comb >> k) & 1;
+            rc = tune_time(h, buf, 1, st, tot);
+            tc[comb] = std::min(tc[comb], tot);
+        }
     }
+    int bestc = 0;
+    for (int comb = 1; comb < ncomb; comb++)
+        if (tc[comb] < tc[bestc]) bestc = comb;
+    if (tc[bestc] > 0.995 * tc[0]) bestc = 0;
+    for (int i = 0; i < 4; i++) h->line_var[i] = 0;
+    for (int k = 0; k < nv; k++) h->line_var[stages[k]] = (bestc >> k) & 1;
     h->force_var = saved_force;
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(buf);

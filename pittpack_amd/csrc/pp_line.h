@@ -755,15 +755,20 @@ inline int line_block_lines(int N, int AX)
     }
 }
 
+template <int M>
+constexpr int alt_lines_x()
+{
+    if constexpr (line_threads_alt(M) > 0) return line_lines<double, M, 0, line_threads_alt(M)>();
+    else return 0;
+}
+
 // lines (x) or columns (y) per block of the alternative FP64 shape (y keeps the default's)
 inline int line_block_lines_alt(int N, int AX)
 {
     if (AX == 1) return line_block_lines<double>(N, 1);
     switch (N / 2) {
-#define PPM(m)                                                                                  \
-    case m:                                                                                     \
-        if constexpr (line_threads_alt(m) > 0) return line_lines<double, m, 0, line_threads_alt(m)>(); \
-        else return 0;
+#define PPM(m) \
+    case m: return alt_lines_x<m>();
         PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256)
         PPM(384) PPM(512)
 #undef PPM

@@ -1,6 +1,6 @@
 """GPU parity of the FP64 solve path against the CPU oracle (through the C ABI).
 
-Bar: max-norm difference <= 1e-12 relative to max|u_oracle| (FP64, BASELINE target), on
+Bar: max-norm difference <= 1e-12 relative to max|u_oracle| (the FP64 parity bar), on
 seeded U[-1,1] right-hand sides, for every boundary combination the oracle restates.
 Combinations the reference's algorithm cannot represent (periodic x with non-periodic y)
 are checked through the discrete residual of the 7-point operator instead.

@@ -32,7 +32,8 @@ enum {
     PP_UNSUPPORTED_SIZE = 26,   /* no compiled transform / z-solve for this size */
     PP_INVALID_HANDLE = 27,
     PP_CALL_ORDER = 28,         /* e.g. pittPack before assignBoundary / setCoords / RHS */
-    PP_NO_DEVICE = 29           /* no HIP device visible */
+    PP_NO_DEVICE = 29,          /* no HIP device visible */
+    PP_IO_FAIL = 30             /* a solution file could not be written */
 };
 
 /* option keys for pp_set_option */
@@ -121,6 +122,15 @@ int pp_get_solution(pp_handle h, double *u);
 
 /* No reference counterpart: copy the solution to a device array. */
 int pp_get_solution_device(pp_handle h, double *d_u);
+
+/* PencilDcmp::IO(int app, int dir, int aligndir) (src/pencilDcmp.cpp:1827-1832, writing
+ * through Phdf5::writeMultiBlockCellCenter, src/phdf5.cpp:1299-1800): write the handle's field
+ * as <folder>/Pxdmf3d<app>.bin (raw little-endian FP64, nz*ny*nx, x fastest) and an XDMF 2.0
+ * index <folder>/Pxdmf3d<app>.xmf (3DCoRectMesh over the (nx+1)(ny+1)(nz+1) cell corners,
+ * attribute "Q", Center="Cell").  The folder must exist (the reference writes into soln/).
+ * One device holds the whole grid, so the reference's chunk arguments dir / aligndir have no
+ * counterpart.  Synchronises the handle's stream.  PP_IO_FAIL when a file cannot be written. */
+int pp_io(pp_handle h, const char *folder, int app);
 
 /* Flow-solver form (no reference counterpart): solve in place on a caller-owned device array
  * of nx*ny*nz doubles holding the RHS on entry and the solution on exit, on the handle's

@@ -27,6 +27,7 @@ UNSUPPORTED_SIZE = 26
 INVALID_HANDLE = 27
 CALL_ORDER = 28
 NO_DEVICE = 29
+IO_FAIL = 30
 
 OPT_ZERO_MODE = 1
 OPT_TIMING = 2
@@ -45,7 +46,7 @@ OPT_LINE_VARIANT = 12
 EXPORTED = (
     "pp_create", "pp_destroy", "pp_assign_boundary", "pp_set_box", "pp_set_coords",
     "pp_assign_rhs", "pp_assign_rhs_device", "pp_pittpack", "pp_get_solution",
-    "pp_get_solution_device", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
+    "pp_get_solution_device", "pp_io", "pp_solve_device", "pp_solve_device_oop", "pp_set_stream", "pp_set_option",
     "pp_get_stage_times", "pp_get_device_field", "pp_get_error_enum", "pp_size_supported",
     "pp_selftest", "pp_set_option_double", "pp_get_kernel_times", "pp_get_tuning",
     "pp_batch_create", "pp_batch_destroy", "pp_batch_add", "pp_batch_finalize", "pp_batch_count",
@@ -94,6 +95,7 @@ def load_library(path: str | None = None):
     L.pp_pittpack.argtypes = [vp]
     L.pp_get_solution.argtypes = [vp, dp]
     L.pp_get_solution_device.argtypes = [vp, vp]
+    L.pp_io.argtypes = [vp, ctypes.c_char_p, i]
     L.pp_solve_device.argtypes = [vp, vp]
     L.pp_solve_device_oop.argtypes = [vp, vp, vp]
     L.pp_set_stream.argtypes = [vp, vp]
@@ -210,6 +212,11 @@ class PoissonHIP:
         u = np.empty((self.nz, self.ny, self.nx), dtype=np.float64)
         _check(self._L.pp_get_solution(self._h, _dp(u)), "getSolution")
         return u
+
+    def IO(self, app: int = 0, folder: str = "soln") -> None:
+        """PencilDcmp::IO (src/pencilDcmp.cpp:1827-1832): write <folder>/Pxdmf3d<app>.xmf and
+        the field it indexes, <folder>/Pxdmf3d<app>.bin (little-endian float64, z slowest)."""
+        _check(self._L.pp_io(self._h, os.fsencode(folder), int(app)), "IO")
 
     # -- device / flow-solver forms ------------------------------------------------------
     def set_option(self, key: int, value: int) -> None:

@@ -69,7 +69,7 @@ def cpu_baseline(n: int = 512, bc: str = BC):
 XY_COMBOS = ["PPPP", "NNNN", "DDDD", "NNDD"]  # x/y faces of the batched and serving workloads
 Z_COMBOS = ["DD", "NN"]
 XY_SIDES = [32, 48, 64, 96, 128, 192]
-SERVE_SIDES = [32, 48, 64, 96, 128, 192, 256, 384, 512]  # compiled line lengths in [32, 512]
+SERVE_SIDES = [32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]  # compiled line lengths in [32, 512]
 
 
 def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, autotune: bool = True):

@@ -142,7 +142,7 @@ hipError_t launch_zperiodic_f64(const ZPerArgs &a, hipStream_t s)
     switch (a.nz) {
 #define PPM(m) \
     case m: return launch_zper_m<m>(a, s);
-        PPM(8) PPM(12) PPM(16) PPM(24) PPM(32) PPM(48) PPM(64) PPM(96) PPM(128) PPM(192) PPM(256) PPM(384) PPM(512)
+        PP_FFT_LENGTHS(PPM)
 #undef PPM
     default: return hipErrorInvalidValue;
     }

@@ -61,6 +61,40 @@ def test_batch_matches_single_and_oracle():
             assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
 
 
+# radix-5 and short lines (N = 8 .. 640) next to radix-2/3 ones in one batch
+LENGTH_MEMBERS = [
+    (16, 12, 10, "NNNNDD", [0, 1, 0, 1, 0, 1]),
+    (24, 20, 18, "DDNDDD", [0, 1, 0, 1, 0, 1]),
+    (40, 80, 12, "PPPPNN", [0, 2, 0, 1, 0, 1]),
+    (640, 8, 4, "DDDDDD", [0, 1, 0, 1, 0, 1]),
+    (160, 320, 4, "NDDNDN", [0, 1, 0, 1, 0, 1]),
+    (20, 40, 33, "DDNNNN", [0, 1, 0, 1, 0, 1]),
+    (32, 48, 17, "PPPPNN", [0, 1, 0, 1, 0, 1]),
+]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_line_lengths(precision):
+    print(f"batch line lengths precision={precision}", flush=True)
+    with pa.Batch(precision=precision) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in LENGTH_MEMBERS]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, LENGTH_MEMBERS):
+            f = prng.uniform_pm1((nz, ny, nx), seed=300 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, LENGTH_MEMBERS):
+            got = B.get_solution(i)
+            if precision == 0:
+                single = pa.solve(rhs[i], bc, box)
+                assert np.array_equal(got, single), f"member {i}: batched != single"
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+            err = rel_maxdiff(got, ref)
+            assert err <= TOL, f"member {i} {bc} ({nx},{ny},{nz}): rel max diff {err:.3e}"
+
+
 def test_batch_repeated_and_device_fill():
     """device generator == Python generator; the right-hand sides are kept, so a second solve
     reproduces the first"""

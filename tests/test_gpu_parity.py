@@ -46,6 +46,26 @@ def test_parity_fp64(bc, shape, zsolver):
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
+# line lengths beyond the power-of-two/radix-3 set: the reference's small fixture sizes
+# 16 x 12 x 10 and 24 x 20 x 18 (N = 12, 20), the shortest lines (N = 8, 12) and radix-5
+# lengths up to N = 640 (shape (nz, ny, nx))
+LENGTH_SHAPES = [(10, 12, 16), (18, 20, 24), (12, 8, 40), (9, 80, 20), (5, 160, 12), (4, 40, 640), (4, 320, 8)]
+
+
+@pytest.mark.parametrize("shape", LENGTH_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", COMBOS)
+def test_parity_line_lengths(bc, shape):
+    print(f"line lengths {bc} {shape}", flush=True)
+    rng = np.random.default_rng(sum(map(ord, bc)) * 7 + shape[1] * 13 + shape[2])
+    f = rng.uniform(-1, 1, shape)
+    box = [0, 1.5, 0, 1, 0, 2]
+    for zero_mode in (0, 1):
+        ref = oracle.solve(f, bc, box, zero_mode=zero_mode)
+        got = pa.solve(f, bc, box, zero_mode=zero_mode)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
+
+
 @pytest.mark.parametrize("zsolver", [0, 1, 2, 3, 4], ids=["pcr", "scan", "mg", "pcrfull", "cr"])
 @pytest.mark.parametrize("nz", [4, 5, 15, 16, 17, 31, 33, 48, 49, 64, 100, 128, 255, 256, 500, 511, 512])
 def test_parity_z_lengths(nz, zsolver):
@@ -139,7 +159,7 @@ def test_call_order_and_errors():
 ZPER_COMBOS = ["PPPPPP", "NNNNPP", "DDDDPP", "DDNNPP", "NDDNPP", "NNPPPP"]
 
 
-@pytest.mark.parametrize("nz", [8, 16, 24, 48, 64, 96, 128, 512])
+@pytest.mark.parametrize("nz", [8, 10, 16, 20, 24, 40, 48, 64, 96, 128, 320, 512])
 @pytest.mark.parametrize("bc", ZPER_COMBOS)
 def test_parity_periodic_z(bc, nz):
     print(f"periodic z {bc} nz={nz}", flush=True)
@@ -164,7 +184,7 @@ def test_periodic_z_residual_and_limits():
     r = fe - oracle.apply_operator(u, "PPDDPP")
     assert float(np.abs(r).max()) <= 1e-12 * max(np.abs(fe).max(), 12.0 * np.abs(u).max())
     with pytest.raises(pa.PittPackError) as e:
-        pa.solve(rng.uniform(-1, 1, (20, 16, 32)), "NNNNPP")
+        pa.solve(rng.uniform(-1, 1, (14, 16, 32)), "NNNNPP")  # 14 = 2 x 7: no compiled FFT
     assert e.value.code == pa.UNSUPPORTED_SIZE
     with pytest.raises(pa.PittPackError) as e:
         pa.solve(f, "NNNNPP", precision=1)
@@ -172,7 +192,7 @@ def test_periodic_z_residual_and_limits():
 
 
 # line lengths with an alternative launch shape (radix plan / block) from 16 to 512
-VARIANT_SHAPES = [(10, 16, 32), (20, 24, 48), (6, 128, 256), (4, 64, 512), (5, 256, 128)]
+VARIANT_SHAPES = [(10, 16, 32), (20, 24, 48), (6, 128, 256), (4, 64, 512), (5, 256, 128), (7, 80, 80)]
 
 
 @pytest.mark.parametrize("shape", VARIANT_SHAPES, ids=lambda s: "x".join(map(str, s)))

@@ -191,9 +191,10 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int):
     }
 
 
-def bench_large_mixed(pa, torch, n: int, refine: int, steps: int):
-    """mixed form at n^3, periodic x/y, Neumann z, FP32 Gaussian RHS (torch generator, seed 1)
-    with the volume mean removed; FP64 discrete residual of the result"""
+def bench_large(pa, torch, n: int, precision: int, refine: int, steps: int):
+    """FP64 form (autotuned) or mixed form at n^3, periodic x/y, Neumann z, FP32 Gaussian RHS
+    (torch generator, seed 1) with the volume mean removed; FP64 discrete residual of the
+    result"""
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev).manual_seed(1)
     f = torch.randn((n, n, n), generator=g, device=dev, dtype=torch.float32).double()
@@ -203,8 +204,11 @@ def bench_large_mixed(pa, torch, n: int, refine: int, steps: int):
     M.assignBoundary("PPPPNN")
     M.setBox([0, 1, 0, 1, 0, 1])
     M.setCoords(2)
-    M.set_option(pa.OPT_PRECISION, 1)
-    M.set_option(pa.OPT_REFINE, refine)
+    if precision == 1:
+        M.set_option(pa.OPT_PRECISION, 1)
+        M.set_option(pa.OPT_REFINE, refine)
+    else:
+        M.set_option(pa.OPT_AUTOTUNE, 1)
     M.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     M.solve_device_oop(f.data_ptr(), u.data_ptr())
     torch.cuda.synchronize()
@@ -225,8 +229,8 @@ def bench_large_mixed(pa, torch, n: int, refine: int, steps: int):
     res = float(r.abs().max()) / scale
     del f, u, r, up, um
     torch.cuda.empty_cache()
-    return {"workload": f"mixed form, {n}^3, bc PPPPNN, FP32 N(0,1) RHS (torch generator seed 1), "
-                        f"mean removed, {refine} refinement steps",
+    form = f"mixed form, {refine} refinement steps" if precision == 1 else "FP64 form, autotuned"
+    return {"workload": f"{form}, {n}^3, bc PPPPNN, FP32 N(0,1) RHS (torch generator seed 1), mean removed",
             "ms_per_step": ms, "value": n ** 3 / (ms * 1e-3), "fp64_residual_rel": res}
 
 
@@ -258,7 +262,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
-    ap.add_argument("--large", type=int, default=1024, help="side of the large mixed-form run (0: skip)")
+    ap.add_argument("--large", type=int, default=1024, help="side of the large FP64 and mixed runs (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -405,7 +409,9 @@ def main():
     del f, u
     torch.cuda.empty_cache()
     if args.large > 0:
-        out["large_mixed"] = bench_large_mixed(pa, torch, args.large, args.refine, max(1, min(args.steps, 5)))
+        ls = max(1, min(args.steps, 5))
+        out["large_fp64"] = bench_large(pa, torch, args.large, 0, args.refine, ls)
+        out["large_mixed"] = bench_large(pa, torch, args.large, 1, args.refine, ls)
     if args.batch_count > 0:
         out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine,
                                        autotune=not args.no_autotune)

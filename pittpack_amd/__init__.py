@@ -469,8 +469,9 @@ class Server:
 
 def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None,
           precision: int = 0, refine: int | None = None, tau: float | None = None,
-          line_variant: int | None = None, autotune: int | None = None) -> np.ndarray:
-    """One-shot solve of an (nz, ny, nx) FP64 RHS (precision 0: FP64; 1: mixed + refinement)."""
+          line_variant: int | None = None, autotune: int | None = None, zlayout: int | None = None) -> np.ndarray:
+    """One-shot solve of an (nz, ny, nx) FP64 RHS (precision 0: FP64; 1: mixed + refinement).
+    zlayout: 1 (default) hands the z solve the y transform's z tiles where it can, 0 in place."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     with PoissonHIP(nx, ny, nz) as M:
@@ -482,6 +483,8 @@ def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | N
             M.set_option(OPT_AUTOTUNE, autotune)
         if zsolver is not None:
             M.set_option(OPT_ZSOLVER, zsolver)
+        if zlayout is not None:
+            M.set_option(OPT_ZLAYOUT, zlayout)
         if precision:
             M.set_option(OPT_PRECISION, precision)
         if refine is not None:

@@ -15,6 +15,10 @@ hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArg
 hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s);
 hipError_t launch_zsolve_scan_f64(int nz, const ZArgs &a, hipStream_t s);
 bool zsolve_supported_f64(int nz);
+// FP64 columns of 513..1024 rows (pp_zsolve_long.hip): in place, not tiled, every column
+// written back; launch_zsolve_pcr_f64 forwards these lengths here
+bool zsolve_long_supported_f64(int nz);
+hipError_t launch_zsolve_long_f64(int nz, const ZArgs &a, hipStream_t s);
 // multigrid alternative (pp_zmg.hip): V-cycles until max|residual| <= tol * max|rhs|
 hipError_t launch_zsolve_mg_f64(const ZArgs &a, int max_cycles, double tol, hipStream_t s);
 // the reference's alternatives (pp_zalt.hip): full parallel cyclic reduction and cyclic

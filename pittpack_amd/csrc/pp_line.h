@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "pp_fft.h"
+#include "pp_xcd.h"
 
 namespace pp {
 
@@ -80,10 +81,11 @@ struct LineArgs {
     const LineSolve *solves;
     int bflags;
     // y only: z-tiled layout of the output (ZT_OUT, forward) or input (ZT_IN, inverse):
-    // element (x, y, z) at ((x/W * ny + y) * tnz + z) * W + x%W with W = columns per block and
-    // tnz >= nz rows per tile, so each (x/W, y) tile is contiguous for the z solve
+    // element (x, y, z) at ((x/W * ny + y) * tnz + z) * W + x%W with W = tile_w (else columns
+    // per block) and tnz >= nz rows per tile, so each (x/W, y) tile is contiguous for the z solve
     int ztiled;
     int tnz;
+    int tile_w;  // tile width W (0: LINES, the block's column count); a multiple of LINES
     // host side, FP64 form: 1 = the alternative launch shape (line_threads_alt), 0 = default
     int variant;
 };
@@ -154,17 +156,6 @@ template <typename R>
 __device__ __forceinline__ R &lr(cpx<R> *L, int p)
 {
     return reinterpret_cast<R *>(&L[padc(p >> 1)])[p & 1];
-}
-
-// Blocks are dealt to the 8 XCDs round-robin by linear id, and each XCD has its own L2.
-// Renumbering so that each XCD's blocks take consecutive logical indices puts neighbouring
-// tiles -- which share cache lines when a tile row is shorter than 128 bytes -- on one XCD.
-__device__ __forceinline__ int xcd_logical(int lin, int total)
-{
-    constexpr int X = 8;
-    const int q = total / X, r = total % X;
-    const int xcd = lin % X, k = lin / X;
-    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
 }
 
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
@@ -251,9 +242,18 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
         base = (long)by * N * a.ncols + col;
         stride = a.ncols;
         if (a.ztiled) {
-            // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
-            tbase = ((long)bx * N * a.tnz + by) * LINES + l;
-            tstride = (long)a.tnz * LINES;
+            if (a.tile_w == 0 || a.tile_w == LINES) {
+                // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
+                tbase = ((long)bx * N * a.tnz + by) * LINES + l;
+                tstride = (long)a.tnz * LINES;
+            } else {
+                // tiles W = tile_w columns wide (a multiple of LINES): this block fills LINES of
+                // tile bx / (W / LINES); neighbouring blocks, which complete each row, are
+                // numbered onto one XCD (W * 8 bytes is one 128-byte line)
+                const int W = a.tile_w, sub = W / LINES;
+                tbase = ((long)(bx / sub) * N * a.tnz + by) * W + (bx % sub) * LINES + l;
+                tstride = (long)a.tnz * W;
+            }
         }
     }
 

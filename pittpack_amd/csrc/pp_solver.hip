@@ -294,6 +294,18 @@ int build_tables(pp_solver *h)
     return PP_SUCCESS;
 }
 
+// z lengths each FP64 z solver takes (non-periodic z)
+bool fp64_z_supported(int zsolver, int nz)
+{
+    switch (zsolver) {
+    case 1: return zsolve_supported_f64(nz);  // partitioned Thomas: up to 32 rows per lane
+    case 2: return nz >= 4 && 6 * (size_t)nz * sizeof(double) <= 64 * 1024;  // multigrid: 6 LDS rows
+    case 3:
+    case 4: return zsolve_alt_supported(nz);
+    default: return zsolve_supported_f64(nz) || zsolve_long_supported_f64(nz);
+    }
+}
+
 int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
@@ -303,7 +315,7 @@ int ensure_ready(pp_solver *h)
         if (!zperiodic_supported(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     }
     if (h->precision == 1 && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
-    if (h->precision == 0 && !h->g.zper && !zsolve_supported_f64(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 0 && !h->g.zper && !fp64_z_supported(h->zsolver, h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (!h->tables_ready) {
         h->mixed_ready = false;
         return build_tables(h);
@@ -341,14 +353,17 @@ LineArgs line_args(pp_solver *h, int axis, double *field)
 
 // the five stages, FP64: rhs -> field (x forward), then in place on field
 // z-tiled layout between the y transforms and the z solve: the y forward writes each
-// (x/16, y) tile of nz rows x 16 columns contiguously, so the z solve reads and writes whole
-// 64-KB-class tiles instead of one 128-byte row per z plane; used when the y transform's
-// block width equals the z solve's (16 FP64 columns) and z is not periodic
+// (x/16, y) tile of nz rows x 16 columns (one 128-byte line per row) contiguously, so the z
+// solve reads and writes whole tiles instead of one row per z plane.  y blocks of 16 columns
+// fill a tile, y blocks of 8 (1280- to 2048-byte y lines' LDS slices) half of one; z blocks
+// take a whole tile (up to 512 rows) or half (longer columns).  z not periodic.
 bool use_ztiles(pp_solver *h)
 {
     // (the partitioned solver rounds its rows per lane up to a coarser set, so it keeps the
     // in-place layout)
-    return h->ztiles && h->zsolver == 0 && !h->g.zper && line_block_lines<double>(h->g.ny, 1) == zsolve_cpb<double>();
+    if (!h->ztiles || h->zsolver != 0 || h->g.zper) return false;
+    if (!zsolve_supported_f64(h->g.nz) && !zsolve_long_supported_f64(h->g.nz)) return false;
+    return zsolve_cpb<double>() % line_block_lines<double>(h->g.ny, 1) == 0;
 }
 
 // rows per z tile: whole lane rows of the z solve (16 J), zero padding below nz
@@ -357,7 +372,7 @@ int ztile_rows(int nz) { return (nz + 15) / 16 * 16; }
 int ensure_ztiles(pp_solver *h)
 {
     if (h->d_zt) return PP_SUCCESS;
-    const size_t W = zsolve_cpb<double>();
+    const size_t W = zsolve_cpb<double>();  // the tile width
     const size_t elems = ((size_t)h->g.nx + W - 1) / W * W * h->g.ny * ztile_rows(h->g.nz);
     if (hipMalloc((void **)&h->d_zt, elems * sizeof(double)) != hipSuccess) {
         h->d_zt = nullptr;
@@ -395,6 +410,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         a.ztiled = zt ? ZT_OUT : 0;
         a.variant = stage_var(h, 1, h->g.ny);
         a.tnz = ztile_rows(h->g.nz);
+        a.tile_w = zt ? zsolve_cpb<double>() : 0;
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
@@ -403,6 +419,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         ZArgs z{};
         z.data = zdata;
         z.tiled = zt ? 1 : 0;
+        z.tile_w = zt ? zsolve_cpb<double>() : 0;
         z.SX = h->g.nx;
         z.SY = h->g.ny;
         z.nz = h->g.nz;
@@ -446,6 +463,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         a.ztiled = zt ? ZT_IN : 0;
         a.variant = stage_var(h, 2, h->g.ny);
         a.tnz = ztile_rows(h->g.nz);
+        a.tile_w = zt ? zsolve_cpb<double>() : 0;
         PP_HIP_CHECK(launch_line(FORM_F64, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
@@ -549,7 +567,7 @@ int autotune(pp_solver *h)
         for (const auto &c : cand) {
             h->zsolver = c[0];
             h->ztiles = c[1];
-            if (c[1] && !use_ztiles(h)) continue;
+            if ((c[1] && !use_ztiles(h)) || !fp64_z_supported(c[0], h->g.nz)) continue;
             if ((rc = tune_time(h, buf, 3, st, tot))) break;
             if (tot < best) {
                 best = tot;

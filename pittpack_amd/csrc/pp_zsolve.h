@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "pp_xcd.h"
 
 namespace pp {
 
@@ -37,10 +38,12 @@ struct ZArgs {
     const ZItem *items;
     const ZSolve *solves;
     int nitems;
-    // z-tiled layout (written by the y forward transform): tile (x/CPB, y) holds 16 J rows
-    // (rows from nz on are zero padding) of CPB columns contiguously, element (x, y, z) at
-    // ((x/CPB * SY + y) * 16 J + z) * CPB + x%CPB
+    // z-tiled layout (written by the y forward transform): tile (x/W, y) holds 16 J rows
+    // (rows from nz on are zero padding) of W columns contiguously, element (x, y, z) at
+    // ((x/W * SY + y) * 16 J + z) * W + x%W; W = tile_w (0: the block's CPB), a multiple of
+    // CPB -- a block then takes CPB columns of a tile
     int tiled;
+    int tile_w;
 };
 
 // one solve of a batched z launch: its field sits at `offset` in the launch's data arena
@@ -225,11 +228,20 @@ __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow 
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
     if (a.tiled) {
-        // z tiles: 16 J rows (zero padding below nz) x CPB columns, contiguous; row r of the
-        // tile is element r * CPB + c, so thread tid = g * CPB + c reads tid + q * 16 * CPB
-        const S *__restrict__ src = reinterpret_cast<const S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+        if (a.tile_w == 0 || a.tile_w == CPB) {
+            // z tiles: 16 J rows (zero padding below nz) x CPB columns, contiguous; row r of
+            // the tile is element r * CPB + c, so thread tid = g * CPB + c reads tid + q * 16 * CPB
+            const S *__restrict__ src = reinterpret_cast<const S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
 #pragma unroll
-        for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[q * 16 * CPB + tid];
+            for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[q * 16 * CPB + tid];
+        } else {
+            // wider tiles: this block's CPB columns of tile bx / (W / CPB), rows W apart
+            const int W = a.tile_w, sub = W / CPB;
+            const S *__restrict__ src = reinterpret_cast<const S *>(a.data) +
+                                        ((long)(bx / sub) * a.SY + by) * (16 * J) * W + (bx % sub) * CPB + g * W + c;
+#pragma unroll
+            for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[(long)q * 16 * W];
+        }
         return;
     }
     const int col0 = bx * CPB;
@@ -252,9 +264,17 @@ __device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n
     const int tid = threadIdx.x;
     const int c = tid % CPB, g = tid / CPB;
     if (a.tiled) {
-        S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+        if (a.tile_w == 0 || a.tile_w == CPB) {
+            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
 #pragma unroll
-        for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];
+            for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];
+        } else {
+            const int W = a.tile_w, sub = W / CPB;
+            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)(bx / sub) * a.SY + by) * (16 * J) * W +
+                                  (bx % sub) * CPB + g * W + c;
+#pragma unroll
+            for (int q = 0; q < J; q++) dst[(long)q * 16 * W] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];
+        }
         return;
     }
     const int col0 = bx * CPB;
@@ -292,8 +312,20 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
 {
     constexpr bool ALL = (J <= 3);
     constexpr int NEX = ALL ? J : 3;
+    // FP64 columns beyond 32 rows per lane (nz > 512): the Thomas factors are kept in this
+    // lane's own tile slots, free once its rows are in registers, instead of J more register
+    // pairs.  The launch requires every column to be written back (tau = 0), since the slots
+    // no longer hold the input afterwards.
+    constexpr bool CP_LDS = sizeof(R) * J > 256;
     ZArgs a = a0;
     int bx = blockIdx.x, by = blockIdx.y;
+    if (CP_LDS && !a0.items) {
+        // such blocks take 8 columns (64-byte rows): the two blocks sharing a 128-byte line
+        // run on one XCD, so the line is fetched into one L2
+        const int lin = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+        bx = lin % gridDim.x;
+        by = lin / gridDim.x;
+    }
     if (a0.items) {
         const ZItem it = a0.items[blockIdx.x];
         const ZSolve sv = a0.solves[it.solve];
@@ -371,22 +403,40 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
             return Row3<R>{ta, tb, ta};
         }
     };
-    R cp[J];
+    R cp[CP_LDS ? 1 : J];
+    R *const cps = tile + t * (CPB + 1) + c;  // this lane's slot of row j: cps[j * 16 * (CPB + 1)]
+    R cprev;
     {
         Row3<R> r0 = coef(0);
         const R inv = rcp_nr(r0.b);
-        cp[0] = r0.c * inv;
+        cprev = r0.c * inv;
+        if constexpr (CP_LDS) cps[0] = cprev;
+        else cp[0] = cprev;
         d[0] = d[0] * inv;
     }
 #pragma unroll
     for (int j = 1; j < J; j++) {
         Row3<R> r = coef(j);
-        const R inv = rcp_nr(r.b - r.a * cp[j - 1]);
-        cp[j] = r.c * inv;
+        const R inv = rcp_nr(r.b - r.a * cprev);
+        cprev = r.c * inv;
+        if constexpr (CP_LDS) cps[j * 16 * (CPB + 1)] = cprev;
+        else cp[j] = cprev;
         d[j] = (d[j] - r.a * d[j - 1]) * inv;
+        // factors to LDS: the row update is pinned here (otherwise it sinks to the backward
+        // sweep and every pivot stays live until then)
+        if constexpr (CP_LDS) asm volatile("" : "+v"(d[j]));
     }
 #pragma unroll
-    for (int j = J - 2; j >= 0; j--) d[j] = d[j] - cp[j] * d[j + 1];
+    for (int j = J - 2; j >= 0; j--) {
+        R cj;
+        if constexpr (CP_LDS) cj = cps[j * 16 * (CPB + 1)];
+        else cj = cp[j];
+        d[j] = d[j] - cj * d[j + 1];
+        // factors from LDS: loaded a few rows ahead at most, not all J beside the J rows
+        if constexpr (CP_LDS) {
+            if (j % 4 == 0) asm volatile("" : "+v"(d[j])::"memory");
+        }
+    }
 
     const bool near = !pin && fabs(eig + 2.) < a.tau;
     const bool skip = FIX ? !near : near;

@@ -3,6 +3,7 @@
 // rows: the kernel keeps explicit coefficients only for rows j in {0, J-2, J-1}, which
 // covers the bottom 16 real rows exactly when the padding is shorter than one lane row.
 
+#include "pp_launch.h"
 #include "pp_zsolve.h"
 
 namespace pp {
@@ -22,6 +23,7 @@ static hipError_t launch_pcr(const ZArgs &a, hipStream_t s)
 template <typename R, int J, int CPB>
 static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s)
 {
+    if (a.tiled && a.tile_w != 0 && a.tile_w != CPB) return hipErrorInvalidValue;
     dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
     bool full = (a.nz == 16 * J);
     if (a.items) {
@@ -67,7 +69,7 @@ hipError_t launch_zsolve_pcr_f64(int nz, const ZArgs &a, hipStream_t s)
 {
     switch ((nz + 15) / 16) {
         PPJ32(double)
-    default: return hipErrorInvalidValue;
+    default: return launch_zsolve_long_f64(nz, a, s);
     }
 }
 

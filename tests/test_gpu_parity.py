@@ -46,6 +46,75 @@ def test_parity_fp64(bc, shape, zsolver):
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
+# (zsolver, zlayout): PCR -> Thomas from 16-column z tiles (half a tile per 8-column block)
+# and in place; the alternates in place
+LONG_CASES = [(0, 1), (0, 0), (2, 1), (3, 1), (4, 1)]
+
+
+@pytest.mark.parametrize("case", LONG_CASES, ids=["pcr-tiles", "pcr-inplace", "mg", "pcrfull", "cr"])
+@pytest.mark.parametrize("nz", [513, 530, 640, 777, 1000, 1024])
+def test_parity_long_columns(nz, case):
+    """FP64 columns of 513..1024 rows (33..64 rows per lane for PCR -> Thomas, 8-column blocks
+    with Thomas factors in LDS); 20 columns leave the last block half empty"""
+    zsolver, zlayout = case
+    print(f"long column {nz} zsolver={zsolver} zlayout={zlayout}", flush=True)
+    rng = np.random.default_rng(nz + 5)
+    shape = (nz, 12, 20)
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]
+    f = rng.uniform(-1, 1, shape)
+    for bc in ("NNNNDD", "PPPPNN", "DDNDDD", "DDDDDN"):
+        for zero_mode in (0, 1):
+            ref = oracle.solve(f, bc, box, zero_mode=zero_mode)
+            got = pa.solve(f, bc, box, zero_mode=zero_mode, zsolver=zsolver, zlayout=zlayout)
+            err = rel_maxdiff(got, ref)
+            assert err <= TOL, f"{bc} nz={nz} zero_mode={zero_mode}: rel max diff {err:.3e}"
+
+
+@pytest.mark.parametrize("shape", [(530, 640, 20), (1024, 768, 8), (500, 768, 20), (512, 1024, 16)],
+                         ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("zlayout", [1, 0], ids=["tiles", "inplace"])
+def test_parity_ztiles_from_8col_y(shape, zlayout):
+    """y lines of 640 to 1024 points run 8 columns per block, so two y blocks fill each
+    16-column z tile; z blocks take whole tiles (up to 512 rows) or half (longer columns)"""
+    print(f"z tiles from 8-column y blocks {shape} zlayout={zlayout}", flush=True)
+    rng = np.random.default_rng(shape[0] + shape[1])
+    f = rng.uniform(-1, 1, shape)
+    box = [0.0, shape[2] / 64.0, 0.0, shape[1] / 64.0, 0.0, shape[0] / 64.0]
+    for bc in ("DDNDDD", "PPPPNN"):
+        ref = oracle.solve(f, bc, box, zero_mode=1)
+        got = pa.solve(f, bc, box, zero_mode=1, zlayout=zlayout)
+        err = rel_maxdiff(got, ref)
+        assert err <= TOL, f"{bc} {shape}: rel max diff {err:.3e}"
+
+
+def test_long_columns_limits_and_autotune():
+    """partitioned Thomas stops at 512 rows and the tuner skips it beyond; a wide 1024-row
+    solve through the tuned handle"""
+    print("long column limits", flush=True)
+    rng = np.random.default_rng(3)
+    f = rng.uniform(-1, 1, (600, 16, 16))
+    with pytest.raises(pa.PittPackError) as e:
+        pa.solve(f, "NNNNDD", zsolver=1)
+    assert e.value.code == pa.UNSUPPORTED_SIZE
+    # dx = dy = dz as in the benchmark grids (a unit box would make dz 10x finer than dx and
+    # the low modes' columns ill-conditioned enough to amplify rounding past the bar)
+    box = [0, 96 / 1024, 0, 64 / 1024, 0, 1]
+    f = rng.uniform(-1, 1, (1024, 64, 96))
+    ref = oracle.solve(f, "DDNDDD", box)
+    with pa.PoissonHIP(96, 64, 1024) as M:
+        M.set_option(pa.OPT_AUTOTUNE, 1)
+        M.assignBoundary("DDNDDD")
+        M.setBox(box)
+        M.setCoords(2)
+        for _ in range(2):
+            M.assignRhs(f)
+            M.pittPack()
+            err = rel_maxdiff(M.getSolution(), ref)
+            assert err <= TOL, f"rel max diff {err:.3e}"
+        tuned = M.tuning()
+    assert tuned["tuned"] and tuned["zsolver"] == 0
+
+
 # line lengths beyond the power-of-two/radix-3 set: the reference's small fixture sizes
 # 16 x 12 x 10 and 24 x 20 x 18 (N = 12, 20), the shortest lines (N = 8, 12) and radix-5
 # lengths up to N = 640 (shape (nz, ny, nx))

@@ -53,7 +53,8 @@ enum {
      *               a V-cycle reduces it by less than 10% (its rounding floor);
      *           3 = full parallel cyclic reduction (TriDiag::pcr, src/triDiag.cpp:1442-1542);
      *           4 = cyclic reduction (TriDiag::crp, src/triDiag.cpp:1285-1437).
-     *           2-4: FP64 form (nz <= 512, as the default), non-periodic z */
+     *           FP64 form, non-periodic z; column lengths: 0, 3, 4 up to 1024 rows, 1 up
+     *           to 512, 2 up to 1365 (the mixed form runs its own FP32 solve, up to 1024) */
     PP_OPT_ZSOLVER = 3,
     /* 0 (default): FP64 throughout.  1: mixed form -- FP32 transforms and z solves with
      * FP64 twiddles and eigenvalues, FP64 solves of the near-singular columns

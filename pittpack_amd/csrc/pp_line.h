@@ -662,13 +662,17 @@ template <typename R, int M, int AX, int T = line_threads(M, AX)>
 constexpr int line_lines()
 {
     constexpr int bytes = line_slots<M>() * 2 * (int)sizeof(R);
-    // y: 16 FP64 columns (128-byte rows) or up to 32 FP32 columns per block
+    // y: 16 FP64 columns (128-byte rows) or up to 32 FP32 columns per block.  FP64 y lines
+    // of 1024 points (64 threads each) keep 16 columns in one 1024-thread block (139 KB of
+    // LDS, still 16 waves per CU), so their z tiles get whole 128-byte rows; shorter lines
+    // with large slices halve the columns instead (more blocks per CU)
+    const bool wide = AX == 1 && sizeof(R) == 8 && T >= 64;
     int lines = (AX == 0) ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
     if (lines < 1) lines = 1;
     if (lines > 64) lines = 64;
-    const int budget = (AX == 0) ? 40 * 1024 : 80 * 1024;
+    const int budget = (AX == 0) ? 40 * 1024 : (wide ? 144 * 1024 : 80 * 1024);
     while (lines > 1 && lines * bytes > budget) lines /= 2;
-    while (lines > 1 && lines * T > 512) lines /= 2;
+    while (lines > 1 && lines * T > (wide ? 1024 : 512)) lines /= 2;
     return lines;
 }
 

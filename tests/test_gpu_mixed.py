@@ -61,7 +61,7 @@ def test_mixed_tau(tau):
 
 @pytest.mark.parametrize("nz", [600, 1000, 1024])
 def test_mixed_long_columns(nz):
-    """columns longer than the FP64 form's 512 rows"""
+    """columns of 600 to 1024 rows (FP32 z solve with up to 64 rows per lane)"""
     print(f"mixed nz={nz}", flush=True)
     shape = (nz, 16, 16)
     box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]

@@ -212,11 +212,16 @@ def test_call_order_and_errors():
     with pytest.raises(pa.PittPackError) as e:
         pa.PoissonHIP(32, 32, 1025)
     assert e.value.code == pa.UNSUPPORTED_SIZE
-    # FP64 columns stop at 512 rows; the mixed form takes up to 1024
+    # FP64 columns: up to 1024 rows with the default z solver, 512 with partitioned Thomas
     with pa.PoissonHIP(32, 32, 513) as M:
         M.assignBoundary("NNNNDD")
         M.setBox([0, 1, 0, 1, 0, 1])
         M.setCoords(2)
+        f = np.random.default_rng(2).uniform(-1, 1, (513, 32, 32))
+        M.assignRhs(f)
+        M.pittPack()
+        assert rel_maxdiff(M.getSolution(), oracle.solve(f, "NNNNDD", zero_mode=1)) <= TOL
+        M.set_option(pa.OPT_ZSOLVER, 1)
         M.assignRhs(np.zeros((513, 32, 32)))
         with pytest.raises(pa.PittPackError) as e:
             M.pittPack()

@@ -319,8 +319,8 @@ __global__ void __launch_bounds__(16 * CPB) k_zsolve(ZArgs a0)
     constexpr bool CP_LDS = sizeof(R) * J > 256;
     ZArgs a = a0;
     int bx = blockIdx.x, by = blockIdx.y;
-    if (CP_LDS && !a0.items) {
-        // such blocks take 8 columns (64-byte rows): the two blocks sharing a 128-byte line
+    if (CPB * sizeof(S) < 128 && !a0.items) {
+        // rows shorter than a 128-byte line (8 FP64 columns): the two blocks sharing a line
         // run on one XCD, so the line is fetched into one L2
         const int lin = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
         bx = lin % gridDim.x;

@@ -23,7 +23,7 @@ static hipError_t launch_pcr(const ZArgs &a, hipStream_t s)
 template <typename R, int J, int CPB>
 static hipError_t launch_pcr_c(const ZArgs &a, hipStream_t s)
 {
-    if (a.tiled && a.tile_w != 0 && a.tile_w != CPB) return hipErrorInvalidValue;
+    if (a.tiled && a.tile_w % CPB != 0) return hipErrorInvalidValue;
     dim3 grid((unsigned)((a.SX + CPB - 1) / CPB), (unsigned)a.SY);
     bool full = (a.nz == 16 * J);
     if (a.items) {

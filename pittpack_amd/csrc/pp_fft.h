@@ -4,7 +4,7 @@
 // elements with index t + T*m (m = 0..E-1) in registers.  That layout is both the input
 // layout of the first pass (so the first pass needs no exchange) and the output layout of
 // the last pass (so results stay in registers).  Passes in between exchange through the
-// line's LDS slice.  Radices are picked at compile time from {16, 8, 4, 2, 3, 5} so that
+// line's LDS slice.  Radices are picked at compile time from {16, 8, 4, 2, 3, 5, 7} so that
 // each divides E; every butterfly is unrolled in registers.
 //
 // Replaces the cuFFT/FFTW Z2Z calls of the reference (src/poissonGPU.cpp:5-98,
@@ -59,7 +59,7 @@ constexpr RadixPlan make_plan(int M, int E)
     RadixPlan p{};
     p.n = 0;
     int rem = M;
-    const int cand[6] = {16, 8, 4, 2, 3, 5};
+    const int cand[7] = {16, 8, 4, 2, 3, 5, 7};
     while (rem > 1 && p.n < 24) {
         int pick = 0;
         for (int c : cand) {
@@ -203,6 +203,39 @@ __device__ __forceinline__ void dft5(cpx<R> *u)
     u[3] = c_sub(m2, n2);
 }
 
+// 7-point DFT by symmetric pairs: with t_j = u_j + u_{7-j}, s_j = u_j - u_{7-j} (j = 1..3),
+// X_0 = u_0 + t_1 + t_2 + t_3 and, for k = 1..3, a_k = u_0 + sum_j cos(2 pi jk/7) t_j,
+// b_k = sum_j sin(2 pi jk/7) s_j, X_k = a_k + i DIR b_k, X_{7-k} = a_k - i DIR b_k
+template <int DIR, typename R>
+__device__ __forceinline__ void dft7(cpx<R> *u)
+{
+    const R c1 = (R)0.62348980185873353052500488400424;   // cos(2pi/7)
+    const R c2 = (R)-0.22252093395631440428890256449679;  // cos(4pi/7)
+    const R c3 = (R)-0.90096886790241912623610231950745;  // cos(6pi/7)
+    const R s1 = (R)DIR * (R)0.78183148246802980870844452667406;  // sin(2pi/7) * DIR
+    const R s2 = (R)DIR * (R)0.97492791218182360701813168299393;  // sin(4pi/7) * DIR
+    const R s3 = (R)DIR * (R)0.43388373911755812047576833284836;  // sin(6pi/7) * DIR
+    const cpx<R> a = u[0];
+    const cpx<R> t1 = c_add(u[1], u[6]), t2 = c_add(u[2], u[5]), t3 = c_add(u[3], u[4]);
+    const cpx<R> d1 = c_sub(u[1], u[6]), d2 = c_sub(u[2], u[5]), d3 = c_sub(u[3], u[4]);
+    u[0] = c_add(a, c_add(t1, c_add(t2, t3)));
+    // cos(2 pi jk/7) and sin(2 pi jk/7) for jk mod 7 = 1..6: cos(7-m) = cos(m), sin(7-m) = -sin(m)
+    const cpx<R> a1 = {a.x + c1 * t1.x + c2 * t2.x + c3 * t3.x, a.y + c1 * t1.y + c2 * t2.y + c3 * t3.y};
+    const cpx<R> a2 = {a.x + c2 * t1.x + c3 * t2.x + c1 * t3.x, a.y + c2 * t1.y + c3 * t2.y + c1 * t3.y};
+    const cpx<R> a3 = {a.x + c3 * t1.x + c1 * t2.x + c2 * t3.x, a.y + c3 * t1.y + c1 * t2.y + c2 * t3.y};
+    // DIR-signed b_k: k = 1: (s1, s2, s3); k = 2: (s2, -s3, -s1); k = 3: (s3, -s1, s2)
+    const cpx<R> b1 = {s1 * d1.x + s2 * d2.x + s3 * d3.x, s1 * d1.y + s2 * d2.y + s3 * d3.y};
+    const cpx<R> b2 = {s2 * d1.x - s3 * d2.x - s1 * d3.x, s2 * d1.y - s3 * d2.y - s1 * d3.y};
+    const cpx<R> b3 = {s3 * d1.x - s1 * d2.x + s2 * d3.x, s3 * d1.y - s1 * d2.y + s2 * d3.y};
+    // i * b = (-b.y, b.x)
+    u[1] = {a1.x - b1.y, a1.y + b1.x};
+    u[6] = {a1.x + b1.y, a1.y - b1.x};
+    u[2] = {a2.x - b2.y, a2.y + b2.x};
+    u[5] = {a2.x + b2.y, a2.y - b2.x};
+    u[3] = {a3.x - b3.y, a3.y + b3.x};
+    u[4] = {a3.x + b3.y, a3.y - b3.x};
+}
+
 template <int RAD, int DIR, typename R>
 __device__ __forceinline__ void dft(cpx<R> *u)
 {
@@ -210,6 +243,7 @@ __device__ __forceinline__ void dft(cpx<R> *u)
     else if constexpr (RAD == 3) dft3<DIR>(u);
     else if constexpr (RAD == 4) dft4<DIR>(u);
     else if constexpr (RAD == 5) dft5<DIR>(u);
+    else if constexpr (RAD == 7) dft7<DIR>(u);
     else if constexpr (RAD == 8) dft8<DIR>(u);
     else if constexpr (RAD == 16) dft16<DIR>(u);
 }

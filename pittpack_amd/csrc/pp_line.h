@@ -613,7 +613,8 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
 constexpr int line_threads(int M, int AX)
 {
     // E = M / T elements per thread: 8 for power-of-two lengths, 12 with a factor 3, 10 or 20
-    // with a factor 5 (the radix-5 pass needs E divisible by 5); T = 1 below M = 8
+    // with a factor 5, 14 with a factor 7 (a radix-R pass needs E divisible by R); T = 1
+    // below M = 8
     (void)AX;
     switch (M) {
     case 4: return 1;
@@ -621,31 +622,37 @@ constexpr int line_threads(int M, int AX)
     case 8: return 1;
     case 10: return 1;
     case 12: return 1;
+    case 14: return 1;
     case 16: return 2;
     case 20: return 2;
     case 24: return 2;
+    case 28: return 2;
     case 32: return 4;
     case 40: return 2;
     case 48: return 4;
+    case 56: return 4;
     case 64: return 8;
     case 80: return 4;
     case 96: return 8;
+    case 112: return 8;
     case 128: return 16;
     case 160: return 8;
     case 192: return 16;
+    case 224: return 16;
     case 256: return 32;
     case 320: return 16;
     case 384: return 32;
+    case 448: return 32;
     case 512: return 64;
     default: return 0;
     }
 }
 
-// The compiled complex lengths M = N / 2 (line length N), radices 2, 3 and 5; every dispatch
-// switch over M expands this list
+// The compiled complex lengths M = N / 2 (line length N), radices 2, 3, 5 and 7; every
+// dispatch switch over M expands this list
 #define PP_FFT_LENGTHS(X) \
-    X(4) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
-        X(320) X(384) X(512)
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96) X(112) \
+        X(128) X(160) X(192) X(224) X(256) X(320) X(384) X(448) X(512)
 
 // Alternative launch shape of the FP64 transforms (chosen per stage by the autotuner): twice
 // the threads per line, half the elements per thread -- fewer registers and more waves in

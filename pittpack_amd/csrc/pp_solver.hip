@@ -367,7 +367,10 @@ bool use_ztiles(pp_solver *h)
 }
 
 // rows per z tile: whole lane rows of the z solve (16 J), zero padding below nz
-int ztile_rows(int nz) { return (nz + 15) / 16 * 16; }
+#ifndef ZTILE_PAD
+#define ZTILE_PAD 0
+#endif
+int ztile_rows(int nz) { return (nz + 15) / 16 * 16 + ZTILE_PAD; }
 
 int ensure_ztiles(pp_solver *h)
 {
@@ -420,6 +423,7 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
         z.data = zdata;
         z.tiled = zt ? 1 : 0;
         z.tile_w = zt ? zsolve_cpb<double>() : 0;
+        z.tile_rows = zt ? ztile_rows(h->g.nz) : 0;
         z.SX = h->g.nx;
         z.SY = h->g.ny;
         z.nz = h->g.nz;

@@ -44,6 +44,7 @@ struct ZArgs {
     // CPB -- a block then takes CPB columns of a tile
     int tiled;
     int tile_w;
+    int tile_rows;  // rows per tile (0: 16 J); more than 16 J pads each tile at its end
 };
 
 // one solve of a batched z launch: its field sits at `offset` in the launch's data arena
@@ -231,14 +232,14 @@ __device__ __forceinline__ void ztile_load(R *tile, const ZArgs &a, int n, LRow 
         if (a.tile_w == 0 || a.tile_w == CPB) {
             // z tiles: 16 J rows (zero padding below nz) x CPB columns, contiguous; row r of
             // the tile is element r * CPB + c, so thread tid = g * CPB + c reads tid + q * 16 * CPB
-            const S *__restrict__ src = reinterpret_cast<const S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+            const S *__restrict__ src = reinterpret_cast<const S *>(a.data) + ((long)bx * a.SY + by) * (a.tile_rows ? a.tile_rows : 16 * J) * CPB;
 #pragma unroll
             for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[q * 16 * CPB + tid];
         } else {
             // wider tiles: this block's CPB columns of tile bx / (W / CPB), rows W apart
             const int W = a.tile_w, sub = W / CPB;
             const S *__restrict__ src = reinterpret_cast<const S *>(a.data) +
-                                        ((long)(bx / sub) * a.SY + by) * (16 * J) * W + (bx % sub) * CPB + g * W + c;
+                                        ((long)(bx / sub) * a.SY + by) * (a.tile_rows ? a.tile_rows : 16 * J) * W + (bx % sub) * CPB + g * W + c;
 #pragma unroll
             for (int q = 0; q < J; q++) tile[lrow(g + 16 * q) * (CPB + 1) + c] = (R)src[(long)q * 16 * W];
         }
@@ -265,12 +266,12 @@ __device__ __forceinline__ void ztile_store(const R *tile, const ZArgs &a, int n
     const int c = tid % CPB, g = tid / CPB;
     if (a.tiled) {
         if (a.tile_w == 0 || a.tile_w == CPB) {
-            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)bx * a.SY + by) * (16 * J) * CPB;
+            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)bx * a.SY + by) * (a.tile_rows ? a.tile_rows : 16 * J) * CPB;
 #pragma unroll
             for (int q = 0; q < J; q++) dst[q * 16 * CPB + tid] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];
         } else {
             const int W = a.tile_w, sub = W / CPB;
-            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)(bx / sub) * a.SY + by) * (16 * J) * W +
+            S *__restrict__ dst = reinterpret_cast<S *>(a.data) + ((long)(bx / sub) * a.SY + by) * (a.tile_rows ? a.tile_rows : 16 * J) * W +
                                   (bx % sub) * CPB + g * W + c;
 #pragma unroll
             for (int q = 0; q < J; q++) dst[(long)q * 16 * W] = (S)tile[lrow(g + 16 * q) * (CPB + 1) + c];

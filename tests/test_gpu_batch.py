@@ -61,7 +61,7 @@ def test_batch_matches_single_and_oracle():
             assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
 
 
-# radix-5 and short lines (N = 8 .. 640) next to radix-2/3 ones in one batch
+# radix-5/7 and short lines (N = 8 .. 640) next to radix-2/3 ones in one batch
 LENGTH_MEMBERS = [
     (16, 12, 10, "NNNNDD", [0, 1, 0, 1, 0, 1]),
     (24, 20, 18, "DDNDDD", [0, 1, 0, 1, 0, 1]),
@@ -69,6 +69,7 @@ LENGTH_MEMBERS = [
     (640, 8, 4, "DDDDDD", [0, 1, 0, 1, 0, 1]),
     (160, 320, 4, "NDDNDN", [0, 1, 0, 1, 0, 1]),
     (20, 40, 33, "DDNNNN", [0, 1, 0, 1, 0, 1]),
+    (112, 56, 9, "NNNNDD", [0, 1, 0, 2, 0, 1]),
     (32, 48, 17, "PPPPNN", [0, 1, 0, 1, 0, 1]),
 ]
 

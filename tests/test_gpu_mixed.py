@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 
-SHAPES = [(10, 16, 32), (33, 64, 16), (7, 16, 96), (18, 20, 24), (6, 160, 40), (5, 12, 320)]
+SHAPES = [(10, 16, 32), (33, 64, 16), (7, 16, 96), (18, 20, 24), (6, 160, 40), (5, 12, 320), (9, 56, 28)]
 COMBOS = ["PPPPNN", "NNNNDD", "DDDDDD", "NNNNNN", "PPPPDD", "DDNDDD", "NDDNDN", "DNNNDD"]
 
 

@@ -116,9 +116,10 @@ def test_long_columns_limits_and_autotune():
 
 
 # line lengths beyond the power-of-two/radix-3 set: the reference's small fixture sizes
-# 16 x 12 x 10 and 24 x 20 x 18 (N = 12, 20), the shortest lines (N = 8, 12) and radix-5
-# lengths up to N = 640 (shape (nz, ny, nx))
-LENGTH_SHAPES = [(10, 12, 16), (18, 20, 24), (12, 8, 40), (9, 80, 20), (5, 160, 12), (4, 40, 640), (4, 320, 8)]
+# 16 x 12 x 10 and 24 x 20 x 18 (N = 12, 20), the shortest lines (N = 8, 12), radix-5
+# lengths up to N = 640 and radix-7 lengths up to N = 896 (shape (nz, ny, nx))
+LENGTH_SHAPES = [(10, 12, 16), (18, 20, 24), (12, 8, 40), (9, 80, 20), (5, 160, 12), (4, 40, 640), (4, 320, 8),
+                 (7, 28, 56), (5, 448, 112), (4, 56, 896), (6, 224, 28)]
 
 
 @pytest.mark.parametrize("shape", LENGTH_SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -233,7 +234,7 @@ def test_call_order_and_errors():
 ZPER_COMBOS = ["PPPPPP", "NNNNPP", "DDDDPP", "DDNNPP", "NDDNPP", "NNPPPP"]
 
 
-@pytest.mark.parametrize("nz", [8, 10, 16, 20, 24, 40, 48, 64, 96, 128, 320, 512])
+@pytest.mark.parametrize("nz", [8, 10, 14, 16, 20, 24, 40, 48, 56, 64, 96, 128, 320, 448, 512])
 @pytest.mark.parametrize("bc", ZPER_COMBOS)
 def test_parity_periodic_z(bc, nz):
     print(f"periodic z {bc} nz={nz}", flush=True)
@@ -258,7 +259,7 @@ def test_periodic_z_residual_and_limits():
     r = fe - oracle.apply_operator(u, "PPDDPP")
     assert float(np.abs(r).max()) <= 1e-12 * max(np.abs(fe).max(), 12.0 * np.abs(u).max())
     with pytest.raises(pa.PittPackError) as e:
-        pa.solve(rng.uniform(-1, 1, (14, 16, 32)), "NNNNPP")  # 14 = 2 x 7: no compiled FFT
+        pa.solve(rng.uniform(-1, 1, (22, 16, 32)), "NNNNPP")  # 22 = 2 x 11: no compiled FFT
     assert e.value.code == pa.UNSUPPORTED_SIZE
     with pytest.raises(pa.PittPackError) as e:
         pa.solve(f, "NNNNPP", precision=1)

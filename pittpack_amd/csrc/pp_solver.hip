@@ -367,10 +367,7 @@ bool use_ztiles(pp_solver *h)
 }
 
 // rows per z tile: whole lane rows of the z solve (16 J), zero padding below nz
-#ifndef ZTILE_PAD
-#define ZTILE_PAD 0
-#endif
-int ztile_rows(int nz) { return (nz + 15) / 16 * 16 + ZTILE_PAD; }
+int ztile_rows(int nz) { return (nz + 15) / 16 * 16; }
 
 int ensure_ztiles(pp_solver *h)
 {

@@ -92,8 +92,8 @@ int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int 
     if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
     if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
-    // FP64 members hand the z solve 16-column tiles, which the y transform forms for y lines
-    // of up to 512 cells
+    // FP64 members hand the z solve 16-column tiles, which the y transform forms where its
+    // blocks are 16 columns wide (y lines up to 512 cells, and 1024)
     if (precision == 0 && line_block_lines<double>(ny, 1) != zsolve_cpb<double>()) return PP_UNSUPPORTED_SIZE;
     BatchMember m;
     m.g.nx = nx;

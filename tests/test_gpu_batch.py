@@ -96,6 +96,29 @@ def test_batch_line_lengths(precision):
             assert err <= TOL, f"member {i} {bc} ({nx},{ny},{nz}): rel max diff {err:.3e}"
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_1024_point_y_lines(precision):
+    """y lines of 1024 points: 16 FP64 columns per y block (one 1024-thread block), so FP64
+    members take z tiles as for shorter lines"""
+    print(f"batch 1024-point y lines precision={precision}", flush=True)
+    members = [(16, 1024, 9, "DDNDDD", [0, 1, 0, 4, 0, 1]), (32, 64, 17, "PPPPNN", [0, 1, 0, 1, 0, 1])]
+    with pa.Batch(precision=precision) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in members]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, members):
+            f = prng.uniform_pm1((nz, ny, nx), seed=700 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, members):
+            got = B.get_solution(i)
+            if precision == 0:
+                assert np.array_equal(got, pa.solve(rhs[i], bc, box)), f"member {i}: batched != single"
+            err = rel_maxdiff(got, oracle.solve(rhs[i], bc, box, zero_mode=1))
+            assert err <= TOL, f"member {i} {bc} ({nx},{ny},{nz}): rel max diff {err:.3e}"
+
+
 def test_batch_repeated_and_device_fill():
     """device generator == Python generator; the right-hand sides are kept, so a second solve
     reproduces the first"""

@@ -683,15 +683,9 @@ constexpr int line_lines()
     return lines;
 }
 
-template <typename TIn, typename TOut, typename R, int M, int AX, int FWD, int KIND, int V = 0>
-hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+hipError_t launch_line_blocks(const LineArgs &a, hipStream_t s)
 {
-    constexpr int T = V ? line_threads_alt(M) : line_threads(M, AX);
-    if constexpr (T == 0) {
-        return hipErrorInvalidValue;
-    } else {
-    // y keeps the default's columns per block in either shape (the z-tile width)
-    constexpr int LINES = (V && AX == 1) ? line_lines<R, M, AX>() : line_lines<R, M, AX, T>();
     const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>);
     auto kern = k_line<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>;
     if (lds > 64 * 1024) {
@@ -709,6 +703,24 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
     else grid = dim3((unsigned)((a.nlines + LINES - 1) / LINES), (unsigned)a.nplanes);
     hipLaunchKernelGGL(kern, grid, block, lds, s, a);
     return hipGetLastError();
+}
+
+template <typename TIn, typename TOut, typename R, int M, int AX, int FWD, int KIND, int V = 0>
+hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
+{
+    constexpr int T = V ? line_threads_alt(M) : line_threads(M, AX);
+    if constexpr (T == 0) {
+        return hipErrorInvalidValue;
+    } else {
+        // y keeps the default's columns per block in either shape (the z-tile width)
+        constexpr int LINES = (V && AX == 1) ? line_lines<R, M, AX>() : line_lines<R, M, AX, T>();
+        // the 16-column FP64 blocks of 1024-point y lines (one block per CU) pay off where they
+        // write or read whole z-tile rows; on the field in place two 8-column blocks per CU
+        // run faster
+        if constexpr (AX == 1 && sizeof(R) == 8 && T >= 64 && LINES == 16) {
+            if (!a.ztiled && !a.items) return launch_line_blocks<TIn, TOut, R, M, T, LINES / 2, AX, FWD, KIND>(a, s);
+        }
+        return launch_line_blocks<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, s);
     }
 }
 

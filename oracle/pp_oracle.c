@@ -15,12 +15,14 @@
  * counterpart and is pinned by closed forms (scipy's DCT-IV, dense solves) in tests/.
  *
  * Pinning (tests/test_oracle.py): FFT against numpy; each signalProc step against scipy's
- * DCT/DST definitions; Thomas / Sherman-Morrison against dense LAPACK solves; the whole
- * solve against a dense 7-point ghost-cell operator on tiny grids, the discrete residual
- * and second-order convergence on manufactured solutions.  Parity unpinned (no reference
- * run is possible here, see the project notes): the exact (0,0)-mode constant of the
- * row-0 rule under z-NN beyond "rows 1..n-1 of the original system hold and u0=r0", and
- * the singular periodic-z averaging of shermanMorrisonThomasV1.
+ * DCT/DST definitions; Thomas / Sherman-Morrison against dense LAPACK solves, including the
+ * singular periodic-z branch (shermanMorrisonThomasV1 = the cyclic system with row 0
+ * replaced by u0 + u_{N-1} = (r0 + r_{N-1})/2, dense solve and its physical form on whole
+ * solves); the whole solve against a dense 7-point ghost-cell operator on tiny grids, the
+ * discrete residual and second-order convergence on manufactured solutions; the Dirichlet
+ * boundary term through the reference's own manufactured solution on shifted boxes with
+ * O(1) face values (second order only if its sign, factor and face coordinates are right).
+ * Planted sign / factor / coordinate errors in td_smt_v1 and ppo_dirichlet fail those tests.
  */
 #ifdef _OPENMP
 #include <omp.h>

@@ -225,3 +225,74 @@ def test_config1_manufactured_64():
     got = got - got.mean() + u.mean()
     err = np.sqrt(np.mean((got - u) ** 2))
     assert err < 2e-3
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 16, 33, 64])
+def test_singular_periodic_column_vs_dense(n):
+    """Singular periodic-z column (eig = -2, shermanMorrisonThomasV1, src/triDiag.cpp:2112-2255):
+    the answer is the cyclic system whose row 0 is replaced by
+    u_0 + u_{N-1} = (r_0 + r_{N-1}) / 2, rows 1..N-1 unchanged (a dense LAPACK solve)."""
+    rng = np.random.default_rng(n)
+    r = rng.standard_normal(n)
+    A = np.diag(np.full(n, -2.0)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    A[0, -1] = A[-1, 0] = 1.0
+    A[0, :] = 0.0
+    A[0, 0] = A[0, -1] = 1.0
+    b = r.copy()
+    b[0] = 0.5 * (r[0] + r[-1])
+    assert rel(oracle.zsolve_line(r, -2.0, "PP"), np.linalg.solve(A, b)) < 1e-13
+
+
+@pytest.mark.parametrize("bc", ["PPPPPP", "NNNNPP", "NNPPPP"])
+def test_singular_periodic_column_physical(bc):
+    """The same rule seen on the whole solve: every component except the xy-mean satisfies
+    L u = f_eff exactly, the xy-mean satisfies its cyclic z equation on planes 1..N-1, and on
+    plane 0 the xy-means obey u_0 + u_{N-1} = (f_0 + f_{N-1}) / 2."""
+    shape = (10, 8, 12)
+    rng = np.random.default_rng(3)
+    f = rng.uniform(-1, 1, shape)
+    fe = oracle.effective_rhs(f, bc)
+    u = oracle.solve(f, bc)
+    r = fe - oracle.apply_operator(u, bc)
+    scale = np.abs(fe).max() + 12 * np.abs(u).max()
+    rm = r.mean(axis=(1, 2))
+    assert np.abs(r - rm[:, None, None]).max() < 1e-13 * scale
+    assert np.abs(rm[1:]).max() < 1e-13 * scale
+    um, fm = u.mean(axis=(1, 2)), fe.mean(axis=(1, 2))
+    assert abs(um[0] + um[-1] - 0.5 * (fm[0] + fm[-1])) < 1e-13 * scale
+
+
+def reference_mms(bc, box, n):
+    """The reference's built-in manufactured solution (initializeTrigonometric / exactValue,
+    src/pencilDcmp.cpp:1837-1866,1871-2035): u = prod_d g_d(omega_d x_d) with
+    omega = (1, 2, 3) pi and g = sin on Dirichlet axes, cos on Neumann axes (z: sin), and
+    f = -|omega|^2 u.  Its Dirichlet face values feed modifyRhsDirichlet."""
+    om = np.pi * np.array([1.0, 2.0, 3.0])
+    c = [box[2 * a] + (np.arange(n) + 0.5) * (box[2 * a + 1] - box[2 * a]) / n for a in range(3)]
+    z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+    gx = np.sin if bc[0] == "D" else np.cos
+    gy = np.sin if bc[2] == "D" else np.cos
+    u = gx(om[0] * x) * gy(om[1] * y) * np.sin(om[2] * z)
+    return u, -(om ** 2).sum() * u
+
+
+@pytest.mark.parametrize("bc,box", [
+    ("DDDDDD", [-0.3, 0.7, 0.2, 1.2, -0.3, 0.7]),
+    ("NNNNDD", [0.0, 1.0, 0.0, 1.0, -0.3, 0.7]),
+    ("DDNNDD", [-0.3, 0.7, 0.0, 1.0, -0.3, 0.7]),
+])
+def test_dirichlet_term_second_order(bc, box):
+    """Dirichlet boundary term (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835): on boxes
+    shifted so the face values of the manufactured solution are O(1) (dx = dy = dz, as the
+    reference's term assumes), the solve converges to the exact solution at second order.
+    A wrong sign or factor on the term leaves an O(1) error that does not shrink."""
+    errs = []
+    for n in (16, 32, 64):
+        u, f = reference_mms(bc, box, n)
+        faces = [u[:, :, 0], u[:, :, -1], u[:, 0, :], u[:, -1, :], u[0], u[-1]]
+        assert max(np.abs(g).max() for g, c in zip(faces, bc) if c == "D") > 0.3
+        got = oracle.solve(f, bc, box, zero_mode=1)
+        errs.append(np.sqrt(np.mean((got - u) ** 2)))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.8 < r1 < 4.2 and 3.8 < r2 < 4.2, (bc, errs)
+    assert errs[-1] < 1e-3

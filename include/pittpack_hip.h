@@ -86,7 +86,11 @@ enum {
     PP_OPT_AUTOTUNE = 11,
     /* testing: force every FP64 line stage to the default (0) or alternative (1) launch
      * shape where it has one; -1 (default) = the autotuner's choice */
-    PP_OPT_LINE_VARIANT = 12
+    PP_OPT_LINE_VARIANT = 12,
+    /* server: how many admitted batches may overtake a queued request that does not fit
+     * before nothing behind it is admitted (default 4); bounds the wait of large requests
+     * under a stream of small ones */
+    PP_OPT_MAX_OVERTAKE = 13
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
@@ -235,16 +239,24 @@ typedef struct pp_server_s *pp_server;
 /* staging_bytes of pinned host memory hold solutions on their way back to callers */
 int pp_server_create(long long hbm_bytes, long long staging_bytes, int streams, pp_server *s);
 int pp_server_destroy(pp_server s);
-/* PP_OPT_REFINE, PP_OPT_BATCH_MEMBERS */
+/* PP_OPT_REFINE, PP_OPT_BATCH_MEMBERS, PP_OPT_MAX_OVERTAKE */
 int pp_server_set_option(pp_server s, int key, long long value);
 /* one request: sizes, bc[6], box X[6], precision (0 FP64, 1 mixed), right-hand side rhs
  * (host, nx*ny*nz doubles, x fastest, kept alive until the request is done) or, when rhs is
- * null, the counter generator with `seed`; want_solution = 1 to receive the solution */
+ * null, the counter generator with `seed`; want_solution = 1 to receive the solution.
+ * The request is planned here: unsupported sizes / boundary conditions return their code, and
+ * PP_ALLOCATION_FAIL when its exact device footprint exceeds the pool. */
 int pp_server_submit(pp_server s, int nx, int ny, int nz, const char *bc, const double *X, int precision,
                      const double *rhs, unsigned long long seed, int want_solution, long long *ticket);
 /* blocks until the request is done; fills u (when it was requested and u is not null) and
  * the submit-to-done latency; returns the request's status */
 int pp_server_wait(pp_server s, long long ticket, double *u, double *latency_ms);
+/* device footprint (bytes of the pool) a request of this shape, boundary and precision takes */
+int pp_server_footprint(pp_server s, int nx, int ny, int nz, const char *bc, const double *X, int precision,
+                        long long *bytes);
+/* the same as pp_server_wait, also returning the launch number of the batch that ran the
+ * request */
+int pp_server_wait_ex(pp_server s, long long ticket, double *u, double *latency_ms, long long *batch_seq);
 int pp_server_stats(pp_server s, long long *completed, long long *points, long long *batches, double *span_ms,
                     int *max_concurrent);
 

@@ -41,6 +41,7 @@ OPT_MG_TOL = 9
 OPT_GRAPH = 10
 OPT_AUTOTUNE = 11
 OPT_LINE_VARIANT = 12
+OPT_MAX_OVERTAKE = 13
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -54,7 +55,7 @@ EXPORTED = (
     "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
     "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
     "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
-    "pp_server_stats", "pp_batch_tuning",
+    "pp_server_stats", "pp_batch_tuning", "pp_server_wait_ex", "pp_server_footprint",
 )
 
 
@@ -133,6 +134,8 @@ def load_library(path: str | None = None):
     L.pp_server_set_option.argtypes = [vp, i, ll]
     L.pp_server_submit.argtypes = [vp, i, i, i, ctypes.c_char_p, dp, i, dp, ctypes.c_ulonglong, i, ctypes.POINTER(ll)]
     L.pp_server_wait.argtypes = [vp, ll, dp, dp]
+    L.pp_server_wait_ex.argtypes = [vp, ll, dp, dp, ctypes.POINTER(ll)]
+    L.pp_server_footprint.argtypes = [vp, i, i, i, ctypes.c_char_p, dp, i, ctypes.POINTER(ll)]
     L.pp_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp, ip]
     L.pp_batch_set_option_double.argtypes = [vp, i, ctypes.c_double]
     L.pp_batch_synchronize.argtypes = [vp]
@@ -433,13 +436,24 @@ class Server:
         self._pending[t.value] = (f, X, (nz, ny, nx), bool(want_solution))
         return t.value
 
-    def wait(self, ticket: int):
+    def wait(self, ticket: int, with_batch: bool = False):
+        """(solution or None, latency ms[, launch number of the batch that ran it])"""
         f, X, shape, want = self._pending.pop(ticket)
         u = np.empty(shape, dtype=np.float64) if want else None
         lat = ctypes.c_double(0)
-        _check(self._L.pp_server_wait(self._h, int(ticket), _dp(u) if u is not None else None, ctypes.byref(lat)),
-               "server_wait")
-        return u, lat.value
+        seq = ctypes.c_longlong(-1)
+        _check(self._L.pp_server_wait_ex(self._h, int(ticket), _dp(u) if u is not None else None, ctypes.byref(lat),
+                                         ctypes.byref(seq)), "server_wait")
+        return (u, lat.value, seq.value) if with_batch else (u, lat.value)
+
+    def footprint(self, shape, bc: str, box=None, precision: int = 0) -> int:
+        """device bytes a request of this shape takes in the pool"""
+        nz, ny, nx = shape
+        X = np.ascontiguousarray(box if box is not None else [0, 1, 0, 1, 0, 1], dtype=np.float64)
+        b = ctypes.c_longlong(0)
+        _check(self._L.pp_server_footprint(self._h, int(nx), int(ny), int(nz), bc.encode(), _dp(X), int(precision),
+                                           ctypes.byref(b)), "server_footprint")
+        return b.value
 
     def stats(self):
         c, p, b = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)

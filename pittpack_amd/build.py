@@ -50,13 +50,28 @@ def _compile(src: str, extra) -> str:
     return obj
 
 
-def build(verbose: bool = False, extra=(), jobs: int | None = None) -> str:
+def _fresh(src: str, newest_header: float) -> bool:
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+    return os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header)
+
+
+def build(verbose: bool = False, extra=(), jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every translation unit (incremental=True: only those older than their source
+    or than any header) and link the library."""
     os.makedirs(OUT_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(SRC, "*.hip")))
+    headers = glob.glob(os.path.join(SRC, "*.h")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    newest = max((os.path.getmtime(h) for h in headers), default=0.0)
     jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
+
+    def one(s):
+        if incremental and not extra and _fresh(s, newest):
+            return os.path.join(OBJ_DIR, os.path.basename(s) + ".o")
+        return _compile(s, list(extra))
+
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, list(extra)), srcs))
+        objs = list(ex.map(one, srcs))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
@@ -70,4 +85,6 @@ def build(verbose: bool = False, extra=(), jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True, extra=sys.argv[1:]))
+    args = sys.argv[1:]
+    inc = "--incremental" in args
+    print(build(verbose=True, extra=[a for a in args if a != "--incremental"], incremental=inc))

@@ -19,6 +19,7 @@
 #include "pittpack_hip.h"
 #include "pp_batch_core.h"
 #include "pp_launch.h"
+#include "pp_line_bs.h"
 
 #define PPB_CHECK(x)                                  \
     do {                                              \
@@ -92,9 +93,12 @@ int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int 
     if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
     if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
-    // FP64 members hand the z solve 16-column tiles, which the y transform forms where its
-    // blocks are 16 columns wide (y lines up to 512 cells, and 1024)
-    if (precision == 0 && line_block_lines<double>(ny, 1) != zsolve_cpb<double>()) return PP_UNSUPPORTED_SIZE;
+    // FP64 members hand the z solve 16-column tiles, which y blocks of 16 columns fill alone
+    // and narrower y blocks (a divisor of 16) fill together
+    if (precision == 0) {
+        const int ly = line_lines_any<double>(ny, 1);
+        if (ly <= 0 || zsolve_cpb<double>() % ly != 0) return PP_UNSUPPORTED_SIZE;
+    }
     BatchMember m;
     m.g.nx = nx;
     m.g.ny = ny;
@@ -124,7 +128,7 @@ int BatchCore::plan_lines(int axis, std::vector<HostClass> &out)
     for (int m = 0; m < (int)mem.size(); m++) {
         const SolveSetup &g = mem[m].g;
         const int N = axis == 0 ? g.nx : g.ny;
-        const int lines = line_block_lines<R>(N, axis);
+        const int lines = line_lines_any<R>(N, axis);
         if (lines <= 0) return PP_UNSUPPORTED_SIZE;
         auto &items = cls[{N, g.kind[axis]}];
         if (axis == 0) {
@@ -486,6 +490,8 @@ int BatchCore::launch_class(hipStream_t s, const HostClass &c, int form, int axi
         a.plane_add = plane;
         a.accumulate = accumulate;
         a.ztiled = ztiled;
+        // z tiles are 16 columns wide; y blocks of fewer columns fill them together
+        a.tile_w = ztiled ? zsolve_cpb<double>() : 0;
         PPB_CHECK(launch_line(form, c.kind, c.key, axis, fwd, a, s));
     }
     return PP_SUCCESS;

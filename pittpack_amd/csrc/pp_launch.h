@@ -43,8 +43,11 @@ struct ZPerArgs {
     const double *tw;
     const double *c2;
 };
+// one-pass kernel: nz with a compiled complex plan (zperiodic_supported); other nz: real
+// z-line transforms around launch_zper_divide
 bool zperiodic_supported(int nz);
 hipError_t launch_zperiodic_f64(const ZPerArgs &a, hipStream_t s);
+hipError_t launch_zper_divide(const ZPerArgs &a, hipStream_t s);
 hipError_t launch_smt_v1(double *data, int SX, int SY, int nz, int sx, int sy, double diag, hipStream_t s);
 
 // counter-generator right-hand side (pp_solver.hip): out[i] = 2 u - 1,

@@ -88,6 +88,9 @@ struct LineArgs {
     int tile_w;  // tile width W (0: LINES, the block's column count); a multiple of LINES
     // host side, FP64 form: 1 = the alternative launch shape (line_threads_alt), 0 = default
     int variant;
+    // line length; read by the arbitrary-length kernels (pp_line_bs.h), whose length is not a
+    // template parameter
+    int n;
 };
 enum : int { ZT_OUT = 1, ZT_IN = 2 };
 
@@ -158,19 +161,15 @@ __device__ __forceinline__ R &lr(cpx<R> *L, int p)
     return reinterpret_cast<R *>(&L[padc(p >> 1)])[p & 1];
 }
 
-template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
-// power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
-// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers, and so does
-// the FP32 -> FP64 x inverse of 1024-point lines (its prefetched solution would spill)
-__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
-                                            !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512))
-                                               ? 4
-                                               : 1) k_line(LineArgs a0)
+// Block coordinates of a line launch.  y blocks narrower than a cache line keep the tiles of
+// one plane on one XCD; a batched launch (a0.items) takes its block's item and the item's
+// solve: per-solve fields, sizes and scalings are resolved into `a`.
+template <typename TIn, typename TOut, typename R, int LINES, int AX>
+__device__ __forceinline__ void line_block(const LineArgs &a0, LineArgs &a, int &bx, int &by)
 {
-    LineArgs a = a0;
-    int bx = blockIdx.x, by = blockIdx.y;
+    bx = blockIdx.x;
+    by = blockIdx.y;
     if constexpr (AX == 1 && LINES * sizeof(R) < 128) {
-        // y tiles narrower than a cache line: keep the tiles of one plane on one XCD
         if (!a0.items) {
             const int lin = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
             bx = lin % gridDim.x;
@@ -202,6 +201,89 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
         bx = it.bx;
         by = it.by;
     }
+}
+
+// Where line l of block (bx, by) lives: element n at base + n * stride (tbase + n * tstride
+// on the z-tiled side of a y transform).  Lines past the end of the grid map to line 0 (in
+// bounds) with valid = false: they are transformed like the others and never stored.
+struct LineAddr {
+    long base, stride, tbase, tstride;
+    bool valid;
+    int ly, lz;  // x lines: the line's y and z
+};
+
+template <int LINES, int AX>
+__device__ __forceinline__ LineAddr line_addr(const LineArgs &a, int N, int bx, int by, int l)
+{
+    LineAddr d{};
+    if constexpr (AX == 0) {
+        long line = (long)bx * LINES + l;
+        d.valid = line < a.nlines;
+        if (!d.valid) line = 0;
+        d.base = line * N;
+        d.stride = 1;
+        d.ly = (int)(line % a.ny);
+        d.lz = (int)(line / a.ny);
+    } else {
+        long col = (long)bx * LINES + l;
+        d.valid = col < a.ncols;
+        if (!d.valid) col = 0;
+        d.base = (long)by * N * a.ncols + col;
+        d.stride = a.ncols;
+        if (a.ztiled) {
+            if (a.tile_w == 0 || a.tile_w == LINES) {
+                // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
+                d.tbase = ((long)bx * N * a.tnz + by) * LINES + l;
+                d.tstride = (long)a.tnz * LINES;
+            } else {
+                // tiles W = tile_w columns wide (a multiple of LINES): this block fills LINES of
+                // tile bx / (W / LINES); neighbouring blocks, which complete each row, are
+                // numbered onto one XCD (W * 8 bytes is one 128-byte line)
+                const int W = a.tile_w, sub = W / LINES;
+                d.tbase = ((long)(bx / sub) * N * a.tnz + by) * W + (bx % sub) * LINES + l;
+                d.tstride = (long)a.tnz * W;
+            }
+        }
+    }
+    return d;
+}
+
+// Dirichlet boundary term of one x line (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835):
+// the z- and y-face terms are ex[n] times a line constant cyz and the x-face terms hit
+// elements 0 (cx0) and N-1 (cxn) only, so the line constants are formed once per thread
+struct DirLine {
+    double cyz = 0.0, cx0 = 0.0, cxn = 0.0;
+};
+
+__device__ __forceinline__ DirLine dirichlet_line(const LineArgs &a, int ly, int lz)
+{
+    DirLine d;
+    if (a.dmask) {
+        const double *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
+        const double *ef = a.dtab + a.nx + a.ny + a.nz;
+        const double eyl = gld(ey, ly), ezl = gld(ez, lz);
+        if ((a.dmask & 16) && lz == 0) d.cyz += 2. * eyl * gld(ef, 4);
+        if ((a.dmask & 32) && lz == a.nz - 1) d.cyz += 2. * eyl * gld(ef, 5);
+        if ((a.dmask & 4) && ly == 0) d.cyz += 2. * gld(ef, 2) * ezl;
+        if ((a.dmask & 8) && ly == a.ny - 1) d.cyz += 2. * gld(ef, 3) * ezl;
+        if (a.dmask & 1) d.cx0 = 2. * gld(ef, 0) * eyl * ezl;
+        if (a.dmask & 2) d.cxn = 2. * gld(ef, 1) * eyl * ezl;
+    }
+    return d;
+}
+
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+// power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
+// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers, and so does
+// the FP32 -> FP64 x inverse of 1024-point lines (its prefetched solution would spill)
+__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
+                                            !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512))
+                                               ? 4
+                                               : 1) k_line(LineArgs a0)
+{
+    LineArgs a = a0;
+    int bx, by;
+    line_block<TIn, TOut, R, LINES, AX>(a0, a, bx, by);
     constexpr int N = 2 * M;
     constexpr int E = M / T;
     constexpr int LS = line_slots<M>();
@@ -223,39 +305,10 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
     const TIn *in = reinterpret_cast<const TIn *>(a.in);
     TOut *out = reinterpret_cast<TOut *>(a.out);
 
-    long base, stride;
-    long tbase = 0, tstride = 0;  // z-tiled side of a y transform
-    bool valid;
-    int ly = 0, lz = 0;
-    if constexpr (AX == 0) {
-        long line = (long)bx * LINES + l;
-        valid = line < a.nlines;
-        if (!valid) line = 0;
-        base = line * N;
-        stride = 1;
-        ly = (int)(line % a.ny);
-        lz = (int)(line / a.ny);
-    } else {
-        long col = (long)bx * LINES + l;
-        valid = col < a.ncols;
-        if (!valid) col = 0;
-        base = (long)by * N * a.ncols + col;
-        stride = a.ncols;
-        if (a.ztiled) {
-            if (a.tile_w == 0 || a.tile_w == LINES) {
-                // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
-                tbase = ((long)bx * N * a.tnz + by) * LINES + l;
-                tstride = (long)a.tnz * LINES;
-            } else {
-                // tiles W = tile_w columns wide (a multiple of LINES): this block fills LINES of
-                // tile bx / (W / LINES); neighbouring blocks, which complete each row, are
-                // numbered onto one XCD (W * 8 bytes is one 128-byte line)
-                const int W = a.tile_w, sub = W / LINES;
-                tbase = ((long)(bx / sub) * N * a.tnz + by) * W + (bx % sub) * LINES + l;
-                tstride = (long)a.tnz * W;
-            }
-        }
-    }
+    const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
+    const long base = ad.base, stride = ad.stride, tbase = ad.tbase, tstride = ad.tstride;
+    const bool valid = ad.valid;
+    const int lz = ad.lz;
 
     // output element n of this thread's line, with the fused output scaling
     auto put = [&](int n, double val) {
@@ -324,26 +377,11 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
     }
 
     // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
-    // Dirichlet term (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835): per line, the z- and
-    // y-face terms are ex[n] times a line constant and the x-face terms hit elements 0 and
-    // N-1 only, so the line constants are formed once and no table load sits in the element
-    // loop except ex[n] on face lines
-    double cyz = 0.0, cx0 = 0.0, cxn = 0.0;
+    // no table load sits in the element loop except ex[n] on face lines
     const double *ex = a.dtab;
     if constexpr (AX == 0 && FWD) {
-        if (a.dmask) {
-            const double *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
-            const double *ef = a.dtab + a.nx + a.ny + a.nz;
-            const double eyl = gld(ey, ly), ezl = gld(ez, lz);
-            if ((a.dmask & 16) && lz == 0) cyz += 2. * eyl * gld(ef, 4);
-            if ((a.dmask & 32) && lz == a.nz - 1) cyz += 2. * eyl * gld(ef, 5);
-            if ((a.dmask & 4) && ly == 0) cyz += 2. * gld(ef, 2) * ezl;
-            if ((a.dmask & 8) && ly == a.ny - 1) cyz += 2. * gld(ef, 3) * ezl;
-            if (a.dmask & 1) cx0 = 2. * gld(ef, 0) * eyl * ezl;
-            if (a.dmask & 2) cxn = 2. * gld(ef, 1) * eyl * ezl;
-        }
-    }
-    if constexpr (AX == 0 && FWD) {
+        const DirLine dl = dirichlet_line(a, ad.ly, lz);
+        const double cyz = dl.cyz, cx0 = dl.cx0, cxn = dl.cxn;
 #pragma unroll
         for (int c = 0; c < 2 * E; c++) {
             xin[c] = xin[c] * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
@@ -773,10 +811,11 @@ hipError_t launch_line_kind(int form, int N, int AX, int FWD, const LineArgs &a,
     }
 }
 
-inline bool line_size_supported(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }
+// a compiled Stockham plan serves length n (other lengths: pp_line_bs.h)
+inline bool line_compiled(int n) { return n % 2 == 0 && line_threads(n / 2, 0) > 0; }
 
 // the FP64 transforms of length N have an alternative launch shape
-inline bool line_has_variant(int n) { return line_size_supported(n) && line_threads_alt(n / 2) > 0; }
+inline bool line_has_variant(int n) { return line_compiled(n) && line_threads_alt(n / 2) > 0; }
 
 // lines (x) or columns (y) per block of the launch for line length N, precision R
 template <typename R>

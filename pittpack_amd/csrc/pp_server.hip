@@ -5,16 +5,23 @@
 // No counterpart in the reference (one grid per PencilDcmp object, src/pencilDcmp.cpp).
 //
 // Design: callers submit requests from any thread; one scheduler thread owns every HIP call.
-// It forms a batch per precision from the queued requests whose footprint fits the free part
-// of one device pool (first fit), plans and binds it, uploads its tables and right-hand
-// sides, launches it on a free stream of its own and records an event; it retires finished
-// batches by polling their events.  Twiddle tables stay resident for the server's lifetime;
+// Each request is planned alone at submission, which gives its exact device footprint (a
+// batch's region is at most the sum of its members' single-member regions, since every part
+// of the plan is a sum over members and each part is rounded up once).  The scheduler forms
+// a batch per precision from the queued requests, in arrival order, whose footprints fit the
+// free part of one device pool (first fit), plans and binds it, uploads its tables and
+// right-hand sides, launches it on a free stream of its own and records an event; it retires
+// finished batches by polling their events.  A request that does not fit may be overtaken
+// by later, smaller ones only a bounded number of times (PP_OPT_MAX_OVERTAKE); after that
+// nothing behind it is admitted until it runs, so a large request cannot starve under a
+// stream of small ones.  Twiddle tables stay resident for the server's lifetime;
 // per-request eigenvalue and boundary tables are planned on the host and uploaded with the
 // batch.  Solutions go device -> pinned staging on the batch's stream; the waiting caller
 // copies them into its own buffer, so the scheduler never blocks on a host copy.  A member's
 // arithmetic does not depend on its batch, so a result is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -107,6 +114,9 @@ struct Request {
     const double *rhs = nullptr;  // host right-hand side (null: counter generator with `seed`)
     unsigned long long seed = 0;
     bool want = false;            // copy the solution back to the host
+    size_t need = 0;              // device footprint: the region of this request planned alone
+    int overtaken = 0;            // batches admitted past this request while it did not fit
+    long long batch_seq = -1;     // launch number of the batch that ran it
     double t_submit = 0, t_launch = 0, t_done = 0;
     int status = 1;               // 1 queued, 2 running, 0 done, >1 error code (negated below)
     int rc = PP_SUCCESS;
@@ -135,6 +145,8 @@ struct pp_server_s {
     std::vector<hipEvent_t> events;
     int max_members = 256;
     int refine = 2;
+    int max_overtake = 4;
+    int group_cap = 1 << 30;  // halved when a group does not fit the empty pool
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request *> queue;
@@ -151,11 +163,19 @@ struct pp_server_s {
 
 namespace {
 
-// footprint estimate used for admission (the exact size comes from the plan)
-size_t estimate_bytes(const Request *r)
+// device footprint of one request: its region planned as a batch of one (the scheduler's
+// batches use the same refinement setting; a batch's region never exceeds the sum of its
+// members' regions).  Returns a PittPack code for a request that cannot be planned.
+int request_bytes(const Request *r, int refine, size_t &bytes)
 {
-    const size_t pts = (size_t)r->points();
-    return pts * (r->precision ? 20 : 24) + 65536;
+    BatchCore c;
+    c.precision = r->precision;
+    c.refine = refine;
+    int id = -1;
+    int rc = c.add(r->nx, r->ny, r->nz, r->bc, r->X, &id);
+    if (!rc) rc = c.plan();
+    bytes = rc ? 0 : c.region_bytes;
+    return rc;
 }
 
 void fail_request(pp_server_s *s, Request *r, int rc)
@@ -245,15 +265,22 @@ bool launch(pp_server_s *s, std::vector<Request *> &group, int slot)
     hipStream_t st = s->streams[slot];
     rc = f->core.bind(s->dev + off, &s->tables);
     if (!rc) rc = f->core.upload(st);
-    const double t = now_ms();
+    {
+        // callers read a request's status under the lock
+        std::lock_guard<std::mutex> lk(s->mu);
+        const double t = now_ms();
+        for (Request *r : ok) {
+            r->t_launch = t;
+            r->status = 2;
+            r->batch_seq = s->batches;
+        }
+    }
     for (size_t i = 0; !rc && i < ok.size(); i++) {
         Request *r = ok[i];
         double *d = f->core.d_rhs + f->core.mem[i].offset;
         const size_t bytes = (size_t)r->points() * sizeof(double);
         if (r->rhs) rc = hipMemcpyAsync(d, r->rhs, bytes, hipMemcpyHostToDevice, st) == hipSuccess ? 0 : PP_HIP_ERROR;
         else rc = launch_fill_uniform(d, (long)r->points(), r->seed, 0, st) == hipSuccess ? 0 : PP_HIP_ERROR;
-        r->t_launch = t;
-        r->status = 2;
     }
     if (!rc) rc = f->core.run(st, nullptr);
     for (size_t i = 0; !rc && i < ok.size(); i++) {
@@ -301,7 +328,11 @@ void worker_loop(pp_server_s *s)
                 if (s->stop && s->queue.empty()) return;
             }
             size_t budget = s->dpool.cap() - s->dpool.used();
-            for (int p = 0; p < 2; p++) {
+            // requests left behind this round because they did not fit; once one of them has
+            // been overtaken max_overtake times, nothing behind it is admitted
+            std::vector<Request *> skipped;
+            bool blocked = false;
+            for (int p = 0; p < 2 && !blocked; p++) {
                 int slot = -1;
                 for (int k = 0; k < (int)s->slot_busy.size(); k++)
                     if (!s->slot_busy[k] && k != slots[0]) {
@@ -309,37 +340,59 @@ void worker_loop(pp_server_s *s)
                         break;
                     }
                 if (slot < 0) break;
-                for (auto it = s->queue.begin(); it != s->queue.end() && (int)groups[p].size() < s->max_members;) {
+                const int cap = std::min(s->max_members, s->group_cap);
+                for (auto it = s->queue.begin(); it != s->queue.end() && (int)groups[p].size() < cap;) {
                     Request *r = *it;
-                    const size_t need = estimate_bytes(r);
-                    if (r->precision != p || need > budget) {
+                    if (r->precision != p) {
                         ++it;
                         continue;
                     }
+                    bool fits = r->need <= budget;
                     // solution staging (pinned host) for callers that want the result
-                    if (r->want && !r->staged) {
-                        if (!s->spool.alloc((size_t)r->points() * sizeof(double), r->stage_off)) {
-                            ++it;
-                            continue;
-                        }
-                        r->staged = true;
+                    if (fits && r->want && !r->staged) {
+                        if (s->spool.alloc((size_t)r->points() * sizeof(double), r->stage_off)) r->staged = true;
+                        else fits = false;
                     }
-                    budget -= need;
+                    if (!fits) {
+                        if (r->overtaken >= s->max_overtake) {
+                            blocked = true;
+                            break;
+                        }
+                        skipped.push_back(r);
+                        ++it;
+                        continue;
+                    }
+                    budget -= r->need;
                     groups[p].push_back(r);
                     it = s->queue.erase(it);
                 }
                 if (!groups[p].empty()) slots[p] = slot;
             }
+            // each admitted batch holding a later request overtakes a skipped one once
+            for (Request *r : skipped)
+                for (int p = 0; p < 2; p++)
+                    if (!groups[p].empty() && groups[p].back()->ticket > r->ticket) r->overtaken++;
         }
         for (int p = 0; p < 2; p++) {
             if (groups[p].empty()) continue;
             if (launch(s, groups[p], slots[p])) {
                 progressed = true;
-            } else {
-                // did not fit after planning: back to the front of the queue
                 std::lock_guard<std::mutex> lk(s->mu);
-                for (auto it = groups[p].rbegin(); it != groups[p].rend(); ++it) s->queue.push_front(*it);
+                s->group_cap = 1 << 30;
+                continue;
             }
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->inflight.empty() && groups[p].size() == 1) {
+                // alone in an empty pool and still not placed: it can never run
+                fail_request(s, groups[p][0], PP_ALLOCATION_FAIL);
+                progressed = true;
+                continue;
+            }
+            // the free space is fragmented by batches in flight (retry after they retire), or
+            // -- which the footprint bound rules out -- a group is too large for the empty
+            // pool (the next groups are then halved until they fit)
+            if (s->inflight.empty()) s->group_cap = std::max(1, (int)groups[p].size() / 2);
+            for (auto it = groups[p].rbegin(); it != groups[p].rend(); ++it) s->queue.push_front(*it);
         }
         if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -386,9 +439,10 @@ int pp_server_create(long long hbm_bytes, long long staging_bytes, int streams, 
         s->events.push_back(e);
     }
     s->slot_busy.assign(streams, false);
-    // resident twiddle tables for every compiled line length
+    // resident twiddle tables for every compiled line length (the arbitrary-length tables are
+    // made on first use and stay resident too)
     for (int n = 16; n <= 1024; n += 2)
-        if (line_size_supported(n)) {
+        if (line_compiled(n)) {
             AxisDev t;
             const int rc = s->tables.get(n, t);
             if (rc) return fail(rc);
@@ -412,6 +466,11 @@ int pp_server_set_option(pp_server s, int key, long long value)
         s->max_members = (int)value;
         return PP_SUCCESS;
     }
+    if (key == PP_OPT_MAX_OVERTAKE) {
+        if (value < 0 || value > 1 << 20) return PP_INPUT_ARGS;
+        s->max_overtake = (int)value;
+        return PP_SUCCESS;
+    }
     return PP_INPUT_ARGS;
 }
 
@@ -433,9 +492,19 @@ int pp_server_submit(pp_server s, int nx, int ny, int nz, const char *bc, const 
     r->rhs = rhs;
     r->seed = seed;
     r->want = want_solution != 0;
-    // reject what can never be admitted
-    if (estimate_bytes(r) > s->dpool.cap() ||
-        (r->want && (size_t)r->points() * sizeof(double) > s->spool.cap())) {
+    int refine;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        refine = s->refine;
+    }
+    // plan the request alone: unsupported sizes or boundary conditions are returned here, and
+    // what can never be admitted is rejected
+    const int prc = request_bytes(r, refine, r->need);
+    if (prc) {
+        delete r;
+        return prc;
+    }
+    if (r->need > s->dpool.cap() || (r->want && (size_t)r->points() * sizeof(double) > s->spool.cap())) {
         delete r;
         return PP_ALLOCATION_FAIL;
     }
@@ -452,9 +521,40 @@ int pp_server_submit(pp_server s, int nx, int ny, int nz, const char *bc, const 
     return PP_SUCCESS;
 }
 
-// blocks until the request is done; copies the solution (nx*ny*nz doubles) into u when it
-// was requested; latency_ms = submit -> done (scheduler clock)
+int pp_server_footprint(pp_server s, int nx, int ny, int nz, const char *bc, const double *X, int precision,
+                        long long *bytes)
+{
+    if (!s) return PP_INVALID_HANDLE;
+    if (!bc || !X || !bytes || (precision != 0 && precision != 1)) return PP_INPUT_ARGS;
+    Request r;
+    r.nx = nx;
+    r.ny = ny;
+    r.nz = nz;
+    for (int i = 0; i < 6; i++) {
+        r.bc[i] = bc[i];
+        r.X[i] = X[i];
+    }
+    r.precision = precision;
+    int refine;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        refine = s->refine;
+    }
+    size_t b = 0;
+    const int rc = request_bytes(&r, refine, b);
+    *bytes = (long long)b;
+    return rc;
+}
+
 int pp_server_wait(pp_server s, long long ticket, double *u, double *latency_ms)
+{
+    return pp_server_wait_ex(s, ticket, u, latency_ms, nullptr);
+}
+
+// blocks until the request is done; copies the solution (nx*ny*nz doubles) into u when it
+// was requested; latency_ms = submit -> done (scheduler clock); batch_seq = launch number of
+// the batch that ran it (-1 if it never ran)
+int pp_server_wait_ex(pp_server s, long long ticket, double *u, double *latency_ms, long long *batch_seq)
 {
     if (!s) return PP_INVALID_HANDLE;
     Request *r;
@@ -469,6 +569,7 @@ int pp_server_wait(pp_server s, long long ticket, double *u, double *latency_ms)
     const int rc = r->rc;
     if (!rc && r->staged && u) std::memcpy(u, s->stage + r->stage_off, (size_t)r->points() * sizeof(double));
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    if (batch_seq) *batch_seq = r->batch_seq;
     if (r->staged) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->spool.release(r->stage_off);

@@ -6,10 +6,12 @@
 // + exactValue, 4623-4835 and 1837-1866) and the FP64 twiddle tables of a line length.
 #pragma once
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "pittpack_hip.h"
 #include "pp_line.h"
+#include "pp_line_bs.h"
 
 namespace pp {
 
@@ -42,8 +44,116 @@ struct AxisTables {
     std::vector<double> tw, A, B, C, D;  // interleaved complex
 };
 
+// Tables of the arbitrary-length transforms (pp_line_bs.h) for line length N, convolution
+// length L = bs_len(N):
+//   tw = exp(-2 pi i e / L), e < L (Stockham twiddles);
+//   A  = P tables, C = Q tables: family {DCT/DST-II/III, DCT/DST-IV, real DFT} x {forward,
+//        inverse}, N complex each (pre- and post-twiddles with the chirp folded in);
+//   B  = FFT_L of the chirp kernel / L: chirp {half-angle (beta = 1/N), full (beta = 2/N)} x
+//        {forward, inverse}, L complex each.
+// Angles of chirp values are reduced exactly in integers (m^2 mod 4N or 2N) and evaluated in
+// long double; the kernel spectra come from a long-double radix-2 FFT.
+inline void make_bs_tables(int N, AxisTables &t)
+{
+    using ld = long double;
+    const ld pi = 3.141592653589793238462643383279502884L;
+    const int L = bs_len(N);
+    auto cis = [](ld a) { return std::pair<ld, ld>(std::cos(a), std::sin(a)); };
+    t.tw.resize(2 * (size_t)L);
+    for (int e = 0; e < L; e++) {
+        const auto w = cis(-2 * pi * (ld)e / (ld)L);
+        t.tw[2 * e] = (double)w.first;
+        t.tw[2 * e + 1] = (double)w.second;
+    }
+    // chirp c_m = exp(i pi beta m^2 / 2): half-angle exp(i pi m^2 / (2N)), full exp(i pi m^2 / N)
+    auto chirp = [&](int m, bool full) {
+        const long long per = full ? 2LL * N : 4LL * N;
+        const long long r = ((long long)m * m) % per;
+        return cis(full ? pi * (ld)r / (ld)N : pi * (ld)r / (2 * (ld)N));
+    };
+    // kernel spectra: b_j = c_j (j < N), b_{L-j} = c_j (0 < j < N), zero elsewhere; inverse
+    // direction uses conj(c)
+    t.B.assign(2 * (size_t)L * 4, 0.0);
+    for (int sec = 0; sec < 4; sec++) {
+        const bool full = sec >= 2, inv = sec & 1;
+        std::vector<ld> re(L, 0), im(L, 0);
+        for (int j = 0; j < N; j++) {
+            const auto c = chirp(j, full);
+            const ld ci = inv ? -c.second : c.second;
+            re[j] = c.first;
+            im[j] = ci;
+            if (j > 0) {
+                re[L - j] = c.first;
+                im[L - j] = ci;
+            }
+        }
+        // in-place radix-2 FFT (forward, exp(-2 pi i jk / L)), long double
+        for (int i = 1, j = 0; i < L; i++) {
+            int bit = L >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) {
+                std::swap(re[i], re[j]);
+                std::swap(im[i], im[j]);
+            }
+        }
+        for (int len = 2; len <= L; len <<= 1)
+            for (int i = 0; i < L; i += len)
+                for (int k = 0; k < len / 2; k++) {
+                    const auto w = cis(-2 * pi * (ld)k / (ld)len);
+                    const ld xr = re[i + k + len / 2] * w.first - im[i + k + len / 2] * w.second;
+                    const ld xi = re[i + k + len / 2] * w.second + im[i + k + len / 2] * w.first;
+                    re[i + k + len / 2] = re[i + k] - xr;
+                    im[i + k + len / 2] = im[i + k] - xi;
+                    re[i + k] += xr;
+                    im[i + k] += xi;
+                }
+        for (int j = 0; j < L; j++) {
+            t.B[2 * ((size_t)sec * L + j)] = (double)(re[j] / (ld)L);
+            t.B[2 * ((size_t)sec * L + j) + 1] = (double)(im[j] / (ld)L);
+        }
+    }
+    // pre (P) and post (Q) twiddles; forward: a_j = x_j pre_j conj(c_j), out from
+    // post_k conj(c_k) S'_k; inverse: pre_j c_j and post_k c_k
+    t.A.assign(2 * (size_t)N * 6, 0.0);
+    t.C.assign(2 * (size_t)N * 6, 0.0);
+    auto put = [&](std::vector<double> &v, int sec, int j, std::pair<ld, ld> a, std::pair<ld, ld> b) {
+        v[2 * ((size_t)sec * N + j)] = (double)(a.first * b.first - a.second * b.second);
+        v[2 * ((size_t)sec * N + j) + 1] = (double)(a.first * b.second + a.second * b.first);
+    };
+    for (int j = 0; j < N; j++) {
+        const auto ch = chirp(j, false), cf = chirp(j, true);
+        const std::pair<ld, ld> chc = {ch.first, -ch.second}, cfc = {cf.first, -cf.second};
+        const std::pair<ld, ld> one = {1, 0};
+        // DCT/DST-II forward: a = x, out = Re(2 exp(-i pi k/(2N)) S_k)
+        const auto h = cis(-pi * (ld)j / (2 * (ld)N));
+        put(t.A, 0, j, one, chc);
+        put(t.C, 0, j, {2 * h.first, 2 * h.second}, chc);
+        // DCT/DST-III/2 inverse: a = w_j exp(+i pi j/(2N)) X_j (w_0 = 1/2), out = Re(S_n)
+        const ld wj = j == 0 ? 0.5L : 1.0L;
+        put(t.A, 1, j, {wj * h.first, -wj * h.second}, ch);
+        put(t.C, 1, j, one, ch);
+        // DCT/DST-IV: a = exp(-i pi j/(2N)) x_j, out = f exp(-i pi (2k+1)/(4N)) S_k, f = 2 / 1
+        const auto q = cis(-pi * (2 * (ld)j + 1) / (4 * (ld)N));
+        put(t.A, 2, j, h, chc);
+        put(t.C, 2, j, {2 * q.first, 2 * q.second}, chc);
+        put(t.A, 3, j, h, chc);
+        put(t.C, 3, j, q, chc);
+        // real DFT: forward exp(-2 pi i jk/N), inverse exp(+2 pi i jk/N), unnormalised
+        put(t.A, 4, j, one, cfc);
+        put(t.C, 4, j, one, cfc);
+        put(t.A, 5, j, one, cf);
+        put(t.C, 5, j, one, cf);
+    }
+    t.D.assign(2, 0.0);
+}
+
 inline void make_axis_tables(int N, AxisTables &t)
 {
+    if (!line_compiled(N)) {
+        make_bs_tables(N, t);
+        return;
+    }
     const int M = N / 2;
     auto put = [](std::vector<double> &v, int cnt, auto f) {
         v.resize(2 * (size_t)cnt);

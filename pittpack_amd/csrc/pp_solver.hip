@@ -15,6 +15,7 @@
 
 #include "pittpack_hip.h"
 #include "pp_line.h"
+#include "pp_line_bs.h"
 #include "pp_launch.h"
 #include "pp_mixed.h"
 #include "pp_setup.h"
@@ -45,9 +46,29 @@ extern template hipError_t launch_line_kind<KIND_PP>(int, int, int, int, const L
 extern template hipError_t launch_line_kind<KIND_ND>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
 
-// line transforms: one translation unit per kind (pp_line_<kind>.hip)
-hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArgs &a, hipStream_t s)
+extern template hipError_t launch_line_bs_kind<KIND_DD>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_bs_kind<KIND_NN>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_bs_kind<KIND_PP>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_bs_kind<KIND_ND>(int, int, int, const LineArgs &, hipStream_t);
+extern template hipError_t launch_line_bs_kind<KIND_DN>(int, int, int, const LineArgs &, hipStream_t);
+
+// line transforms: one translation unit per kind (pp_line_<kind>.hip for the compiled
+// lengths, pp_line_bs_<kind>.hip for the others)
+hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArgs &a0, hipStream_t s)
 {
+    if (!line_compiled(N)) {
+        LineArgs a = a0;
+        a.n = N;
+        switch (kind) {
+        case KIND_DD: return launch_line_bs_kind<KIND_DD>(form, AX, FWD, a, s);
+        case KIND_NN: return launch_line_bs_kind<KIND_NN>(form, AX, FWD, a, s);
+        case KIND_PP: return launch_line_bs_kind<KIND_PP>(form, AX, FWD, a, s);
+        case KIND_ND: return launch_line_bs_kind<KIND_ND>(form, AX, FWD, a, s);
+        case KIND_DN: return launch_line_bs_kind<KIND_DN>(form, AX, FWD, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    const LineArgs &a = a0;
     switch (kind) {
     case KIND_DD: return launch_line_kind<KIND_DD>(form, N, AX, FWD, a, s);
     case KIND_NN: return launch_line_kind<KIND_NN>(form, N, AX, FWD, a, s);
@@ -123,6 +144,7 @@ struct pp_solver {
     double zsing_eig = -2.0;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors
     DevTables tab64[2];
+    DevTables tabz;  // periodic z without a compiled complex plan: real z-line tables
     // kernel-timing ring: slot s holds up to kMaxMarks events recorded between the kernel
     // launches of one solve
     static constexpr int kMaxMarks = 64;
@@ -229,9 +251,8 @@ void free_mixed(pp_solver *h)
 
 void free_tables(pp_solver *h)
 {
-    for (int a = 0; a < 2; a++) {
-        DevTables &t = h->tab64[a];
-        for (void **p : {&t.tw, &t.A, &t.B, &t.C, &t.D}) {
+    for (DevTables *t : {&h->tab64[0], &h->tab64[1], &h->tabz}) {
+        for (void **p : {&t->tw, &t->A, &t->B, &t->C, &t->D}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
         }
@@ -276,8 +297,17 @@ int build_tables(pp_solver *h)
     h->zsing_x = h->zsing_y = -1;
     if (h->g.zper) {
         AxisTables zt;
-        make_axis_tables(2 * h->g.nz, zt);  // complex FFT of length nz
-        if ((rc = upload((void **)&h->d_ztw, zt.tw))) return rc;
+        if (zperiodic_supported(h->g.nz)) {
+            make_axis_tables(2 * h->g.nz, zt);  // complex FFT of length nz
+            if ((rc = upload((void **)&h->d_ztw, zt.tw))) return rc;
+        } else {
+            // real z-line transforms of length nz (R2HC / HC2R)
+            make_axis_tables(h->g.nz, zt);
+            DevTables &d = h->tabz;
+            if ((rc = upload(&d.tw, zt.tw)) || (rc = upload(&d.A, zt.A)) || (rc = upload(&d.B, zt.B)) ||
+                (rc = upload(&d.C, zt.C)) || (rc = upload(&d.D, zt.D)))
+                return rc;
+        }
         if ((rc = upload((void **)&h->d_zc2, zper_cos2(h->g.nz)))) return rc;
         const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
         const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
@@ -310,9 +340,9 @@ int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
     if (h->g.zper) {
-        // periodic z: FP64 form only, z lengths of the compiled FFTs
+        // periodic z: FP64 form only, any z length from 2 to 1024
         if (h->precision == 1) return PP_UNSUPPORTED_BC;
-        if (!zperiodic_supported(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+        if (!zperiodic_supported(h->g.nz) && !line_size_supported(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     }
     if (h->precision == 1 && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (h->precision == 0 && !h->g.zper && !fp64_z_supported(h->zsolver, h->g.nz)) return PP_UNSUPPORTED_SIZE;
@@ -363,7 +393,8 @@ bool use_ztiles(pp_solver *h)
     // in-place layout)
     if (!h->ztiles || h->zsolver != 0 || h->g.zper) return false;
     if (!zsolve_supported_f64(h->g.nz) && !zsolve_long_supported_f64(h->g.nz)) return false;
-    return zsolve_cpb<double>() % line_block_lines<double>(h->g.ny, 1) == 0;
+    const int ly = line_lines_any<double>(h->g.ny, 1);
+    return ly > 0 && zsolve_cpb<double>() % ly == 0;
 }
 
 // rows per z tile: whole lane rows of the z solve (16 J), zero padding below nz
@@ -443,7 +474,28 @@ int run_fp64(pp_solver *h, const double *rhs, double *field)
             zp.Cy = z.Cy;
             zp.tw = h->d_ztw;
             zp.c2 = h->d_zc2;
-            PP_HIP_CHECK(launch_zperiodic_f64(zp, s));
+            if (zperiodic_supported(h->g.nz)) {
+                PP_HIP_CHECK(launch_zperiodic_f64(zp, s));
+            } else {
+                // z lines as the columns of a [nz][ny*nx] array: R2HC, divide, HC2R
+                LineArgs a{};
+                a.in = field;
+                a.out = field;
+                const DevTables &t = h->tabz;
+                a.tw = (const double *)t.tw;
+                a.A = (const double *)t.A;
+                a.B = (const double *)t.B;
+                a.C = (const double *)t.C;
+                a.D = (const double *)t.D;
+                a.in_scale = 1.0;
+                a.out_scale = 1.0;
+                a.nlines = (long)h->g.nx * h->g.ny;
+                a.ncols = h->g.nx * h->g.ny;
+                a.nplanes = 1;
+                PP_HIP_CHECK(launch_line(FORM_F64, KIND_PP, h->g.nz, 1, 1, a, s));
+                PP_HIP_CHECK(launch_zper_divide(zp, s));
+                PP_HIP_CHECK(launch_line(FORM_F64, KIND_PP, h->g.nz, 1, 0, a, s));
+            }
             if (h->zsing_x >= 0)
                 PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
         } else {
@@ -976,8 +1028,9 @@ int pp_set_box(pp_handle h, const double *X)
     if (!h) return PP_INVALID_HANDLE;
     drop_graph(h);
     if (!X) return PP_INPUT_ARGS;
-    for (int i = 0; i < 6; i++) h->g.X[i] = X[i];
+    // an invalid box leaves the handle's box (and its tables) as they were
     if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
+    for (int i = 0; i < 6; i++) h->g.X[i] = X[i];
     h->box_set = true;
     h->tables_ready = false;
     if (h->coords_set) return pp_set_coords(h, 2);

@@ -11,6 +11,9 @@
 // reference's own branch (shermanMorrisonThomasV1, src/triDiag.cpp:2112-2255: first row
 // replaced by the average of the first and last right-hand sides, pinned Thomas sweeps and
 // the Sherman-Morrison correction), restated serially in FP64 by k_smt_v1.
+//
+// Lengths without a compiled complex plan take three passes instead of one: real z-line
+// transforms of any length (the line kernels) around a division kernel (k_zper_divide).
 #include <hip/hip_runtime.h>
 
 #include "pp_fft.h"
@@ -136,6 +139,33 @@ static hipError_t launch_zper_m(const ZPerArgs &a, hipStream_t s)
 }
 
 bool zperiodic_supported(int nz) { return nz >= 8 && line_threads(nz, 0) > 0; }
+
+// Periodic z of any other length: the field's z lines go through the real line transforms
+// (R2HC forward, HC2R inverse, pp_line.h / pp_line_bs.h, launched by the host on the
+// [nz][SY*SX] view) and this kernel divides half-complex entry p of column (sx, sy) by
+// nz * (eig + 2 cos(2 pi p / nz)) -- modes k and nz - k share the factor, so the real and
+// imaginary parts of a mode divide alike.  The singular column only takes 1 / nz, so the
+// inverse transform restores its right-hand side for the reference's branch (k_smt_v1).
+__global__ void k_zper_divide(ZPerArgs a)
+{
+    const long plane = (long)a.SX * a.SY;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane * a.nz) return;
+    const int p = (int)(i / plane);
+    const long c = i - (long)p * plane;
+    const int sy = (int)(c / a.SX), sx = (int)(c - (long)sy * a.SX);
+    const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sx], a.Cx)), __dmul_rn(a.lamy[sy], a.Cy));
+    const bool singular = fabs(eig + 2.) < 1.e-10;
+    const double den = singular ? (double)a.nz : (eig + a.c2[p]) * (double)a.nz;
+    a.data[i] = a.data[i] / den;
+}
+
+hipError_t launch_zper_divide(const ZPerArgs &a, hipStream_t s)
+{
+    const long n = (long)a.SX * a.SY * a.nz;
+    hipLaunchKernelGGL(k_zper_divide, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_zperiodic_f64(const ZPerArgs &a, hipStream_t s)
 {

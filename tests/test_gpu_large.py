@@ -1,11 +1,13 @@
-"""Full-size workloads checked through size-independent properties.
+"""Full-size workloads.
 
-1024^3 periodic-x/y, Neumann-z mixed-precision solve (FP32 work field, FP64 (0,0) and
-near-singular columns, two FP64-residual refinement steps) on an FP32 Gaussian right-hand
-side with its mean removed: the FP64 discrete residual of the 7-point operator must be at
-the FP64 level.  The FP64 form does not take 1024-row columns, and the oracle at this size
-would take minutes, so the residual is the check.  512^3 FP64 drop-in (the benchmark's
-boundary conditions): residual at the FP64 level and linearity of the solve.
+* 512^3, x DD / y ND / z DD, RHS U[-1,1] FP64 from the counter generator with seed 0 (the
+  benchmark's FP64 workload): FP64 solve against the oracle on the same RHS, max-norm
+  difference <= 1e-12 relative (the oracle takes about 2 s on 16 host threads).
+* 512^3 FP64, Neumann faces: linearity of the solve and the FP64 discrete residual.
+* 1024^3 periodic-x/y, Neumann-z mixed-precision solve (FP32 work field, FP64 (0,0) and
+  near-singular columns, two FP64-residual refinement steps) on an FP32 Gaussian right-hand
+  side with its mean removed: the FP64 discrete residual of the 7-point operator must be at
+  the FP64 level, and the result must agree with the FP64 form's 1024^3 solve.
 """
 import numpy as np
 import pytest
@@ -60,7 +62,47 @@ def test_mixed_1024_cubed_periodic_xy_neumann_z():
     # with the volume mean removed the pinned (0,0) system is consistent, so its solution
     # satisfies every row: the bar is the FP64 rounding level of L u
     assert rel <= 1e-12
-    del f, u
+    u64 = torch.empty_like(f)
+    with pa.PoissonHIP(n, n, n) as M:
+        M.assignBoundary("PPPPNN")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+        M.solve_device_oop(f.data_ptr(), u64.data_ptr())
+        torch.cuda.synchronize()
+    d = float((u - u64).abs().max() / u64.abs().max())
+    print(f"1024^3 mixed vs FP64 form: rel max diff {d:.2e}", flush=True)
+    assert d <= 1e-12
+    del f, u, u64
+    torch.cuda.empty_cache()
+
+
+def test_fp64_512_cubed_ddnddd_vs_oracle():
+    """The benchmark's FP64 workload at full size against the oracle (same RHS)."""
+    import torch
+
+    import oracle
+    from pittpack_amd import rng
+
+    print("fp64 512^3 DDNDDD vs oracle", flush=True)
+    n = 512
+    f = rng.uniform_pm1((n, n, n), seed=0)
+    ref = oracle.solve(f, "DDNDDD", zero_mode=1)
+    dev = torch.device("cuda:0")
+    fd = torch.from_numpy(f).to(dev)
+    ud = torch.empty_like(fd)
+    with pa.PoissonHIP(n, n, n) as M:
+        M.assignBoundary("DDNDDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+        M.solve_device_oop(fd.data_ptr(), ud.data_ptr())
+        torch.cuda.synchronize()
+    u = ud.cpu().numpy()
+    err = float(np.abs(u - ref).max() / np.abs(ref).max())
+    print(f"512^3 DDNDDD rel max diff vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-12
+    del fd, ud
     torch.cuda.empty_cache()
 
 
@@ -75,8 +117,8 @@ def test_fp64_512_cubed_residual_and_linearity():
     f2 = torch.rand((n, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
     u1, u2, u3 = torch.empty_like(f1), torch.empty_like(f1), torch.empty_like(f1)
     with pa.PoissonHIP(n, n, n) as M:
-        # homogeneous Dirichlet terms vanish only with the exact-value tables at zero, so the
-        # linearity check uses Neumann faces; the Dirichlet run checks the solve's residual
+        # the Dirichlet boundary term is an affine part of the solve, so the linearity check
+        # uses Neumann faces (the Dirichlet workload is checked against the oracle above)
         M.assignBoundary("NNNNNN")
         M.setBox([0, 1, 0, 1, 0, 1])
         M.setCoords(2)

@@ -74,7 +74,58 @@ def test_server_rejects_oversized():
         with pytest.raises(pa.PittPackError) as e:
             S.submit((64, 64, 64), "NNNNDD")
         assert e.value.code == pa.ALLOCATION_FAIL
+        # sizes nothing can solve are refused at submission (the request is planned there)
         with pytest.raises(pa.PittPackError) as e:
-            S.submit((16, 30, 32), "NNNNDD", seed=1)
-            S.wait(max(S._pending))
+            S.submit((16, 2000, 32), "NNNNDD", seed=1)
         assert e.value.code == pa.UNSUPPORTED_SIZE
+
+
+def test_server_exact_footprint_padded_shape():
+    """A shape whose z tiles are padded (nx = 20 -> 32 columns, nz = 17 -> 32 rows) takes more
+    than 24 B per point; a pool one byte short of its planned footprint rejects it at
+    submission, and a pool of exactly that footprint runs it (no admission loop that can
+    never place a request)."""
+    print("server exact footprint", flush=True)
+    shape, bc = (17, 32, 20), "NNNNDD"
+    with pa.Server(hbm_bytes=64 << 20, staging_bytes=1 << 20, streams=1) as S:
+        need = S.footprint(shape, bc)
+    pts = 20 * 32 * 17
+    assert need > 24 * pts + 65536  # the padded tiles
+    with pa.Server(hbm_bytes=need - 256, staging_bytes=1 << 20, streams=1) as S:
+        with pytest.raises(pa.PittPackError) as e:
+            S.submit(shape, bc, seed=3)
+        assert e.value.code == pa.ALLOCATION_FAIL
+    f = prng.uniform_pm1(shape, seed=3)
+    with pa.Server(hbm_bytes=need, staging_bytes=1 << 20, streams=1) as S:
+        u, _ = S.wait(S.submit(shape, bc, rhs=f))
+    assert rel_maxdiff(u, oracle.solve(f, bc, zero_mode=1)) <= 1e-12
+
+
+def test_server_large_request_not_starved():
+    """A large request that fits only the empty pool, queued behind small ones in flight and
+    ahead of many more small requests: it may be overtaken by at most PP_OPT_MAX_OVERTAKE admitted
+    batches, so it runs within (limit + streams + 1) launches of its submission instead of
+    after the whole stream of small requests."""
+    print("server aging", flush=True)
+    big, small, bc = (96, 128, 128), (16, 32, 32), "NNNNDD"
+    limit, streams = 2, 2
+    with pa.Server(hbm_bytes=64 << 20, staging_bytes=1 << 20, streams=1) as S:
+        nb, ns = S.footprint(big, bc), S.footprint(small, bc)
+    # the big request fits only an empty pool; small ones fit beside each other
+    with pa.Server(hbm_bytes=nb + ns // 2, staging_bytes=256 << 20, streams=streams) as S:
+        S.set_option(pa.OPT_BATCH_MEMBERS, 2)
+        S.set_option(pa.OPT_MAX_OVERTAKE, limit)
+        head = [S.submit(small, bc, seed=10 + i) for i in range(4)]
+        tb = S.submit(big, bc, seed=99)
+        tail = [S.submit(small, bc, seed=100 + i) for i in range(60)]
+        seqs = [S.wait(t, with_batch=True)[2] for t in head]
+        u, _, seq_big = S.wait(tb, with_batch=True)
+        tail_seqs = [S.wait(t, with_batch=True)[2] for t in tail]
+        st = S.stats()
+    assert st["completed"] == 65
+    before = sum(1 for q in tail_seqs if q < seq_big)
+    print(f"big request ran in batch {seq_big}; {before} of 60 later requests ran before it", flush=True)
+    assert seq_big <= max(seqs) + limit + streams + 1
+    assert before <= 2 * (limit + streams)
+    ref = oracle.solve(prng.uniform_pm1(big, seed=99), bc, zero_mode=1)
+    assert rel_maxdiff(u, ref) <= 1e-12

@@ -11,9 +11,12 @@ FP64 near-singular columns, `--refine` FP64-residual refinement steps): its thro
 ratio to the FP64 form and its error against the FP64 result.
 
 Prints one JSON line: grid points solved per second, the roofline of the dominant kernel
-(HIP events recorded on the solver's stream between its launches, inside the timed region)
-and a CPU baseline (the oracle, i.e. the C restatement of the reference algorithm, on one
-solve of the same 512^3 workload, on the host's cores through OpenMP).
+(HIP events recorded on the solver's stream between its launches, inside the timed region),
+the achievable bandwidth of a streaming copy of the same array, the 1024^3 FP64 and mixed
+runs (the mixed one against the FP64 one), the batched form (a stratified sample of members
+against the oracle), the serving form at several arrival rates and closed loop, and a CPU
+baseline (the oracle, i.e. the C restatement of the reference algorithm, on 10 solves of the
+same 512^3 workload on the host's cores through OpenMP, compared with the GPU solution).
 """
 from __future__ import annotations
 
@@ -46,37 +49,95 @@ def kernel_table(precision: int, refine: int):
     return out
 
 
-def cpu_baseline(n: int = 512, bc: str = BC):
+def host_cores():
+    """(physical cores of the host from lscpu, logical CPUs this process may run on)"""
+    import subprocess
+
+    phys = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        phys = int(kv["Core(s) per socket"].strip()) * int(kv["Socket(s)"].strip())
+    except Exception:
+        pass
+    return phys, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(u_gpu=None, n: int = 512, bc: str = BC, solves: int = 10):
+    """The oracle (C restatement of the reference algorithm) on the same 512^3 workload, timed
+    over `solves` solves into a preallocated workspace (the reference's window averages 10
+    solves, src/pencilDcmp.cpp:5900,6332-6376), with the FFT plan made and destroyed per line
+    as the reference's CPU path does (src/poissonCPU.cpp:52-63); three plan-once solves are
+    timed beside it.  u_gpu: the GPU FP64 solution of the same RHS, compared with the
+    oracle's."""
+    import numpy as np
+
     import oracle
     from pittpack_amd import rng
 
     f = rng.uniform_pm1((n, n, n), seed=0)
+    w = oracle.workspace(f.shape)
+    u = np.zeros_like(f)
+    oracle.set_per_line_plans(True)
     t0 = time.perf_counter()
-    oracle.solve(f, bc)
-    dt = time.perf_counter() - t0
+    for _ in range(solves):
+        oracle.solve(f, bc, zero_mode=1, work=w, out=u)
+    dt = (time.perf_counter() - t0) / solves
+    oracle.set_per_line_plans(False)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        oracle.solve(f, bc, zero_mode=1, work=w, out=u)
+    dt_once = (time.perf_counter() - t0) / 3
     threads = oracle.threads()
-    return {
+    phys, logical = host_cores()
+    out = {
         "value": n ** 3 / dt,
         "unit": "grid points/s",
         "cores": threads,
+        "host_physical_cores": phys,
+        "host_logical_cpus": logical,
         "kind": "port",
-        "sample": f"one FP64 solve at {n}^3, bc {bc}, seed-0 U[-1,1] RHS, oracle (C restatement "
-                  f"of the reference algorithm; lines, planes and z columns split over {threads} "
-                  f"OpenMP threads, as the reference splits them over ranks): {dt:.2f} s",
+        "sample": f"{solves} FP64 solves at {n}^3, bc {bc}, seed-0 U[-1,1] RHS, oracle (C restatement of "
+                  f"the reference algorithm, FFT plan made and destroyed per line as poissonCPU.cpp does; "
+                  f"lines, planes and z columns split over {threads} OpenMP threads, as the reference splits "
+                  f"them over ranks; workspace preallocated): {dt:.2f} s per solve",
+        "plan_once_value": n ** 3 / dt_once,
     }
+    if u_gpu is not None:
+        d = np.abs(u_gpu - u)
+        out["gpu_fp64_rel_max_diff_vs_oracle"] = float(d.max() / np.abs(u).max())
+        out["gpu_fp64_rel_l2_diff_vs_oracle"] = float(np.sqrt((d ** 2).sum() / (u ** 2).sum()))
+    return out
 
 
 XY_COMBOS = ["PPPP", "NNNN", "DDDD", "NNDD"]  # x/y faces of the batched and serving workloads
 Z_COMBOS = ["DD", "NN"]
 XY_SIDES = [32, 48, 64, 96, 128, 192]
-SERVE_SIDES = [32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]  # compiled line lengths in [32, 512]
+
+
+def stratified_members(members, per_class: int = 8):
+    """indices of up to per_class members from each boundary combination, spread over the
+    class's sizes (smallest to largest)"""
+    by_bc = {}
+    for i, m in enumerate(members):
+        by_bc.setdefault(m[3], []).append(i)
+    picks = []
+    for bc in sorted(by_bc):
+        idx = sorted(by_bc[bc], key=lambda i: members[i][0] * members[i][1] * members[i][2])
+        step = max(1, len(idx) // per_class)
+        picks += idx[::step][:per_class]
+    return sorted(picks)
 
 
 def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, autotune: bool = True):
     """4096 independent solves (nx, ny from {32..192}, nz in [16, 256], BC uniform over four
     x/y combos x z in {DD, NN}, RHS U[-1,1] rounded to FP32 from seed 2), mixed form with
-    `refine` steps; the same batch in FP64 gives the error reference and its own rate."""
+    `refine` steps; the same batch in FP64 gives the error reference and its own rate.  A
+    stratified sample (8 members of each of the 8 boundary combinations) of both forms is
+    checked against the oracle."""
     import numpy as np
+
+    import oracle
 
     r = np.random.default_rng(2)
     members = []
@@ -85,9 +146,10 @@ def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, auto
         nz = int(r.integers(16, 257))
         bc = XY_COMBOS[int(r.integers(4))] + Z_COMBOS[int(r.integers(2))]
         members.append((nx, ny, nz, bc))
-    picks = list(range(0, count, max(1, count // sample)))[:sample]
+    picks = stratified_members(members)
     out = {}
     ref = {}
+    oracle_sol = {}
     for prec in (0, 1):
         # FP64: line classes tuned at finalize (before the right-hand sides are written)
         B = pa.Batch(precision=prec, refine=refine if prec else None, autotune=int(autotune and prec == 0))
@@ -115,10 +177,19 @@ def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, auto
         launches = B.launch_count()
         key = "fp64" if prec == 0 else "mixed"
         sols = {i: B.get_solution(i) for i in picks}
+        errs_o = []
+        for i in picks:
+            nx, ny, nz, bc = members[i]
+            if prec == 0:
+                oracle_sol[i] = oracle.solve(B.get_rhs(i), bc, zero_mode=1)
+            o = oracle_sol[i]
+            errs_o.append(float(np.abs(sols[i] - o).max() / np.abs(o).max()))
         if prec == 0:
             ref = sols
         res = {"ms_per_step": ms, "value": pts / (ms * 1e-3), "launches_per_stage_xyz": launches,
                "plan_and_upload_s": plan_s, "hip_graph": True, "kernels_ms": kt}
+        res["oracle_checked_members"] = len(picks)
+        res["rel_max_diff_vs_oracle_max"] = max(errs_o)
         if prec == 0:
             res["alternative_shapes"] = B.tuning()
         if prec == 1:
@@ -133,25 +204,32 @@ def bench_batched(pa, count: int, steps: int, refine: int, sample: int = 8, auto
     return out
 
 
-def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int):
-    """Poisson arrivals at `rate`/s; sides log-uniform in [32, 512] per axis (x and y rounded to
-    the nearest compiled line length, z any integer), BC uniform over four x/y combos x z in
-    {DD, NN}, precision FP64 or mixed 50/50, RHS from the device counter generator; every
-    solution is copied back to the host."""
+def serving_requests(n_req: int, seed: int = 5):
+    """request list: sides log-uniform integers in [32, 512] per axis, BC uniform over four x/y
+    combos x z in {DD, NN}, FP64 or mixed 50/50"""
+    import numpy as np
+
+    r = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_req):
+        side = np.exp(r.uniform(np.log(32), np.log(512), 3))
+        nx, ny, nz = (int(round(v)) for v in side)
+        bc = XY_COMBOS[int(r.integers(4))] + Z_COMBOS[int(r.integers(2))]
+        reqs.append(((nz, ny, nx), bc, int(r.integers(2)), 7000 + i))
+    return reqs
+
+
+def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, reqs=None):
+    """Poisson arrivals at `rate`/s (rate <= 0: closed loop, every request submitted at once);
+    RHS from the device counter generator; every solution is copied back to the host."""
     import threading
 
     import numpy as np
 
-    r = np.random.default_rng(5)
-    reqs = []
-    for i in range(n_req):
-        side = np.exp(r.uniform(np.log(32), np.log(512), 3))
-        nx = int(min(SERVE_SIDES, key=lambda v: abs(np.log(v) - np.log(side[0]))))
-        ny = int(min(SERVE_SIDES, key=lambda v: abs(np.log(v) - np.log(side[1]))))
-        nz = int(round(side[2]))
-        bc = XY_COMBOS[int(r.integers(4))] + Z_COMBOS[int(r.integers(2))]
-        reqs.append(((nz, ny, nx), bc, int(r.integers(2)), 7000 + i))
-    gaps = r.exponential(1.0 / rate, n_req)
+    reqs = reqs or serving_requests(n_req)
+    n_req = len(reqs)
+    r = np.random.default_rng(6)
+    gaps = r.exponential(1.0 / rate, n_req) if rate > 0 else np.zeros(n_req)
     S = pa.Server(hbm_bytes=int(budget_gb * (1 << 30)), staging_bytes=16 << 30, streams=streams)
     tickets = [None] * n_req
     ready = [threading.Event() for _ in range(n_req)]
@@ -180,21 +258,33 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int):
     pts = sum(s[0] * s[1] * s[2] for s, _, _, _ in reqs)
     lat_a = np.array(lat)
     return {
-        "requests": n_req, "rate_per_s": rate, "hbm_budget_gb": budget_gb, "streams": streams,
+        "requests": n_req, "rate_per_s": rate if rate > 0 else "closed loop", "hbm_budget_gb": budget_gb,
+        "streams": streams,
         "p50_ms": float(np.percentile(lat_a, 50)), "p99_ms": float(np.percentile(lat_a, 99)),
         "mean_ms": float(lat_a.mean()), "solves_per_s": n_req / wall, "points_per_s": pts / wall,
-        "batches": st["batches"], "max_concurrent_batches": st["max_concurrent_batches"],
+        "batches": st["batches"], "requests_per_batch": n_req / max(1, st["batches"]),
+        "max_concurrent_batches": st["max_concurrent_batches"],
         "mean_points_per_request": pts / n_req,
-        "workload": ("Poisson arrivals; sides log-uniform in [32,512] (x, y rounded to compiled lengths), "
-                     f"bc x/y in {XY_COMBOS} x z in {Z_COMBOS}, FP64 / mixed 50/50, device-generated RHS, "
-                     "solutions copied to host; latency = submit to solution in pinned host memory"),
     }
 
 
-def bench_large(pa, torch, n: int, precision: int, refine: int, steps: int):
+def bench_serving_sweep(pa, n_req: int, rates, budget_gb: float = 200.0, streams: int = 4):
+    """the serving workload at several arrival rates and closed loop (same request list)"""
+    reqs = serving_requests(n_req)
+    runs = [bench_serving(pa, n_req, rate, budget_gb, streams, reqs) for rate in rates]
+    return {
+        "workload": ("Poisson arrivals; sides log-uniform integers in [32,512] per axis (any length), "
+                     f"bc x/y in {XY_COMBOS} x z in {Z_COMBOS}, FP64 / mixed 50/50, device-generated RHS, "
+                     "solutions copied to host; latency = submit to solution in pinned host memory"),
+        "runs": runs,
+    }
+
+
+def bench_large(pa, torch, n: int, precision: int, refine: int, steps: int, ref=None):
     """FP64 form (autotuned) or mixed form at n^3, periodic x/y, Neumann z, FP32 Gaussian RHS
     (torch generator, seed 1) with the volume mean removed; FP64 discrete residual of the
-    result"""
+    result and, given the FP64 form's solution `ref`, the difference to it.  Returns
+    (record, solution)."""
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev).manual_seed(1)
     f = torch.randn((n, n, n), generator=g, device=dev, dtype=torch.float32).double()
@@ -227,11 +317,17 @@ def bench_large(pa, torch, n: int, precision: int, refine: int, steps: int):
     um = torch.cat([u[:1], u[:-1]], 0)
     r -= up + um - 2.0 * u
     res = float(r.abs().max()) / scale
-    del f, u, r, up, um
-    torch.cuda.empty_cache()
+    del f, r, up, um
     form = f"mixed form, {refine} refinement steps" if precision == 1 else "FP64 form, autotuned"
-    return {"workload": f"{form}, {n}^3, bc PPPPNN, FP32 N(0,1) RHS (torch generator seed 1), mean removed",
-            "ms_per_step": ms, "value": n ** 3 / (ms * 1e-3), "fp64_residual_rel": res}
+    rec = {"workload": f"{form}, {n}^3, bc PPPPNN, FP32 N(0,1) RHS (torch generator seed 1), mean removed",
+           "ms_per_step": ms, "value": n ** 3 / (ms * 1e-3), "fp64_residual_rel": res}
+    if ref is not None:
+        d = u - ref
+        rec["rel_max_diff_vs_fp64"] = float(d.abs().max() / ref.abs().max())
+        rec["rel_l2_diff_vs_fp64"] = float(d.norm() / ref.norm())
+        del d
+    torch.cuda.empty_cache()
+    return rec, u
 
 
 def load_traffic(workload_key: str, kernel: str):
@@ -260,8 +356,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="FP64 form in the fixed default configuration")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel of every solve (no HIP graph)")
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate per second")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests per serving run (0: skip)")
+    ap.add_argument("--serve-rates", type=str, default="100,1000,3000,0",
+                    help="Poisson arrival rates per second of the serving sweep (0: closed loop)")
     ap.add_argument("--large", type=int, default=1024, help="side of the large FP64 and mixed runs (0: skip)")
     args = ap.parse_args()
 
@@ -365,16 +462,27 @@ def main():
         }
         del diff
         M.close()
+    # host copy of the FP64 solution for the comparison with the oracle (CPU baseline leg)
+    u64_host = u64.cpu().numpy() if not args.no_cpu_baseline else None
     del u64
 
-    # a plain elementwise copy of the same array (8 B read + 8 B written per point)
-    torch.neg(f, out=u)
-    torch.cuda.synchronize()
-    c0 = time.perf_counter()
-    for _ in range(5):
-        torch.neg(f, out=u)
-    torch.cuda.synchronize()
-    copy_gbs = 5 * 2 * pts * 8 / (time.perf_counter() - c0) / 1e9
+    # achievable bandwidth on the same array (8 B read + 8 B written per point): torch's
+    # elementwise kernel and the library's streaming copy (cached / nontemporal), HIP events
+    # on the stream the copies run on
+    def copy_rate(fn, reps=20):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return reps * 2 * pts * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+    copy_gbs = copy_rate(lambda: torch.neg(f, out=u))
+    stream_copy = {m: copy_rate(lambda: pa.copy_device(f.data_ptr(), u.data_ptr(), int(pts), mode,
+                                                        stream.cuda_stream))
+                   for m, mode in (("cached", 0), ("nontemporal", 1))}
     M = None
 
     out = {
@@ -403,6 +511,7 @@ def main():
         "kernels_ms": kms,
         "kernels_gbs": gbs,
         "copy_gbs": copy_gbs,
+        "stream_copy_gbs": stream_copy,
     }
     if mixed is not None:
         out["mixed"] = mixed
@@ -410,15 +519,18 @@ def main():
     torch.cuda.empty_cache()
     if args.large > 0:
         ls = max(1, min(args.steps, 5))
-        out["large_fp64"] = bench_large(pa, torch, args.large, 0, args.refine, ls)
-        out["large_mixed"] = bench_large(pa, torch, args.large, 1, args.refine, ls)
+        out["large_fp64"], u_large = bench_large(pa, torch, args.large, 0, args.refine, ls)
+        out["large_mixed"], u_mixed = bench_large(pa, torch, args.large, 1, args.refine, ls, ref=u_large)
+        del u_large, u_mixed
+        torch.cuda.empty_cache()
     if args.batch_count > 0:
         out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine,
                                        autotune=not args.no_autotune)
     if args.serve_requests > 0:
-        out["serving"] = bench_serving(pa, args.serve_requests, args.serve_rate, 200.0, 4)
+        rates = [float(v) for v in args.serve_rates.split(",") if v.strip()]
+        out["serving"] = bench_serving_sweep(pa, args.serve_requests, rates)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(u64_host, n=n)
     print(json.dumps(out), flush=True)
 
 

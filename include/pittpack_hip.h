@@ -180,6 +180,11 @@ int pp_size_supported(int n);
 /* GPU self test of the 16-lane rotation the z solve relies on; 0 when it behaves as assumed */
 int pp_selftest(void);
 
+/* No reference counterpart: streaming copy of n doubles between 16-byte aligned device
+ * arrays on a HIP stream (mode 0 cached, 1 nontemporal), asynchronous.  The yardstick of the
+ * achievable HBM bandwidth (16 bytes per element, as one FP64 stage per grid point). */
+int pp_copy_device(const double *d_in, double *d_out, long long n, int mode, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Batched form (no reference counterpart; the reference solves one grid per PencilDcmp
  * object, src/pencilDcmp.cpp:5801-6430).  Many independent solves of different sizes,

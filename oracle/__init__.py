@@ -43,6 +43,10 @@ def lib():
         L.ppo_zsolve_line.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_char, ctypes.c_char, ctypes.c_int, dp]
         L.ppo_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, dp, dp, ctypes.c_int]
         L.ppo_solve.restype = ctypes.c_int
+        L.ppo_solve_ws.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, dp, dp, ctypes.c_int,
+                                   dp]
+        L.ppo_solve_ws.restype = ctypes.c_int
+        L.ppo_set_per_line_plans.argtypes = [ctypes.c_int]
         L.ppo_effective_rhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, dp, dp]
         L.ppo_effective_rhs.restype = ctypes.c_int
         L.ppo_apply_operator.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, dp, dp]
@@ -90,16 +94,38 @@ def _box(box):
     return np.ascontiguousarray(np.asarray(box if box is not None else [0, 1, 0, 1, 0, 1], dtype=np.float64))
 
 
-def solve(rhs: np.ndarray, bc: str, box=None, zero_mode: int = 0) -> np.ndarray:
-    """Whole FP64 solve.  rhs has shape (nz, ny, nx) (x fastest), unscaled."""
+def solve(rhs: np.ndarray, bc: str, box=None, zero_mode: int = 0, work: np.ndarray | None = None,
+          out: np.ndarray | None = None) -> np.ndarray:
+    """Whole FP64 solve.  rhs has shape (nz, ny, nx) (x fastest), unscaled.  work: optional
+    workspace from `workspace(shape)` (repeated solves then allocate nothing); out: optional
+    solution array."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     X = _box(box)
-    u = np.empty_like(f)
-    rc = lib().ppo_solve(nx, ny, nz, bc.encode(), _dp(X), _dp(f), _dp(u), int(zero_mode))
+    u = out if out is not None else np.empty_like(f)
+    if work is not None:
+        if work.size < 2 * f.size or work.dtype != np.float64 or not work.flags.c_contiguous:
+            raise ValueError("workspace too small")
+        rc = lib().ppo_solve_ws(nx, ny, nz, bc.encode(), _dp(X), _dp(f), _dp(u), int(zero_mode), _dp(work))
+    else:
+        rc = lib().ppo_solve(nx, ny, nz, bc.encode(), _dp(X), _dp(f), _dp(u), int(zero_mode))
     if rc != 0:
         raise ValueError(f"oracle solve failed with code {rc} for bc={bc}")
     return u
+
+
+def workspace(shape) -> np.ndarray:
+    """Workspace of one solve of this (nz, ny, nx) shape: the reference's complex-interleaved
+    field (2 doubles per point), touched once here."""
+    w = np.empty(2 * int(np.prod(shape)), dtype=np.float64)
+    w[:] = 0.0
+    return w
+
+
+def set_per_line_plans(on: bool) -> None:
+    """Make and destroy the FFT plan of every line, as the reference's CPU path does with
+    FFTW (src/poissonCPU.cpp:52-63); off (default): one plan per line set.  Same results."""
+    lib().ppo_set_per_line_plans(int(bool(on)))
 
 
 def effective_rhs(rhs: np.ndarray, bc: str, box=None) -> np.ndarray:

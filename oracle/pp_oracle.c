@@ -623,15 +623,49 @@ static void ppo_dirichlet(const ppo_setup *S, double *P)
 #undef PIDX
 }
 
-/* Transform every contiguous line of P (nlines lines of length size) forward or inverse,
- * following pre/postprocessSignalAccordingly[Reverse] (4464-4618) around the FFT hooks
+/* One line L of length `size`, forward or inverse, following
+ * pre/postprocessSignalAccordingly[Reverse] (4464-4618) around the FFT hooks
  * (poissonCPU.cpp:40-234).  inverse = 0: pre, FFT FORWARD, post; inverse = 1: pre-rev,
  * FFT BACKWARD, post-rev.  Quarter-wave kinds (3,4) apply ext_dct4 both ways. */
+static void ppo_line_one(double *L, int size, int kind, int inverse, const ppo_fft_plan *pl,
+                         const ppo_fft_plan *pl2, double *W, double *Z)
+{
+    if (kind == 3 || kind == 4) {
+        double f = inverse ? 1.0 : 2.0;
+        if (kind == 4) ext_reverse(L, size);
+        ext_dct4(L, size, f, pl2, Z, W);
+        if (kind == 4) ext_altsign(L, size);
+        return;
+    }
+    if (!inverse) {
+        if (kind == 0) { sp_swap(L, size); sp_pre_dct10(L, size); }
+        else if (kind == 1) sp_pre_dct10(L, size);
+        ppo_fft_exec(pl, L, W, -1);
+        if (kind == 0) { sp_post_dct10(L, size); sp_swap(L, size); }
+        else if (kind == 1) sp_post_dct10(L, size);
+    } else {
+        if (kind == 0) { sp_swap(L, size); sp_pre_dct01(L, size); }
+        else if (kind == 1) sp_pre_dct01(L, size);
+        ppo_fft_exec(pl, L, W, +1);
+        if (kind == 0) { sp_post_dct01(L, size); sp_swap(L, size); }
+        else if (kind == 1) sp_post_dct01(L, size);
+    }
+}
+
+/* Plans made once per line set (0, default) or, as the reference's CPU path does, made and
+ * destroyed for every line (1: fftw_plan_dft_1d ... fftw_destroy_plan + fftw_cleanup per
+ * line, src/poissonCPU.cpp:52-63,83-94,158-170,200-225).  Results are identical either way;
+ * only the time differs. */
+static int g_per_line_plans = 0;
+void ppo_set_per_line_plans(int on) { g_per_line_plans = on ? 1 : 0; }
+
+/* Transform every contiguous line of P (nlines lines of length size) */
 static void ppo_lines(double *P, long nlines, int size, int kind, int inverse)
 {
     ppo_fft_plan pl, pl2;
     ppo_fft_plan_init(&pl, size);
     ppo_fft_plan_init(&pl2, 2 * size);
+    const int per_line = g_per_line_plans;
     /* lines are independent: OpenMP threads split them, each with its own work buffers (the
      * plans are read-only), so the result is bitwise the serial one */
 #pragma omp parallel
@@ -641,25 +675,15 @@ static void ppo_lines(double *P, long nlines, int size, int kind, int inverse)
 #pragma omp for schedule(static)
     for (long l = 0; l < nlines; l++) {
         double *L = P + 2 * (size_t)size * l;
-        if (kind == 3 || kind == 4) {
-            double f = inverse ? 1.0 : 2.0;
-            if (kind == 4) ext_reverse(L, size);
-            ext_dct4(L, size, f, &pl2, Z, W);
-            if (kind == 4) ext_altsign(L, size);
-            continue;
-        }
-        if (!inverse) {
-            if (kind == 0) { sp_swap(L, size); sp_pre_dct10(L, size); }
-            else if (kind == 1) sp_pre_dct10(L, size);
-            ppo_fft_exec(&pl, L, W, -1);
-            if (kind == 0) { sp_post_dct10(L, size); sp_swap(L, size); }
-            else if (kind == 1) sp_post_dct10(L, size);
+        if (per_line) {
+            ppo_fft_plan lp, lp2;
+            ppo_fft_plan_init(&lp, size);
+            ppo_fft_plan_init(&lp2, 2 * size);
+            ppo_line_one(L, size, kind, inverse, &lp, &lp2, W, Z);
+            ppo_fft_plan_free(&lp);
+            ppo_fft_plan_free(&lp2);
         } else {
-            if (kind == 0) { sp_swap(L, size); sp_pre_dct01(L, size); }
-            else if (kind == 1) sp_pre_dct01(L, size);
-            ppo_fft_exec(&pl, L, W, +1);
-            if (kind == 0) { sp_post_dct01(L, size); sp_swap(L, size); }
-            else if (kind == 1) sp_post_dct01(L, size);
+            ppo_line_one(L, size, kind, inverse, &pl, &pl2, W, Z);
         }
     }
     free(W);
@@ -698,7 +722,22 @@ static void ppo_x2y(double *P, double *R0, int nx, int ny, int nz, int inverse)
  * after pittPack).  zero_mode as in td_thomas_lowmem.  Returns 0 or a PittPack-style code
  * (25 = unsupported boundary combination).
  */
+int ppo_solve_ws(int nx, int ny, int nz, const char *bc, const double *X, const double *rhs, double *u, int zero_mode,
+                 double *P);
+
 int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const double *rhs, double *u, int zero_mode)
+{
+    double *P = (double *)malloc(sizeof(double) * 2 * (size_t)nx * ny * nz);
+    if (!P) return 15;
+    const int rc = ppo_solve_ws(nx, ny, nz, bc, X, rhs, u, zero_mode, P);
+    free(P);
+    return rc;
+}
+
+/* the same with the caller's workspace P of 2*nx*ny*nz doubles (every entry is written
+ * before it is read), so repeated solves time no allocation */
+int ppo_solve_ws(int nx, int ny, int nz, const char *bc, const double *X, const double *rhs, double *u, int zero_mode,
+                 double *P)
 {
     ppo_setup S;
     int rc = ppo_setup_init(&S, nx, ny, nz, bc, X);
@@ -707,9 +746,7 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
      * x followed by a non-periodic y is not representable in its algorithm */
     if (S.kind[0] == 2 && S.kind[1] != 2) return 25;
     const size_t N = (size_t)nx * ny * nz;
-    double *P = (double *)calloc(2 * N, sizeof(double));
-    double *R = (double *)malloc(sizeof(double) * 2 * (size_t)nx * ny);
-    if (!P || !R) { free(P); free(R); return 15; }
+    double *R = NULL;
     /* assignRhs (5591-5647): P = rhs * c3 * c3, Im = 0 */
     double c3 = S.dxyz[2];
 #pragma omp parallel for schedule(static)
@@ -761,8 +798,6 @@ int ppo_solve(int nx, int ny, int nz, const char *bc, const double *X, const dou
         P[2 * q + 1] = 1. / (S.scale) * P[2 * q + 1];
     }
     for (size_t q = 0; q < N; q++) u[q] = P[2 * q];
-    free(P);
-    free(R);
     return 0;
 }
 

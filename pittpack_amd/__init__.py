@@ -55,7 +55,7 @@ EXPORTED = (
     "pp_batch_fill_uniform", "pp_batch_set_stream", "pp_batch_set_option", "pp_batch_solve",
     "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
     "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
-    "pp_server_stats", "pp_batch_tuning", "pp_server_wait_ex", "pp_server_footprint",
+    "pp_server_stats", "pp_batch_tuning", "pp_server_wait_ex", "pp_server_footprint", "pp_copy_device",
 )
 
 
@@ -138,6 +138,7 @@ def load_library(path: str | None = None):
     L.pp_server_footprint.argtypes = [vp, i, i, i, ctypes.c_char_p, dp, i, ctypes.POINTER(ll)]
     L.pp_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp, ip]
     L.pp_batch_set_option_double.argtypes = [vp, i, ctypes.c_double]
+    L.pp_copy_device.argtypes = [vp, vp, ctypes.c_longlong, i, vp]
     L.pp_batch_synchronize.argtypes = [vp]
     for name in EXPORTED:
         if name != "pp_get_error_enum":
@@ -162,6 +163,11 @@ def _dp(a: np.ndarray):
 
 def size_supported(n: int) -> bool:
     return bool(load_library().pp_size_supported(int(n)))
+
+
+def copy_device(d_in: int, d_out: int, n: int, mode: int = 0, stream: int = 0) -> None:
+    """streaming device copy of n doubles (the achievable-bandwidth yardstick)"""
+    _check(load_library().pp_copy_device(d_in, d_out, int(n), int(mode), stream or None), "copy_device")
 
 
 def selftest() -> int:

@@ -13,7 +13,7 @@
 // a circular convolution of length L >= 2N - 1 (L a power of two) done as forward FFT,
 // product with the precomputed FFT of the chirp (divided by L), inverse FFT -- both through
 // the Stockham engine of pp_fft.h.  The host folds the pre-/post-twiddles and the chirp into
-// FP64 tables per family and direction (P_j, B_j, Q_k; pp_setup.h make_bs_tables); the
+// FP64 tables per family and direction (P_j, B_j, Q_k; pp_setup.h fill_bs_tables); the
 // families are DCT/DST-II/III (NN, DD), DCT/DST-IV (ND, DN) and the real DFT (PP):
 //
 //   kind / direction      input a_j                     output (position)
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(bs_threads(L) * LINES) k_line_bs(LineArgs a0)
     TOut *out = reinterpret_cast<TOut *>(a.out);
     const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
 
-    // table sections of this family and direction (host layout, pp_setup.h make_bs_tables:
+    // table sections of this family and direction (host layout, pp_setup.h fill_bs_tables:
     // P and Q per family {II/III, IV, PP} x {forward, inverse}, N complex each; B per chirp
     // {half-angle, full} x {forward, inverse}, L complex each)
     constexpr int FAM = (KIND == KIND_NN || KIND == KIND_DD) ? 0 : (KIND == KIND_PP ? 2 : 1);

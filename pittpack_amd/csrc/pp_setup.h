@@ -39,13 +39,28 @@ inline int axis_kind(char lo, char hi)
     return -1;
 }
 
-// per-axis twiddle tables in FP64 (host) for a real line of length N = 2M
+// per-axis twiddle tables in FP64 (host) for a real line of length N
 struct AxisTables {
     std::vector<double> tw, A, B, C, D;  // interleaved complex
 };
 
+// sizes (doubles) of the five tables of line length N: tw, A, B, C, D
+struct AxisTableSizes {
+    size_t n[5];
+};
+
+inline AxisTableSizes axis_table_sizes(int N)
+{
+    if (!line_compiled(N)) {
+        const size_t L = (size_t)bs_len(N);
+        return {{2 * L, 12 * (size_t)N, 8 * L, 12 * (size_t)N, 2}};
+    }
+    const size_t M = (size_t)N / 2;
+    return {{2 * M, 2 * (M + 1), 2 * (M + 1), 2 * M, 2 * M}};
+}
+
 // Tables of the arbitrary-length transforms (pp_line_bs.h) for line length N, convolution
-// length L = bs_len(N):
+// length L = bs_len(N), written to caller arrays of axis_table_sizes(N):
 //   tw = exp(-2 pi i e / L), e < L (Stockham twiddles);
 //   A  = P tables, C = Q tables: family {DCT/DST-II/III, DCT/DST-IV, real DFT} x {forward,
 //        inverse}, N complex each (pre- and post-twiddles with the chirp folded in);
@@ -53,17 +68,16 @@ struct AxisTables {
 //        {forward, inverse}, L complex each.
 // Angles of chirp values are reduced exactly in integers (m^2 mod 4N or 2N) and evaluated in
 // long double; the kernel spectra come from a long-double radix-2 FFT.
-inline void make_bs_tables(int N, AxisTables &t)
+inline void fill_bs_tables(int N, double *tw, double *A, double *B, double *C, double *D)
 {
     using ld = long double;
     const ld pi = 3.141592653589793238462643383279502884L;
     const int L = bs_len(N);
     auto cis = [](ld a) { return std::pair<ld, ld>(std::cos(a), std::sin(a)); };
-    t.tw.resize(2 * (size_t)L);
     for (int e = 0; e < L; e++) {
         const auto w = cis(-2 * pi * (ld)e / (ld)L);
-        t.tw[2 * e] = (double)w.first;
-        t.tw[2 * e + 1] = (double)w.second;
+        tw[2 * e] = (double)w.first;
+        tw[2 * e + 1] = (double)w.second;
     }
     // chirp c_m = exp(i pi beta m^2 / 2): half-angle exp(i pi m^2 / (2N)), full exp(i pi m^2 / N)
     auto chirp = [&](int m, bool full) {
@@ -73,10 +87,10 @@ inline void make_bs_tables(int N, AxisTables &t)
     };
     // kernel spectra: b_j = c_j (j < N), b_{L-j} = c_j (0 < j < N), zero elsewhere; inverse
     // direction uses conj(c)
-    t.B.assign(2 * (size_t)L * 4, 0.0);
+    static thread_local ld re[2 * kBsMaxN], im[2 * kBsMaxN];
     for (int sec = 0; sec < 4; sec++) {
         const bool full = sec >= 2, inv = sec & 1;
-        std::vector<ld> re(L, 0), im(L, 0);
+        for (int j = 0; j < L; j++) re[j] = im[j] = 0;
         for (int j = 0; j < N; j++) {
             const auto c = chirp(j, full);
             const ld ci = inv ? -c.second : c.second;
@@ -109,15 +123,13 @@ inline void make_bs_tables(int N, AxisTables &t)
                     im[i + k] += xi;
                 }
         for (int j = 0; j < L; j++) {
-            t.B[2 * ((size_t)sec * L + j)] = (double)(re[j] / (ld)L);
-            t.B[2 * ((size_t)sec * L + j) + 1] = (double)(im[j] / (ld)L);
+            B[2 * ((size_t)sec * L + j)] = (double)(re[j] / (ld)L);
+            B[2 * ((size_t)sec * L + j) + 1] = (double)(im[j] / (ld)L);
         }
     }
     // pre (P) and post (Q) twiddles; forward: a_j = x_j pre_j conj(c_j), out from
     // post_k conj(c_k) S'_k; inverse: pre_j c_j and post_k c_k
-    t.A.assign(2 * (size_t)N * 6, 0.0);
-    t.C.assign(2 * (size_t)N * 6, 0.0);
-    auto put = [&](std::vector<double> &v, int sec, int j, std::pair<ld, ld> a, std::pair<ld, ld> b) {
+    auto put = [&](double *v, int sec, int j, std::pair<ld, ld> a, std::pair<ld, ld> b) {
         v[2 * ((size_t)sec * N + j)] = (double)(a.first * b.first - a.second * b.second);
         v[2 * ((size_t)sec * N + j) + 1] = (double)(a.first * b.second + a.second * b.first);
     };
@@ -127,47 +139,55 @@ inline void make_bs_tables(int N, AxisTables &t)
         const std::pair<ld, ld> one = {1, 0};
         // DCT/DST-II forward: a = x, out = Re(2 exp(-i pi k/(2N)) S_k)
         const auto h = cis(-pi * (ld)j / (2 * (ld)N));
-        put(t.A, 0, j, one, chc);
-        put(t.C, 0, j, {2 * h.first, 2 * h.second}, chc);
+        put(A, 0, j, one, chc);
+        put(C, 0, j, {2 * h.first, 2 * h.second}, chc);
         // DCT/DST-III/2 inverse: a = w_j exp(+i pi j/(2N)) X_j (w_0 = 1/2), out = Re(S_n)
         const ld wj = j == 0 ? 0.5L : 1.0L;
-        put(t.A, 1, j, {wj * h.first, -wj * h.second}, ch);
-        put(t.C, 1, j, one, ch);
+        put(A, 1, j, {wj * h.first, -wj * h.second}, ch);
+        put(C, 1, j, one, ch);
         // DCT/DST-IV: a = exp(-i pi j/(2N)) x_j, out = f exp(-i pi (2k+1)/(4N)) S_k, f = 2 / 1
         const auto q = cis(-pi * (2 * (ld)j + 1) / (4 * (ld)N));
-        put(t.A, 2, j, h, chc);
-        put(t.C, 2, j, {2 * q.first, 2 * q.second}, chc);
-        put(t.A, 3, j, h, chc);
-        put(t.C, 3, j, q, chc);
+        put(A, 2, j, h, chc);
+        put(C, 2, j, {2 * q.first, 2 * q.second}, chc);
+        put(A, 3, j, h, chc);
+        put(C, 3, j, q, chc);
         // real DFT: forward exp(-2 pi i jk/N), inverse exp(+2 pi i jk/N), unnormalised
-        put(t.A, 4, j, one, cfc);
-        put(t.C, 4, j, one, cfc);
-        put(t.A, 5, j, one, cf);
-        put(t.C, 5, j, one, cf);
+        put(A, 4, j, one, cfc);
+        put(C, 4, j, one, cfc);
+        put(A, 5, j, one, cf);
+        put(C, 5, j, one, cf);
     }
-    t.D.assign(2, 0.0);
+    D[0] = D[1] = 0.0;
 }
 
-inline void make_axis_tables(int N, AxisTables &t)
+// the five tables of line length N into caller arrays of axis_table_sizes(N)
+inline void fill_axis_tables(int N, double *tw, double *A, double *B, double *C, double *D)
 {
     if (!line_compiled(N)) {
-        make_bs_tables(N, t);
+        fill_bs_tables(N, tw, A, B, C, D);
         return;
     }
     const int M = N / 2;
-    auto put = [](std::vector<double> &v, int cnt, auto f) {
-        v.resize(2 * (size_t)cnt);
+    auto put = [](double *v, int cnt, auto f) {
         for (int k = 0; k < cnt; k++) {
             double ang = f(k);
             v[2 * k] = std::cos(ang);
             v[2 * k + 1] = std::sin(ang);
         }
     };
-    put(t.tw, M, [&](int e) { return -2.0 * kPi * (double)e / (double)M; });
-    put(t.A, M + 1, [&](int k) { return -2.0 * kPi * (double)k / (double)N; });
-    put(t.B, M + 1, [&](int k) { return -kPi * (double)k / (2.0 * N); });
-    put(t.C, M, [&](int n) { return -kPi * (4.0 * n + 1.0) / (4.0 * N); });
-    put(t.D, M, [&](int k) { return -kPi * (double)k / (double)N; });
+    put(tw, M, [&](int e) { return -2.0 * kPi * (double)e / (double)M; });
+    put(A, M + 1, [&](int k) { return -2.0 * kPi * (double)k / (double)N; });
+    put(B, M + 1, [&](int k) { return -kPi * (double)k / (2.0 * N); });
+    put(C, M, [&](int n) { return -kPi * (4.0 * n + 1.0) / (4.0 * N); });
+    put(D, M, [&](int k) { return -kPi * (double)k / (double)N; });
+}
+
+inline void make_axis_tables(int N, AxisTables &t)
+{
+    const AxisTableSizes z = axis_table_sizes(N);
+    std::vector<double> *v[5] = {&t.tw, &t.A, &t.B, &t.C, &t.D};
+    for (int i = 0; i < 5; i++) v[i]->assign(z.n[i], 0.0);
+    fill_axis_tables(N, t.tw.data(), t.A.data(), t.B.data(), t.C.data(), t.D.data());
 }
 
 // everything one solve's kernels need from (sizes, bc, box)
@@ -245,24 +265,30 @@ inline void setup_coords(SolveSetup &g)
 }
 
 // eigenvalue factors -2 + 2 cos((j + num) * freq / (n + denum)) (src/pencilDcmp.cpp:4410-4413)
-inline std::vector<double> eig_factors(const SolveSetup &g, int axis)
+inline void fill_eig_factors(const SolveSetup &g, int axis, double *l)
 {
     const int n = axis == 0 ? g.nx : g.ny;
-    std::vector<double> l(n);
     for (int j = 0; j < n; j++) l[j] = (-2.0 + 2. * std::cos((j + g.num[axis]) * g.freqs[axis] / (n + g.denum[axis])));
+}
+
+inline std::vector<double> eig_factors(const SolveSetup &g, int axis)
+{
+    std::vector<double> l(axis == 0 ? g.nx : g.ny);
+    fill_eig_factors(g, axis, l.data());
     return l;
 }
 
 // Dirichlet term tables (modifyRhsDirichlet, src/pencilDcmp.cpp:4623-4835, SHIFT = 1):
 // [ex(nx) ey(ny) ez(nz) x-faces(2) y-faces(2) z-faces(2)]
-inline std::vector<double> dirichlet_table(const SolveSetup &g)
+inline size_t dirichlet_table_size(const SolveSetup &g) { return (size_t)g.nx + g.ny + g.nz + 6; }
+
+inline void fill_dirichlet_table(const SolveSetup &g, double *dt)
 {
     const double pi = 4. * std::atan(1.0);
     const double omega[3] = {1 * pi, 2 * pi, 3 * pi};
     const double Xa = g.X[0], Ya = g.X[2], Za = g.X[4];
     const double c1 = g.dxyz[0], c2 = g.dxyz[1], c3 = g.dxyz[2];
     const double shift = 1.0;
-    std::vector<double> dt((size_t)g.nx + g.ny + g.nz + 6);
     size_t o = 0;
     for (int i = 0; i < g.nx; i++) dt[o++] = exact_value(omega[0], Xa + i * c1 + shift * c1 * .5, g.dtag[0]);
     for (int j = 0; j < g.ny; j++) dt[o++] = exact_value(omega[1], Ya + j * c2 + shift * c2 * 0.5, g.dtag[1]);
@@ -273,14 +299,25 @@ inline std::vector<double> dirichlet_table(const SolveSetup &g)
     dt[o++] = exact_value(omega[1], Ya + ((g.ny - 1) + 1) * c2, g.dtag[1]);
     dt[o++] = exact_value(omega[2], Za, g.dtag[2]);
     dt[o++] = exact_value(omega[2], Za + (g.nz - 1) * c3 + c3, g.dtag[2]);
+}
+
+inline std::vector<double> dirichlet_table(const SolveSetup &g)
+{
+    std::vector<double> dt(dirichlet_table_size(g));
+    fill_dirichlet_table(g, dt.data());
     return dt;
 }
 
 // periodic z: 2 cos(2 pi k / nz), k < nz (eigenvalues of the circulant z operator minus eig)
+inline void fill_zper_cos2(int nz, double *c)
+{
+    for (int k = 0; k < nz; k++) c[k] = 2.0 * std::cos(2.0 * kPi * (double)k / (double)nz);
+}
+
 inline std::vector<double> zper_cos2(int nz)
 {
     std::vector<double> c(nz);
-    for (int k = 0; k < nz; k++) c[k] = 2.0 * std::cos(2.0 * kPi * (double)k / (double)nz);
+    fill_zper_cos2(nz, c.data());
     return c;
 }
 

@@ -33,7 +33,14 @@ enum {
     PP_INVALID_HANDLE = 27,
     PP_CALL_ORDER = 28,         /* e.g. pittPack before assignBoundary / setCoords / RHS */
     PP_NO_DEVICE = 29,          /* no HIP device visible */
-    PP_IO_FAIL = 30             /* a solution file could not be written */
+    PP_IO_FAIL = 30,            /* a solution file could not be written */
+    PP_NO_HDF5 = 31             /* HDF5 output asked for, but no HDF5 1.10+ library loads */
+};
+
+/* solution file formats (PP_OPT_IO_FORMAT, pp_write_solution) */
+enum {
+    PP_IO_HDF5 = 0, /* the reference's layout: Pxdmf3d<app>.h5 (X, Y, Z, Q) + XDMF index */
+    PP_IO_RAW = 1   /* Pxdmf3d<app>.bin (raw little-endian FP64) + XDMF 3DCoRectMesh index */
 };
 
 /* option keys for pp_set_option */
@@ -90,7 +97,9 @@ enum {
     /* server: how many admitted batches may overtake a queued request that does not fit
      * before nothing behind it is admitted (default 4); bounds the wait of large requests
      * under a stream of small ones */
-    PP_OPT_MAX_OVERTAKE = 13
+    PP_OPT_MAX_OVERTAKE = 13,
+    /* pp_io file format: PP_IO_HDF5 (default) or PP_IO_RAW */
+    PP_OPT_IO_FORMAT = 14
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).
@@ -122,6 +131,29 @@ int pp_assign_rhs_device(pp_handle h, const double *d_rhs);
  * Blocks until the solve has finished on the handle's stream. */
 int pp_pittpack(pp_handle h);
 
+/* PencilDcmp::initializeTrigonometric (src/pencilDcmp.cpp:1871-2035, with the reference's
+ * SHIFT = 1, EXACT = 0, COEFF = 1, 2, 3): set the handle's right-hand side to the reference's
+ * manufactured field on the current box and boundary conditions (after pp_set_coords), in
+ * place of pp_assign_rhs.  That field is complex in the reference; part 0 sets its real part,
+ * part 1 its imaginary part (two independent real problems).  Cases, as the reference picks
+ * them: x, y and z periodic: u = cos / sin(2 pi (x+y+z)); else x or y periodic:
+ * u = sin(2 pi z) cos / sin(2 pi (x+y)); else u = prod_d exactValue(omega_d, x_d, tag_d) with
+ * omega = (1, 2, 3) pi (imaginary part 0); f = laplacian(u).  Synchronises the stream. */
+int pp_init_trigonometric(pp_handle h, int part);
+
+/* PencilDcmp::getError (src/pencilDcmp.cpp:2040-2208): *err = RMS over the grid of the
+ * handle's field minus the exact solution of the manufactured case -- after
+ * pp_init_trigonometric(part), that part's exact solution; otherwise (right-hand side from
+ * pp_assign_rhs, the reference's INITANALYTIC = 0 path) cos(2 pi z), the solution of
+ * pp_fill_trigonometric's right-hand side.  Unlike the reference it leaves the field intact.
+ * Synchronises the handle's stream. */
+int pp_get_error(pp_handle h, double *err);
+
+/* PencilDcmp::fillTrigonometric (src/pencilDcmp.cpp:5685-5765): write the right-hand side
+ * -4 pi^2 cos(2 pi z) at the cell centres of the current box into the host array rhs
+ * (nx*ny*nz doubles, x fastest), for pp_assign_rhs. */
+int pp_fill_trigonometric(pp_handle h, double *rhs);
+
 /* No reference counterpart (P is protected): copy the solution to a host array. */
 int pp_get_solution(pp_handle h, double *u);
 
@@ -129,12 +161,12 @@ int pp_get_solution(pp_handle h, double *u);
 int pp_get_solution_device(pp_handle h, double *d_u);
 
 /* PencilDcmp::IO(int app, int dir, int aligndir) (src/pencilDcmp.cpp:1827-1832, writing
- * through Phdf5::writeMultiBlockCellCenter, src/phdf5.cpp:1299-1800): write the handle's field
- * as <folder>/Pxdmf3d<app>.bin (raw little-endian FP64, nz*ny*nx, x fastest) and an XDMF 2.0
- * index <folder>/Pxdmf3d<app>.xmf (3DCoRectMesh over the (nx+1)(ny+1)(nz+1) cell corners,
- * attribute "Q", Center="Cell").  The folder must exist (the reference writes into soln/).
- * One device holds the whole grid, so the reference's chunk arguments dir / aligndir have no
- * counterpart.  Synchronises the handle's stream.  PP_IO_FAIL when a file cannot be written. */
+ * through Phdf5::writeMultiBlockCellCenter and xdmfMultiBlockCellCenter,
+ * src/phdf5.cpp:1299-1808,1809-2237): write the handle's field into <folder> in the format
+ * PP_OPT_IO_FORMAT selects (pp_write_solution).  The folder must exist (the reference writes
+ * into soln/).  One device holds the whole grid, so the reference's chunk arguments dir /
+ * aligndir have no counterpart.  Synchronises the handle's stream.  PP_IO_FAIL when a file
+ * cannot be written, PP_NO_HDF5 when HDF5 output is asked for and no HDF5 library loads. */
 int pp_io(pp_handle h, const char *folder, int app);
 
 /* Flow-solver form (no reference counterpart): solve in place on a caller-owned device array
@@ -173,6 +205,23 @@ int pp_get_device_field(pp_handle h, double **d_field);
 
 /* PittPackGetErrorEnum (src/pencilDcmp.cpp:4869-4913) extended with the codes above. */
 const char *pp_get_error_enum(int code);
+
+/* Host-side writer behind pp_io (no GPU needed): the field u (nz*ny*nx doubles, x fastest)
+ * of the box X[6] as
+ *   PP_IO_HDF5: <folder>/Pxdmf3d<app>.h5 holding the reference's four FP64 datasets X, Y, Z
+ *     (node coordinates, (nz+1) x (ny+1) x (nx+1) x 1) and Q (nz x ny x nx x 1), each chunked
+ *     as one block (src/phdf5.cpp:1380-1420,1700-1760), and <folder>/Pxdmf3d<app>.xmf, an
+ *     XDMF 2.0 spatial collection with one 3DSMesh grid whose X_Y_Z geometry and cell
+ *     attribute Q are hyperslabs of those datasets (src/phdf5.cpp:1809-2070);
+ *   PP_IO_RAW: <folder>/Pxdmf3d<app>.bin (raw little-endian FP64, z slowest) and an XDMF
+ *     index describing the same mesh as a 3DCoRectMesh (origin and spacing).
+ * HDF5 is loaded at run time (dlopen; $PP_HDF5_LIB first, then libhdf5.so on the loader's
+ * path and the image's known locations); PP_NO_HDF5 when none loads. */
+int pp_write_solution(const char *folder, int app, int nx, int ny, int nz, const double *X, const double *u,
+                      int format);
+
+/* 1 when an HDF5 1.10+ library could be loaded for PP_IO_HDF5 output, else 0 */
+int pp_hdf5_available(void);
 
 /* 1 if the transform length n is compiled in, else 0 */
 int pp_size_supported(int n);

@@ -28,6 +28,10 @@ INVALID_HANDLE = 27
 CALL_ORDER = 28
 NO_DEVICE = 29
 IO_FAIL = 30
+NO_HDF5 = 31
+
+IO_HDF5 = 0
+IO_RAW = 1
 
 OPT_ZERO_MODE = 1
 OPT_TIMING = 2
@@ -42,6 +46,7 @@ OPT_GRAPH = 10
 OPT_AUTOTUNE = 11
 OPT_LINE_VARIANT = 12
 OPT_MAX_OVERTAKE = 13
+OPT_IO_FORMAT = 14
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -56,6 +61,7 @@ EXPORTED = (
     "pp_batch_get_kernel_times", "pp_batch_synchronize", "pp_batch_solution", "pp_batch_set_option_double", "pp_batch_get_rhs",
     "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
     "pp_server_stats", "pp_batch_tuning", "pp_server_wait_ex", "pp_server_footprint", "pp_copy_device",
+    "pp_init_trigonometric", "pp_get_error", "pp_fill_trigonometric", "pp_write_solution", "pp_hdf5_available",
 )
 
 
@@ -140,6 +146,11 @@ def load_library(path: str | None = None):
     L.pp_batch_set_option_double.argtypes = [vp, i, ctypes.c_double]
     L.pp_copy_device.argtypes = [vp, vp, ctypes.c_longlong, i, vp]
     L.pp_batch_synchronize.argtypes = [vp]
+    L.pp_init_trigonometric.argtypes = [vp, i]
+    L.pp_get_error.argtypes = [vp, dp]
+    L.pp_fill_trigonometric.argtypes = [vp, dp]
+    L.pp_write_solution.argtypes = [ctypes.c_char_p, i, i, i, i, dp, dp, i]
+    L.pp_hdf5_available.argtypes = []
     for name in EXPORTED:
         if name != "pp_get_error_enum":
             getattr(L, name).restype = ctypes.c_int
@@ -168,6 +179,19 @@ def size_supported(n: int) -> bool:
 def copy_device(d_in: int, d_out: int, n: int, mode: int = 0, stream: int = 0) -> None:
     """streaming device copy of n doubles (the achievable-bandwidth yardstick)"""
     _check(load_library().pp_copy_device(d_in, d_out, int(n), int(mode), stream or None), "copy_device")
+
+
+def write_solution(folder: str, app: int, u, X, format: int = IO_HDF5) -> None:
+    """Host-side solution writer behind PoissonHIP.IO (no GPU needed); u has shape (nz, ny, nx)."""
+    f = np.ascontiguousarray(u, dtype=np.float64)
+    nz, ny, nx = f.shape
+    x = np.ascontiguousarray(X, dtype=np.float64)
+    _check(load_library().pp_write_solution(os.fsencode(folder), int(app), nx, ny, nz, _dp(x), _dp(f), int(format)),
+           "write_solution")
+
+
+def hdf5_available() -> bool:
+    return bool(load_library().pp_hdf5_available())
 
 
 def selftest() -> int:
@@ -222,10 +246,31 @@ class PoissonHIP:
         _check(self._L.pp_get_solution(self._h, _dp(u)), "getSolution")
         return u
 
-    def IO(self, app: int = 0, folder: str = "soln") -> None:
-        """PencilDcmp::IO (src/pencilDcmp.cpp:1827-1832): write <folder>/Pxdmf3d<app>.xmf and
-        the field it indexes, <folder>/Pxdmf3d<app>.bin (little-endian float64, z slowest)."""
+    def IO(self, app: int = 0, folder: str = "soln", format: int | None = None) -> None:
+        """PencilDcmp::IO (src/pencilDcmp.cpp:1827-1832): write <folder>/Pxdmf3d<app>.h5 (datasets
+        X, Y, Z, Q as src/phdf5.cpp:1299-1808 lays them out) and its XDMF index
+        <folder>/Pxdmf3d<app>.xmf; format=IO_RAW writes <folder>/Pxdmf3d<app>.bin instead."""
+        if format is not None:
+            self.set_option(OPT_IO_FORMAT, format)
         _check(self._L.pp_io(self._h, os.fsencode(folder), int(app)), "IO")
+
+    def initializeTrigonometric(self, part: int = 0) -> None:
+        """PencilDcmp::initializeTrigonometric (src/pencilDcmp.cpp:1871-2035): the reference's
+        manufactured right-hand side (part 0 real, 1 imaginary part of its complex field)."""
+        _check(self._L.pp_init_trigonometric(self._h, int(part)), "initializeTrigonometric")
+
+    def getError(self) -> float:
+        """PencilDcmp::getError (src/pencilDcmp.cpp:2040-2208): RMS error of the field against
+        the manufactured case's exact solution (non-destructive)."""
+        e = ctypes.c_double(0.0)
+        _check(self._L.pp_get_error(self._h, ctypes.byref(e)), "getError")
+        return e.value
+
+    def fillTrigonometric(self) -> np.ndarray:
+        """PencilDcmp::fillTrigonometric (src/pencilDcmp.cpp:5685-5765): -4 pi^2 cos(2 pi z)."""
+        f = np.empty((self.nz, self.ny, self.nx), dtype=np.float64)
+        _check(self._L.pp_fill_trigonometric(self._h, _dp(f)), "fillTrigonometric")
+        return f
 
     # -- device / flow-solver forms ------------------------------------------------------
     def set_option(self, key: int, value: int) -> None:

@@ -75,6 +75,11 @@ struct pp_solver {
     pp::ZSolve *d_zself = nullptr;     // this handle as a one-member batch (tile items read it)
     bool pinned00 = false;
     bool mixed_ready = false;
+    int io_format = 0;             // PP_OPT_IO_FORMAT
+    // manufactured solution of the reference (pp_init_trigonometric): -1 none (pp_get_error
+    // then uses the cos(2 pi z) case), 0 / 1 real / imaginary part of its complex field
+    int trig_part = -1;
+    double *d_partials = nullptr;  // pp_get_error's per-block partial sums
 };
 
 

@@ -943,6 +943,7 @@ int pp_destroy(pp_handle h)
     free_tables(h);
     free_mixed(h);
     if (h->d_zt) (void)hipFree(h->d_zt);
+    if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->d_field) (void)hipFree(h->d_field);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -997,6 +998,7 @@ int pp_assign_rhs(pp_handle h, const double *rhs)
     PP_HIP_CHECK(hipMemcpyAsync(h->d_field, rhs, bytes, hipMemcpyHostToDevice, h->stream));
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->rhs_set = true;
+    h->trig_part = -1;
     return PP_SUCCESS;
 }
 
@@ -1007,6 +1009,7 @@ int pp_assign_rhs_device(pp_handle h, const double *d_rhs)
     const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(h->d_field, d_rhs, bytes, hipMemcpyDeviceToDevice, h->stream));
     h->rhs_set = true;
+    h->trig_part = -1;
     return PP_SUCCESS;
 }
 
@@ -1048,76 +1051,6 @@ int pp_get_solution(pp_handle h, double *u)
     const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
     PP_HIP_CHECK(hipMemcpyAsync(u, h->d_field, bytes, hipMemcpyDeviceToHost, h->stream));
     PP_HIP_CHECK(hipStreamSynchronize(h->stream));
-    return PP_SUCCESS;
-}
-
-// Solution output (src/pencilDcmp.cpp:1827-1832 -> Phdf5::writeMultiBlockCellCenter,
-// src/phdf5.cpp:1299-1800, and its XDMF index xdmfMultiBlockCellCenter, src/phdf5.cpp:1809).
-// The reference writes "soln/Pxdmf3d<app>.h5" (node coordinates X, Y, Z on the
-// (nx+1)(ny+1)(nz+1) cell corners and the cell-centred field Q, z slowest) and an XDMF 2.0
-// index "soln/Pxdmf3d<app>.xmf" that names Q with Center="Cell".  HDF5 is not part of this
-// build: the field goes to a raw little-endian FP64 file "<folder>/Pxdmf3d<app>.bin" (nz, ny,
-// nx order, x fastest) and the index describes the same uniform mesh as a 3DCoRectMesh with
-// origin and spacing, which ParaView and VisIt read as they read the reference's file.
-int pp_io(pp_handle h, const char *folder, int app)
-{
-    if (!h) return PP_INVALID_HANDLE;
-    if (!folder || app < 0) return PP_INPUT_ARGS;
-    if (!h->box_set || !h->coords_set) return PP_CALL_ORDER;
-    const int nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
-    const size_t n = (size_t)nx * ny * nz;
-    std::vector<double> u;
-    try {
-        u.resize(n);
-    } catch (const std::bad_alloc &) {
-        return PP_ALLOCATION_FAIL;
-    }
-    PP_HIP_CHECK(hipMemcpyAsync(u.data(), h->d_field, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    PP_HIP_CHECK(hipStreamSynchronize(h->stream));
-    const unsigned probe = 1;
-    unsigned char first;
-    std::memcpy(&first, &probe, 1);
-    if (first != 1) {  // big-endian host: the file is little-endian
-        for (double &v : u) {
-            unsigned char b[8];
-            std::memcpy(b, &v, 8);
-            for (int i = 0; i < 4; i++) std::swap(b[i], b[7 - i]);
-            std::memcpy(&v, b, 8);
-        }
-    }
-    const std::string dir(folder), stem = "Pxdmf3d" + std::to_string(app);
-    const std::string bin = (dir.empty() ? "" : dir + "/") + stem + ".bin";
-    const std::string xmf = (dir.empty() ? "" : dir + "/") + stem + ".xmf";
-    FILE *f = std::fopen(bin.c_str(), "wb");
-    if (!f) return PP_IO_FAIL;
-    const bool ok = std::fwrite(u.data(), sizeof(double), n, f) == n;
-    if (std::fclose(f) != 0 || !ok) return PP_IO_FAIL;
-    const double *X = h->g.X;
-    const double dx = (X[1] - X[0]) / nx, dy = (X[3] - X[2]) / ny, dz = (X[5] - X[4]) / nz;
-    f = std::fopen(xmf.c_str(), "w");
-    if (!f) return PP_IO_FAIL;
-    int w = std::fprintf(f,
-                         "<?xml version=\"1.0\" ?>\n"
-                         "<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n"
-                         "<Xdmf Version=\"2.0\">\n"
-                         "  <Domain>\n"
-                         "    <Grid Name=\"P\" GridType=\"Uniform\">\n"
-                         "      <Topology TopologyType=\"3DCoRectMesh\" Dimensions=\"%d %d %d\"/>\n"
-                         "      <Geometry GeometryType=\"ORIGIN_DXDYDZ\">\n"
-                         "        <DataItem Name=\"Origin\" Dimensions=\"3\" NumberType=\"Float\" Precision=\"8\" "
-                         "Format=\"XML\">%.17g %.17g %.17g</DataItem>\n"
-                         "        <DataItem Name=\"Spacing\" Dimensions=\"3\" NumberType=\"Float\" Precision=\"8\" "
-                         "Format=\"XML\">%.17g %.17g %.17g</DataItem>\n"
-                         "      </Geometry>\n"
-                         "      <Attribute Name=\"Q\" AttributeType=\"Scalar\" Center=\"Cell\">\n"
-                         "        <DataItem Dimensions=\"%d %d %d\" NumberType=\"Float\" Precision=\"8\" "
-                         "Format=\"Binary\" Endian=\"Little\">%s.bin</DataItem>\n"
-                         "      </Attribute>\n"
-                         "    </Grid>\n"
-                         "  </Domain>\n"
-                         "</Xdmf>\n",
-                         nz + 1, ny + 1, nx + 1, X[4], X[2], X[0], dz, dy, dx, nz, ny, nx, stem.c_str());
-    if (std::fclose(f) != 0 || w < 0) return PP_IO_FAIL;
     return PP_SUCCESS;
 }
 
@@ -1179,6 +1112,10 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_PRECISION:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->precision = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_IO_FORMAT:
+        if (value != PP_IO_HDF5 && value != PP_IO_RAW) return PP_INPUT_ARGS;
+        h->io_format = (int)value;
         return PP_SUCCESS;
     case PP_OPT_REFINE:
         if (value < 0 || value > 16) return PP_INPUT_ARGS;
@@ -1308,6 +1245,7 @@ const char *pp_get_error_enum(int code)
     case PP_CALL_ORDER: return "CALL_ORDER";
     case PP_NO_DEVICE: return "NO_DEVICE";
     case PP_IO_FAIL: return "IO_FAIL";
+    case PP_NO_HDF5: return "NO_HDF5";
     default: return "UNKNOWN_ERROR";
     }
 }

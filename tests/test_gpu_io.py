@@ -3,6 +3,7 @@
 The written field must be the solution bit for bit (the reference stores Q as native double,
 src/phdf5.cpp:1780-1783), in z, y, x order, and the XDMF index must describe a cell-centred
 attribute "Q" on the (nx+1)(ny+1)(nz+1) corner mesh of the box (src/phdf5.cpp:1809-2070).
+Default format: the reference's HDF5 file (datasets X, Y, Z, Q); PP_IO_RAW: a raw binary field.
 """
 import os
 import xml.etree.ElementTree as ET
@@ -11,6 +12,7 @@ import numpy as np
 import pytest
 
 import pittpack_amd as pa
+import _h5
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +35,7 @@ def test_io_writes_solution_and_index(tmp_path, shape):
     print(f"io {shape}", flush=True)
     M = _solve(nx, ny, nz, "DDNDDD", X, 5)
     u = M.getSolution()
-    M.IO(7, str(tmp_path))
+    M.IO(7, str(tmp_path), format=pa.IO_RAW)
     xmf, binf = tmp_path / "Pxdmf3d7.xmf", tmp_path / "Pxdmf3d7.bin"
     assert os.path.getsize(binf) == nx * ny * nz * 8
     q = np.fromfile(binf, dtype="<f8").reshape(nz, ny, nx)
@@ -59,6 +61,23 @@ def test_io_writes_solution_and_index(tmp_path, shape):
     assert d.text.strip() == "Pxdmf3d7.bin"
 
 
+@pytest.mark.skipif(_h5.lib() is None, reason="no HDF5 library in this image")
+def test_io_hdf5_default(tmp_path):
+    nx, ny, nz = 24, 20, 18
+    X = [-0.3, 0.7, 0.0, 2.0, 0.5, 1.0]
+    print("io hdf5 (24, 20, 18)", flush=True)
+    M = _solve(nx, ny, nz, "NNNNDD", X, 9)
+    u = M.getSolution()
+    M.IO(2, str(tmp_path))
+    q, chunk, _ = _h5.read(str(tmp_path / "Pxdmf3d2.h5"), "/Q")
+    assert q.shape == (nz, ny, nx, 1) and np.array_equal(q[..., 0], u)
+    z, _, _ = _h5.read(str(tmp_path / "Pxdmf3d2.h5"), "/Z")
+    assert z.shape == (nz + 1, ny + 1, nx + 1, 1)
+    assert z[0, 0, 0, 0] == X[4] and abs(z[-1, 0, 0, 0] - X[5]) < 1e-15
+    root = ET.parse(tmp_path / "Pxdmf3d2.xmf").getroot()
+    assert root.find("Domain/Grid/Grid/Topology").get("TopologyType") == "3DSMesh"
+
+
 def test_io_errors(tmp_path):
     M = _solve(16, 16, 16, "NNNNDD", [0, 1, 0, 1, 0, 1], 1)
     with pytest.raises(pa.PittPackError) as e:
@@ -70,5 +89,8 @@ def test_io_errors(tmp_path):
     M.IO(0, str(tmp_path))  # the handle stays usable
     M.assignRhs(np.ones((16, 16, 16)))
     M.pittPack()
-    M.IO(1, str(tmp_path))
-    assert (tmp_path / "Pxdmf3d1.xmf").exists()
+    M.IO(1, str(tmp_path), format=pa.IO_RAW)
+    assert (tmp_path / "Pxdmf3d1.xmf").exists() and (tmp_path / "Pxdmf3d1.bin").exists()
+    with pytest.raises(pa.PittPackError) as e:
+        M.set_option(pa.OPT_IO_FORMAT, 5)
+    assert e.value.code == pa.INPUT_ARGS

@@ -18,6 +18,9 @@ SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(OUT_DIR, "libpittpack_hip.so")
+# the C++ binding of include/pittpack_hip.hpp, driven in main.cpp's call order (tests load it)
+CXX_SRC = os.path.join(HERE, "cxx", "pp_cxx_check.cpp")
+CXX_OUT = os.path.join(OUT_DIR, "libpp_cxx_check.so")
 
 HIPCC_FLAGS = [
     "-O3",
@@ -81,7 +84,21 @@ def build(verbose: bool = False, extra=(), jobs: int | None = None, incremental:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, OUT)
+    build_cxx_check()
     return OUT
+
+
+def build_cxx_check() -> str:
+    """g++ build of the C++ binding against the library (rpath: its own folder)."""
+    tmp = CXX_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), CXX_SRC,
+           "-L" + OUT_DIR, "-lpittpack_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+        raise RuntimeError("g++ failed on the C++ binding")
+    os.replace(tmp, CXX_OUT)
+    return CXX_OUT
 
 
 if __name__ == "__main__":

@@ -192,6 +192,7 @@ int BatchCore::plan_mixed()
     m00_.clear();
     m00_mem_.clear();
     ritems_.clear();
+    res_two_ = false;
     for (size_t i = 0; i < mem.size(); i++) {
         BatchMember &m = mem[i];
         const SolveSetup &g = m.g;
@@ -227,6 +228,7 @@ int BatchCore::plan_mixed()
             m00_.push_back(it);
             m00_mem_.push_back((int)i);
         }
+        res_two_ = res_two_ || residual_two_halo(g.nx, g.ny);
         const int zp = residual_zplanes();
         const int tx = residual_tile_x(), ty = residual_tile_y();
         const int gx = (g.nx + tx - 1) / tx, gy = (g.ny + ty - 1) / ty, gz = (g.nz + zp - 1) / zp;
@@ -633,7 +635,7 @@ int BatchCore::run_mixed(hipStream_t s, Marks *mk)
     if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 0))) return rc;
     if ((rc = mark(s, mk))) return rc;
     for (int it = 0; it < refine; it++) {
-        PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), s));
+        PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), res_two_, s));
         if ((rc = mark(s, mk))) return rc;
         if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0))) return rc;
         if ((rc = mark(s, mk))) return rc;

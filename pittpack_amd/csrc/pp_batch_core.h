@@ -123,6 +123,7 @@ class BatchCore {
     std::vector<int> m00_mem_;  // member of each m00_ entry
     std::vector<ResArgs> res_;
     std::vector<int4> ritems_;
+    bool res_two_ = false;  // some member's residual tiles need the second halo load
     int nz_max_ = 0;
     long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)

@@ -52,17 +52,22 @@ constexpr int bs_len(int N)
 constexpr int kBsMaxN = 1024;
 constexpr int bs_threads(int L) { return L / 8; }  // 8 complex elements per thread
 
-// columns per block of a y launch / lines per block of an x launch
+// columns per block of a y launch / lines per block of an x launch.  The chirp convolution
+// computes in FP64 in every precision form (R is the storage type of the FP32 forms and does
+// not change the shape): its two length-L FFTs and kernel product carry FP32 rounding well
+// above the compiled plans', more than two refinement steps of the mixed form remove, while
+// FP32 arithmetic issues at the FP64 rate here (packed FP32 is off), so FP64 costs only LDS
+// and registers.
 template <typename R, int L, int AX>
 constexpr int bs_lines()
 {
     constexpr int T = bs_threads(L);
-    constexpr int bytes = line_slots<L>() * 2 * (int)sizeof(R);
-    int lines = AX == 0 ? 256 / T : (sizeof(R) == 8 ? 16 : 32);
+    constexpr int bytes = line_slots<L>() * 2 * (int)sizeof(double);
+    int lines = AX == 0 ? 256 / T : 16;
     if (lines < 1) lines = 1;
     while (lines > 1 && lines * bytes > 144 * 1024) lines /= 2;
-    // FP64 blocks of at most 512 threads (up to 256 VGPRs: the long lines need ~140)
-    while (lines > 1 && lines * T > (sizeof(R) == 8 ? 512 : 1024)) lines /= 2;
+    // blocks of at most 512 threads (up to 256 VGPRs: the long lines need ~140)
+    while (lines > 1 && lines * T > 512) lines /= 2;
     return lines;
 }
 
@@ -292,16 +297,17 @@ hipError_t launch_line_bs_kind(int form, int AX, int FWD, const LineArgs &a, hip
                                 : launch_line_bs_sz<double, double, double, 0, 0, KIND>(a, s);
         return FWD ? launch_line_bs_sz<double, double, double, 1, 1, KIND>(a, s)
                    : launch_line_bs_sz<double, double, double, 1, 0, KIND>(a, s);
+    // FP32 storage, FP64 chirp convolution (bs_lines)
     case FORM_F32:
-        if (AX == 0) return FWD ? launch_line_bs_sz<float, float, float, 0, 1, KIND>(a, s)
-                                : launch_line_bs_sz<float, float, float, 0, 0, KIND>(a, s);
-        return FWD ? launch_line_bs_sz<float, float, float, 1, 1, KIND>(a, s)
-                   : launch_line_bs_sz<float, float, float, 1, 0, KIND>(a, s);
+        if (AX == 0) return FWD ? launch_line_bs_sz<float, float, double, 0, 1, KIND>(a, s)
+                                : launch_line_bs_sz<float, float, double, 0, 0, KIND>(a, s);
+        return FWD ? launch_line_bs_sz<float, float, double, 1, 1, KIND>(a, s)
+                   : launch_line_bs_sz<float, float, double, 1, 0, KIND>(a, s);
     case FORM_F64_TO_F32:
-        if (AX == 0 && FWD) return launch_line_bs_sz<double, float, float, 0, 1, KIND>(a, s);
+        if (AX == 0 && FWD) return launch_line_bs_sz<double, float, double, 0, 1, KIND>(a, s);
         return hipErrorInvalidValue;
     case FORM_F32_TO_F64:
-        if (AX == 0 && !FWD) return launch_line_bs_sz<float, double, float, 0, 0, KIND>(a, s);
+        if (AX == 0 && !FWD) return launch_line_bs_sz<float, double, double, 0, 0, KIND>(a, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }

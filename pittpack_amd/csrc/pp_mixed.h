@@ -48,7 +48,10 @@ struct Mode00Item {
 // cols[i] = {sx, sy, member, 0}; nz_max bounds every listed column's length
 hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s);
 hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, int correct, hipStream_t s);
-hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, hipStream_t s);
+// two: some member has nx = 1 (mod residual_tile_x()) or ny = 1 (mod residual_tile_y())
+// (residual_two_halo), which takes a second halo load per direction
+bool residual_two_halo(int nx, int ny);
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s);
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
                          double *plane_add, const double *upart, int nparts, int correct, hipStream_t s);
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s);

@@ -157,7 +157,10 @@ constexpr int RES_TY = 4;   // ... four rows
 // tile's edge lanes (x) and edge waves (y) need a halo or ghost value from memory: four
 // loads per cell -- u(z+1), f and one halo per direction -- instead of seven.
 // Batched launches: block b works on items[b] = {member, bx, by, bz} with solves[member].
-template <typename TOut>
+// TWO: grids with nx = 1 (mod 64) or ny = 1 (mod 4), whose last column (row) is the first lane
+// (wave) of its tile and so needs both neighbours of that direction from memory: one more
+// halo load per direction and plane.
+template <typename TOut, bool TWO>
 __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
     ResArgs a = a0;
@@ -205,9 +208,10 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     const double *ef = a.dtab + a.nx + a.ny + a.nz;
     // Halo lanes and waves: the neighbour is outside the tile, or outside the grid where the
     // ghost closure the spectral solve diagonalises applies (P wraps, N mirrors, D is
-    // antisymmetric).  With nx and ny even no cell needs both neighbours of one direction
-    // from memory, so one halo load per direction serves every lane (a lane with no halo in
-    // a direction reads its own cell): unconditional loads keep the compiler's count of
+    // antisymmetric).  Only a cell that is both the first lane (wave) of its tile and the
+    // grid's last column (row) needs both neighbours of one direction from memory (TWO);
+    // otherwise one halo load per direction serves every lane (a lane with no halo in a
+    // direction reads its own cell): unconditional loads keep the compiler's count of
     // outstanding loads exact, which the pipelined march depends on.
     auto ghost = [](char bc, long wrap, long &off, double &sg) {
         off = bc == 'P' ? wrap : 0;
@@ -222,6 +226,9 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     if (yc == 0) ghost(a.bc[2], sy * (a.ny - 1), oym, gym);
     if (yc == a.ny - 1) ghost(a.bc[3], -sy * (a.ny - 1), oyp, gyp);
     const long ohx = hxm ? oxm : (hxp ? oxp : 0), ohy = hym ? oym : (hyp ? oyp : 0);
+    // the upper neighbour of a cell whose lower one already takes the halo load
+    const bool bothx = TWO && hxm && hxp, bothy = TWO && hym && hyp;
+    const long ohx2 = bothx ? oxp : 0, ohy2 = bothy ? oyp : 0;
     // z-profile of the x/y-face Dirichlet terms for this block's planes, and the u(z) tile
     __shared__ double ezs[RES_ZC];
     __shared__ double tile[2][RES_TY][RES_TX];
@@ -247,7 +254,7 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     // loads of one plane: u(z+1) (at the top face: plane 0 for P, else this plane, replaced
     // by the ghost below), the right-hand side and the halo value per direction
     struct Plane {
-        double up, f, hx, hy;
+        double up, f, hx, hy, hx2, hy2;
     };
     const bool ztop_wrap = a.bc[5] == 'P';
     auto fetch = [&](int z, Plane &q) {
@@ -256,6 +263,13 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
         q.f = f_p[o];
         q.hx = uc_p[o + ohx];
         q.hy = uc_p[o + ohy];
+        if constexpr (TWO) {
+            q.hx2 = uc_p[o + ohx2];
+            q.hy2 = uc_p[o + ohy2];
+        } else {
+            q.hx2 = q.hx;
+            q.hy2 = q.hy;
+        }
     };
     // software pipelined march: the loads of plane z + 1 are issued before plane z is
     // computed and stored, so their latency overlaps this plane's arithmetic
@@ -271,10 +285,14 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
         const long o = sz * z;
         const double up = (z + 1 < a.nz || ztop_wrap) ? cur.up : (a.bc[5] == 'N' ? uc : -uc);
         const double uxm = hxm ? gxm * cur.hx : tile[b][w][lane > 0 ? lane - 1 : 0];
-        const double uxp = hxp ? gxp * cur.hx : tile[b][w][lane < 63 ? lane + 1 : 63];
+        const double uxp = hxp ? gxp * (bothx ? cur.hx2 : cur.hx) : tile[b][w][lane < 63 ? lane + 1 : 63];
         const double uym = hym ? gym * cur.hy : tile[b][w > 0 ? w - 1 : 0][lane];
-        const double uyp = hyp ? gyp * cur.hy : tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
-        const double Lu = (uxp - 2.0 * uc + uxm) * a.Cx + (uyp - 2.0 * uc + uym) * a.Cy + (up - 2.0 * uc + um);
+        const double uyp = hyp ? gyp * (bothy ? cur.hy2 : cur.hy) : tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
+        // second differences as differences of first differences: neighbouring values are
+        // close, so each first difference is exact (Sterbenz) and the rounding scales with the
+        // differences, not with u -- on cells with dz >> dx, dy the factors Cx, Cy are large
+        // and eps |u| Cy would otherwise set the floor the refinement converges to
+        const double Lu = ((uxp - uc) - (uc - uxm)) * a.Cx + ((uyp - uc) - (uc - uym)) * a.Cy + ((up - uc) - (uc - um));
         // effective right-hand side: rhs * dz^2 minus the Dirichlet boundary term
         double v = cur.f;
         v = v * a.c3;
@@ -326,19 +344,23 @@ int residual_zplanes() { return RES_ZC; }
 
 int residual_blocks(int nx, int ny, int nz) { return residual_parts(nx, ny) * ((nz + RES_ZC - 1) / RES_ZC); }
 
+bool residual_two_halo(int nx, int ny) { return nx % RES_TX == 1 || ny % RES_TY == 1; }
+
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.nx + RES_TX - 1) / RES_TX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
               (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
-    hipLaunchKernelGGL(k_residual<float>, grid, dim3(256), 0, s, a, nullptr, nullptr);
+    if (residual_two_halo(a.nx, a.ny)) hipLaunchKernelGGL((k_residual<float, true>), grid, dim3(256), 0, s, a, nullptr, nullptr);
+    else hipLaunchKernelGGL((k_residual<float, false>), grid, dim3(256), 0, s, a, nullptr, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, hipStream_t s)
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s)
 {
     if (nitems == 0) return hipSuccess;
     ResArgs a{};
-    hipLaunchKernelGGL(k_residual<float>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
+    if (two) hipLaunchKernelGGL((k_residual<float, true>), dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
+    else hipLaunchKernelGGL((k_residual<float, false>), dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
     return hipGetLastError();
 }
 

@@ -70,15 +70,27 @@ def test_parity_any_length_both_axes(bc, shape):
 @pytest.mark.parametrize("n", LENGTHS + [600, 1000])
 @pytest.mark.parametrize("bc", ["PPPPNN", "NNNNDD", "DDNDDD", "NDDNDN", "DNNNDD", "NNNNNN"])
 def test_parity_any_length_mixed(bc, n):
-    """FP32 arbitrary-length transforms with FP64 refinement: the FP64 bar after two steps"""
+    """FP32-stored arbitrary-length transforms with FP64 refinement: the FP64 bar after two
+    steps.  The box makes dz equal to the long axis' spacing: the refinement converges to the
+    exact 7-point solution, which on cells much thinner than dz differs from the reference's
+    spectral solution by its eigenvalue rounding (test_gpu_mixed.py,
+    test_mixed_converges_to_stencil_solution)."""
     shape = (5, 24, n) if n % 2 else (4, n, 24)
-    print(f"any length mixed {bc} {shape}", flush=True)
+    nz = shape[0]
+    box = [0, 1, 0, 1, 0, nz / n]
+    # With Neumann z faces the columns of the lowest (kx, ky) modes have eig + 2 ~ (pi / n)^2,
+    # and the reference's eigenvalue arithmetic (-2 + 2 cos theta, then -2 + ... sums), which
+    # the oracle and the FP64 form share, carries a relative error of about 4 eps / (pi / n)^2
+    # there; the refinement converges to the exact 7-point solution instead.  The bar is the
+    # larger of 1e-12 and that bound (Dirichlet z lifts every column: 1e-12).
+    tol = TOL if "D" in bc[4:] else max(TOL, 4 * np.finfo(float).eps * (n / np.pi) ** 2)
+    print(f"any length mixed {bc} {shape} tol {tol:.1e}", flush=True)
     f = prng.uniform_pm1(shape, seed=n + 3)
     for zero_mode in (0, 1):
-        ref = oracle.solve(f, bc, zero_mode=zero_mode)
-        got = pa.solve(f, bc, zero_mode=zero_mode, precision=1)
+        ref = oracle.solve(f, bc, box, zero_mode=zero_mode)
+        got = pa.solve(f, bc, box, zero_mode=zero_mode, precision=1)
         err = rel_maxdiff(got, ref)
-        assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
+        assert err <= tol, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
 ZPER_COMBOS = ["PPPPPP", "NNNNPP", "DDDDPP", "NDDNPP", "NNPPPP"]

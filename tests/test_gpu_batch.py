@@ -148,7 +148,7 @@ def test_batch_errors():
             B.solve()
         assert e.value.code == pa.CALL_ORDER
         with pytest.raises(pa.PittPackError) as e:
-            B.add(30, 32, 16, "PPPPNN")
+            B.add(1030, 32, 16, "PPPPNN")
         assert e.value.code == pa.UNSUPPORTED_SIZE
         with pytest.raises(pa.PittPackError) as e:
             B.add(32, 32, 16, "PNPPNN")

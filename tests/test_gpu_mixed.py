@@ -141,3 +141,49 @@ def test_handle_graph_replays_match_eager(precision):
     finally:
         E.close()
         G.close()
+
+
+def _stencil_solution(f, bc, box):
+    """the 7-point ghost-cell system solved directly (sparse LU plus one refinement step with an
+    extended-precision residual), independent of any spectral eigenvalue formula"""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    nz, ny, nx = f.shape
+    d = [(box[1] - box[0]) / nx, (box[3] - box[2]) / ny, (box[5] - box[4]) / nz]
+
+    def op(n, lo, hi):
+        main = -2.0 * np.ones(n)
+        main[0] += -1.0 if lo == "D" else 1.0
+        main[-1] += -1.0 if hi == "D" else 1.0
+        return sp.diags([np.ones(n - 1), main, np.ones(n - 1)], [-1, 0, 1], format="csr")
+
+    Ax, Ay, Az = op(nx, bc[0], bc[1]), op(ny, bc[2], bc[3]), op(nz, bc[4], bc[5])
+    Ix, Iy, Iz = sp.identity(nx), sp.identity(ny), sp.identity(nz)
+    L = ((d[2] / d[0]) ** 2 * sp.kron(sp.kron(Iz, Iy), Ax) + (d[2] / d[1]) ** 2 * sp.kron(sp.kron(Iz, Ay), Ix)
+         + sp.kron(sp.kron(Az, Iy), Ix)).tocsc()
+    fe = oracle.effective_rhs(f, bc, box).ravel()
+    lu = spl.splu(L)
+    u = lu.solve(fe)
+    r = fe.astype(np.longdouble) - L.astype(np.longdouble) @ u.astype(np.longdouble)
+    return (u + lu.solve(r.astype(np.float64))).reshape(f.shape)
+
+
+@pytest.mark.parametrize("bc", ["DDDDDD", "DDNNDD", "NNDDDD"])
+def test_mixed_converges_to_stencil_solution(bc):
+    """On cells much thinner than dz (here dy = dz / 250, so Cy = 62500) the reference's
+    eigenvalues -2 + 2 cos(theta) lose digits to cancellation at small theta, which the FP64
+    spectral solve (oracle and GPU alike) carries: it sits ~1e-12 off the exact solution of
+    the 7-point system.  The mixed form's FP64-residual refinement converges to that exact
+    solution instead.  Both facts are pinned against a direct sparse solve."""
+    shape, box = (4, 1000, 24), [0, 1, 0, 1, 0, 1]
+    print(f"mixed vs stencil {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=1003)
+    exact = _stencil_solution(f, bc, box)
+    ref = oracle.solve(f, bc, box, zero_mode=1)
+    scale = np.abs(exact).max()
+    u64 = pa.solve(f, bc, box, zero_mode=1)
+    um = pa.solve(f, bc, box, zero_mode=1, precision=1)
+    assert np.abs(u64 - ref).max() <= 1e-14 * scale         # FP64 form: the reference's arithmetic
+    assert np.abs(ref - exact).max() >= 3e-13 * scale        # ... which is off the exact solution
+    assert np.abs(um - exact).max() <= 1e-13 * scale         # mixed form: the exact solution

@@ -179,6 +179,22 @@ int pp_solve_device(pp_handle h, double *d_field);
  * d_rhs).  Asynchronous on the handle's stream. */
 int pp_solve_device_oop(pp_handle h, const double *d_rhs, double *d_u);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The FP64 solve
+ * of pp_solve_device_oop without a handle: sizes, boundary conditions bc[6], box X[6] and the
+ * PP_OPT_ZERO_MODE rule (0 or 1) are arguments; d_rhs is kept, the solution goes to d_u (may
+ * equal d_rhs), asynchronous on `stream` (hipStream_t as void*, null = default stream).  It
+ * calls no allocator: its tables live in a fixed 2 MiB device array per GPU, held until the
+ * process ends, and the z solve runs on d_u in place (no z-tile buffer).  Limits: the
+ * handle's sizes (lines 2..1024, z columns 4..1024, periodic z 2..1024); larger grids get
+ * PP_UNSUPPORTED_SIZE.  A call with new sizes, boundary conditions, box or zero-mode rule
+ * rebuilds the tables on the host, copies them in and synchronises `stream` (PP_CALL_ORDER
+ * under stream capture: make one uncaptured call with the same arguments first); a repeated
+ * configuration only enqueues kernels.  Calls that overlap on one GPU race on that array:
+ * serialise them. */
+int pp_solve_device_oop_noalloc(int nx, int ny, int nz, const char *bc, const double *X, int zero_mode,
+                                const double *d_rhs, double *d_u, void *stream);
+
 /* Use the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
 int pp_set_stream(pp_handle h, void *stream);
 

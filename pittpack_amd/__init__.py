@@ -62,6 +62,7 @@ EXPORTED = (
     "pp_server_create", "pp_server_destroy", "pp_server_set_option", "pp_server_submit", "pp_server_wait",
     "pp_server_stats", "pp_batch_tuning", "pp_server_wait_ex", "pp_server_footprint", "pp_copy_device",
     "pp_init_trigonometric", "pp_get_error", "pp_fill_trigonometric", "pp_write_solution", "pp_hdf5_available",
+    "pp_solve_device_oop_noalloc",
 )
 
 
@@ -151,6 +152,7 @@ def load_library(path: str | None = None):
     L.pp_fill_trigonometric.argtypes = [vp, dp]
     L.pp_write_solution.argtypes = [ctypes.c_char_p, i, i, i, i, dp, dp, i]
     L.pp_hdf5_available.argtypes = []
+    L.pp_solve_device_oop_noalloc.argtypes = [i, i, i, ctypes.c_char_p, dp, i, vp, vp, vp]
     for name in EXPORTED:
         if name != "pp_get_error_enum":
             getattr(L, name).restype = ctypes.c_int

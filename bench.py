@@ -219,9 +219,12 @@ def serving_requests(n_req: int, seed: int = 5):
     return reqs
 
 
-def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, reqs=None):
+def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, reqs=None, collectors: int = 4,
+                  to_host: bool = True):
     """Poisson arrivals at `rate`/s (rate <= 0: closed loop, every request submitted at once);
-    RHS from the device counter generator; every solution is copied back to the host."""
+    RHS from the device counter generator.  to_host: every solution is copied back to the host
+    (pinned staging, then into a buffer each of `collectors` waiting threads reuses); else the
+    solutions stay on the device and a request completes when its batch has finished."""
     import threading
 
     import numpy as np
@@ -234,14 +237,17 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, r
     tickets = [None] * n_req
     ready = [threading.Event() for _ in range(n_req)]
     lat = [0.0] * n_req
+    biggest = max(s[0] * s[1] * s[2] for s, _, _, _ in reqs)
 
-    def collector():
-        for i in range(n_req):
+    def collector(k):
+        buf = np.ones(biggest) if to_host else None  # touched once: no page faults per request
+        for i in range(k, n_req, collectors):
             ready[i].wait()
-            _, lat[i] = S.wait(tickets[i])
+            _, lat[i] = S.wait(tickets[i], out=buf)
 
-    th = threading.Thread(target=collector)
-    th.start()
+    ths = [threading.Thread(target=collector, args=(k,)) for k in range(collectors)]
+    for th in ths:
+        th.start()
     t0 = time.perf_counter()
     t_next = t0
     for i, (shape, bc, prec, seed) in enumerate(reqs):
@@ -249,9 +255,10 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, r
         d = t_next - time.perf_counter()
         if d > 0:
             time.sleep(d)
-        tickets[i] = S.submit(shape, bc, precision=prec, seed=seed)
+        tickets[i] = S.submit(shape, bc, precision=prec, seed=seed, want_solution=to_host)
         ready[i].set()
-    th.join()
+    for th in ths:
+        th.join()
     wall = time.perf_counter() - t0
     st = S.stats()
     S.close()
@@ -265,6 +272,7 @@ def bench_serving(pa, n_req: int, rate: float, budget_gb: float, streams: int, r
         "batches": st["batches"], "requests_per_batch": n_req / max(1, st["batches"]),
         "max_concurrent_batches": st["max_concurrent_batches"],
         "mean_points_per_request": pts / n_req,
+        "solutions": "copied to host" if to_host else "left on the device",
     }
 
 
@@ -272,10 +280,15 @@ def bench_serving_sweep(pa, n_req: int, rates, budget_gb: float = 200.0, streams
     """the serving workload at several arrival rates and closed loop (same request list)"""
     reqs = serving_requests(n_req)
     runs = [bench_serving(pa, n_req, rate, budget_gb, streams, reqs) for rate in rates]
+    # the engine alone: the same requests at the highest rates with the solutions left on the
+    # device (the host copy of ~35 MB per request bounds the runs above)
+    runs += [bench_serving(pa, n_req, rate, budget_gb, streams, reqs, to_host=False)
+             for rate in rates if rate <= 0 or rate >= 1000]
     return {
         "workload": ("Poisson arrivals; sides log-uniform integers in [32,512] per axis (any length), "
-                     f"bc x/y in {XY_COMBOS} x z in {Z_COMBOS}, FP64 / mixed 50/50, device-generated RHS, "
-                     "solutions copied to host; latency = submit to solution in pinned host memory"),
+                     f"bc x/y in {XY_COMBOS} x z in {Z_COMBOS}, FP64 / mixed 50/50, device-generated RHS; "
+                     "latency = submit to solution in pinned host memory (or, solutions left on the "
+                     "device, to the end of the request's batch)"),
         "runs": runs,
     }
 
@@ -482,7 +495,8 @@ def main():
     copy_gbs = copy_rate(lambda: torch.neg(f, out=u))
     stream_copy = {m: copy_rate(lambda: pa.copy_device(f.data_ptr(), u.data_ptr(), int(pts), mode,
                                                         stream.cuda_stream))
-                   for m, mode in (("cached", 0), ("nontemporal", 1))}
+                   for m, mode in (("cached", 0), ("nontemporal", 1), ("nontemporal_stores", 2),
+                                   ("nontemporal_loads", 3))}
     M = None
 
     out = {

@@ -246,7 +246,8 @@ int pp_size_supported(int n);
 int pp_selftest(void);
 
 /* No reference counterpart: streaming copy of n doubles between 16-byte aligned device
- * arrays on a HIP stream (mode 0 cached, 1 nontemporal), asynchronous.  The yardstick of the
+ * arrays on a HIP stream (mode 0 cached, 1 nontemporal loads and stores, 2 nontemporal stores,
+ * 3 nontemporal loads), asynchronous.  The yardstick of the
  * achievable HBM bandwidth (16 bytes per element, as one FP64 stage per grid point). */
 int pp_copy_device(const double *d_in, double *d_out, long long n, int mode, void *stream);
 

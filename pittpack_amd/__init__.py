@@ -489,10 +489,20 @@ class Server:
         self._pending[t.value] = (f, X, (nz, ny, nx), bool(want_solution))
         return t.value
 
-    def wait(self, ticket: int, with_batch: bool = False):
-        """(solution or None, latency ms[, launch number of the batch that ran it])"""
+    def wait(self, ticket: int, with_batch: bool = False, out: np.ndarray | None = None):
+        """(solution or None, latency ms[, launch number of the batch that ran it]); `out`: a
+        C-contiguous float64 buffer of at least the request's size to receive the solution
+        (returned as a view of it), e.g. one reused across requests"""
         f, X, shape, want = self._pending.pop(ticket)
-        u = np.empty(shape, dtype=np.float64) if want else None
+        u = None
+        if want:
+            n = shape[0] * shape[1] * shape[2]
+            if out is not None:
+                if out.dtype != np.float64 or not out.flags.c_contiguous or out.size < n:
+                    raise ValueError("out: C-contiguous float64 buffer of the request's size needed")
+                u = out.reshape(-1)[:n].reshape(shape)
+            else:
+                u = np.empty(shape, dtype=np.float64)
         lat = ctypes.c_double(0)
         seq = ctypes.c_longlong(-1)
         _check(self._L.pp_server_wait_ex(self._h, int(ticket), _dp(u) if u is not None else None, ctypes.byref(lat),

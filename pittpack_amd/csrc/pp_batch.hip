@@ -67,8 +67,7 @@ int TableCache::get(int N, AxisDev &out)
         out = it->second;
         return PP_SUCCESS;
     }
-    AxisTables t;
-    make_axis_tables(N, t);
+    const AxisTables &t = axis_tables_cached(N);
     AxisDev d;
     auto up = [](double **dst, const std::vector<double> &v) -> int {
         if (hipMalloc((void **)dst, v.size() * sizeof(double)) != hipSuccess) return PP_ALLOCATION_FAIL;

@@ -6,6 +6,9 @@
 // + exactValue, 4623-4835 and 1837-1866) and the FP64 twiddle tables of a line length.
 #pragma once
 #include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -74,17 +77,31 @@ inline void fill_bs_tables(int N, double *tw, double *A, double *B, double *C, d
     const ld pi = 3.141592653589793238462643383279502884L;
     const int L = bs_len(N);
     auto cis = [](ld a) { return std::pair<ld, ld>(std::cos(a), std::sin(a)); };
-    for (int e = 0; e < L; e++) {
-        const auto w = cis(-2 * pi * (ld)e / (ld)L);
-        tw[2 * e] = (double)w.first;
-        tw[2 * e + 1] = (double)w.second;
+    // exp(-2 pi i k / L), k < L / 2: the FFT twiddles of every section and, with
+    // exp(-2 pi i (k + L/2) / L) = -exp(-2 pi i k / L), the Stockham table
+    static thread_local ld wr[kBsMaxN], wi[kBsMaxN];
+    for (int k = 0; k < L / 2; k++) {
+        const auto w = cis(-2 * pi * (ld)k / (ld)L);
+        wr[k] = w.first;
+        wi[k] = w.second;
     }
-    // chirp c_m = exp(i pi beta m^2 / 2): half-angle exp(i pi m^2 / (2N)), full exp(i pi m^2 / N)
-    auto chirp = [&](int m, bool full) {
-        const long long per = full ? 2LL * N : 4LL * N;
-        const long long r = ((long long)m * m) % per;
-        return cis(full ? pi * (ld)r / (ld)N : pi * (ld)r / (2 * (ld)N));
-    };
+    for (int e = 0; e < L; e++) {
+        const ld sg = e < L / 2 ? 1 : -1;
+        tw[2 * e] = (double)(sg * wr[e % (L / 2)]);
+        tw[2 * e + 1] = (double)(sg * wi[e % (L / 2)]);
+    }
+    // chirp c_m = exp(i pi beta m^2 / 2): half-angle exp(i pi m^2 / (2N)), full exp(i pi m^2 / N),
+    // m < N, each evaluated once
+    static thread_local ld chr[2][kBsMaxN], chi[2][kBsMaxN];
+    for (int m = 0; m < N; m++)
+        for (int full = 0; full < 2; full++) {
+            const long long per = full ? 2LL * N : 4LL * N;
+            const long long r = ((long long)m * m) % per;
+            const auto c = cis(full ? pi * (ld)r / (ld)N : pi * (ld)r / (2 * (ld)N));
+            chr[full][m] = c.first;
+            chi[full][m] = c.second;
+        }
+    auto chirp = [&](int m, bool full) { return std::pair<ld, ld>(chr[full][m], chi[full][m]); };
     // kernel spectra: b_j = c_j (j < N), b_{L-j} = c_j (0 < j < N), zero elsewhere; inverse
     // direction uses conj(c)
     static thread_local ld re[2 * kBsMaxN], im[2 * kBsMaxN];
@@ -114,9 +131,10 @@ inline void fill_bs_tables(int N, double *tw, double *A, double *B, double *C, d
         for (int len = 2; len <= L; len <<= 1)
             for (int i = 0; i < L; i += len)
                 for (int k = 0; k < len / 2; k++) {
-                    const auto w = cis(-2 * pi * (ld)k / (ld)len);
-                    const ld xr = re[i + k + len / 2] * w.first - im[i + k + len / 2] * w.second;
-                    const ld xi = re[i + k + len / 2] * w.second + im[i + k + len / 2] * w.first;
+                    // exp(-2 pi i k / len) = twiddle k L / len of the length-L table
+                    const ld cr = wr[k * (L / len)], ci = wi[k * (L / len)];
+                    const ld xr = re[i + k + len / 2] * cr - im[i + k + len / 2] * ci;
+                    const ld xi = re[i + k + len / 2] * ci + im[i + k + len / 2] * cr;
                     re[i + k + len / 2] = re[i + k] - xr;
                     im[i + k + len / 2] = im[i + k] - xi;
                     re[i + k] += xr;
@@ -188,6 +206,21 @@ inline void make_axis_tables(int N, AxisTables &t)
     std::vector<double> *v[5] = {&t.tw, &t.A, &t.B, &t.C, &t.D};
     for (int i = 0; i < 5; i++) v[i]->assign(z.n[i], 0.0);
     fill_axis_tables(N, t.tw.data(), t.A.data(), t.B.data(), t.C.data(), t.D.data());
+}
+
+// the host tables of line length N, built once per process and shared by every handle, batch
+// and server (an arbitrary-length table set takes about a millisecond to build)
+inline const AxisTables &axis_tables_cached(int N)
+{
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<AxisTables>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    std::unique_ptr<AxisTables> &p = cache[N];
+    if (!p) {
+        p.reset(new AxisTables);
+        make_axis_tables(N, *p);
+    }
+    return *p;
 }
 
 // everything one solve's kernels need from (sizes, bc, box)

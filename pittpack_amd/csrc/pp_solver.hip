@@ -216,8 +216,7 @@ int build_tables(pp_solver *h)
     const int n[2] = {h->g.nx, h->g.ny};
     int rc;
     for (int a = 0; a < 2; a++) {
-        AxisTables t;
-        make_axis_tables(n[a], t);
+        const AxisTables &t = axis_tables_cached(n[a]);
         DevTables &d = h->tab64[a];
         if ((rc = upload(&d.tw, t.tw))) return rc;
         if ((rc = upload(&d.A, t.A))) return rc;

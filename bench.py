@@ -366,7 +366,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--zsolver", type=int, default=None,
                     help="0 PCR->Thomas, 1 partitioned Thomas, 2 multigrid, 3 full PCR, 4 cyclic reduction")
-    ap.add_argument("--no-autotune", action="store_true", help="FP64 form in the fixed default configuration")
+    ap.add_argument("--no-autotune", action="store_true", help="FP64 and mixed forms in the fixed default configuration")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel of every solve (no HIP graph)")
     ap.add_argument("--batch-count", type=int, default=4096, help="solves in the batched workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per serving run (0: skip)")
@@ -403,7 +403,7 @@ def main():
         M.setCoords(2)
         if args.zsolver is not None:
             M.set_option(pa.OPT_ZSOLVER, args.zsolver)
-        if precision == 0 and not args.no_autotune:
+        if not args.no_autotune:
             M.set_option(pa.OPT_AUTOTUNE, 1)  # tuned on the first (warm-up) solve
         if precision:
             M.set_option(pa.OPT_PRECISION, precision)
@@ -472,6 +472,7 @@ def main():
             "rel_l2_diff_vs_fp64": float(diff.norm() / u64.norm()),
             "roofline": mroof,
             "kernels_ms": mkms,
+            "tuning": {k: v for k, v in M.tuning().items() if k.startswith("mixed")},
         }
         del diff
         M.close()

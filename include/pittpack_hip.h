@@ -71,8 +71,10 @@ enum {
     PP_OPT_REFINE = 5, /* refinement steps of the mixed form (default 2) */
     PP_OPT_TAU = 6,    /* pp_set_option_double: near-singular threshold (default 0.05) */
     PP_OPT_BATCH_MEMBERS = 7, /* server: most requests per batch (default 256) */
-    /* FP64 form: 1 (default) = the y transforms hand the z solve contiguous column tiles
-     * through a second field-sized buffer; 0 = the z solve works on the field in place */
+    /* 1 (default) = the y transforms hand the z solve contiguous column tiles through a
+     * second field-sized buffer (FP64 form: 16-column FP64 tiles; mixed form: 32-column FP32
+     * tiles, where the y length and z length allow); 0 = the z solve works on the field in
+     * place */
     PP_OPT_ZLAYOUT = 8,
     PP_OPT_MG_TOL = 9, /* pp_set_option_double: multigrid residual tolerance */
     /* batched form: 1 (default) = the second pp_batch_solve captures the solve's kernel
@@ -84,15 +86,17 @@ enum {
      * or boundary change, or a new stream, drops it.  Needs a non-null stream; an in-place
      * mixed-form solve runs eagerly. */
     PP_OPT_GRAPH = 10,
-    /* FP64 form: 1 = before the next solve (batched form: at pp_batch_finalize, per line
-     * class; the batch's right-hand sides are set afterwards), time the candidate
-     * configurations on this shape
-     * (z: PCR->Thomas on z tiles, the same in place, or partitioned Thomas; each line stage:
-     * default or alternative radix plan / block shape) and keep the fastest; the choice is
-     * cached per process and shape.  0 (default) = fixed default configuration. */
+    /* 1 = before the next solve (batched form: at pp_batch_finalize, per line class; the
+     * batch's right-hand sides are set afterwards), time the candidate configurations on this
+     * shape and keep the fastest; the choice is cached per process and shape.  FP64 form --
+     * z: PCR->Thomas on z tiles, the same in place, or partitioned Thomas; each line stage:
+     * default or alternative radix plan / block shape.  Mixed form -- FP32 z tiles or the
+     * work field in place, and the tiled y shape.  0 (default) = fixed default
+     * configuration. */
     PP_OPT_AUTOTUNE = 11,
-    /* testing: force every FP64 line stage to the default (0) or alternative (1) launch
-     * shape where it has one; -1 (default) = the autotuner's choice */
+    /* testing: force every FP64 line stage (and the mixed form's FP32 tiled y stages) to the
+     * default (0) or alternative (1) launch shape where it has one; -1 (default) = the
+     * autotuner's choice */
     PP_OPT_LINE_VARIANT = 12,
     /* server: how many admitted batches may overtake a queued request that does not fit
      * before nothing behind it is admitted (default 4); bounds the wait of large requests
@@ -213,7 +217,9 @@ int pp_get_kernel_times(pp_handle h, float *ms, int max, int *count);
 
 /* Configuration in use: out[0] = 1 when tuned, out[1] = z solver, out[2] = 1 when z tiles
  * are used, out[3..6] = launch shape (0 default, 1 alternative) of the x forward, y forward,
- * y inverse and x inverse stages.  n >= 7.  No counterpart in the reference. */
+ * y inverse and x inverse stages; with n >= 9, the mixed form's out[7] = 1 when FP32 z tiles
+ * are used and out[8] = their y shape (1: half the threads per column).  n >= 7.  No
+ * counterpart in the reference. */
 int pp_get_tuning(pp_handle h, int *out, int n);
 
 /* Device pointer of the handle's field (nx*ny*nz doubles). */

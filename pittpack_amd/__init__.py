@@ -307,10 +307,11 @@ class PoissonHIP:
 
     def tuning(self) -> dict:
         """Configuration in use (after PP_OPT_AUTOTUNE: the tuner's choice)."""
-        out = (ctypes.c_int * 7)()
-        _check(self._L.pp_get_tuning(self._h, out, 7), "get_tuning")
+        out = (ctypes.c_int * 9)()
+        _check(self._L.pp_get_tuning(self._h, out, 9), "get_tuning")
         return {"tuned": bool(out[0]), "zsolver": out[1], "ztiles": out[2],
-                "line_variant": {"x_forward": out[3], "y_forward": out[4], "y_inverse": out[5], "x_inverse": out[6]}}
+                "line_variant": {"x_forward": out[3], "y_forward": out[4], "y_inverse": out[5], "x_inverse": out[6]},
+                "mixed_ztiles": out[7], "mixed_y_variant": out[8]}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value is not None:

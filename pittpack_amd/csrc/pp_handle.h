@@ -64,6 +64,10 @@ struct pp_solver {
     int refine = 2;
     double tau = 0.05;
     float *d_w = nullptr;          // FP32 work field
+    float *d_wt = nullptr;         // FP32 z tiles (32 columns x 16 J rows) between the y transforms
+    int wtiles = 1;                // mixed form: hand the z solve FP32 z tiles where the sizes allow
+    int wvar = 0;                  // mixed form, FP32 tiled y stages: 1 = half the threads per column
+    bool mtuned = false;           // mixed form tuned (PP_OPT_AUTOTUNE)
     double *d_aux = nullptr;       // FP64 scratch field (RHS copy or solution) for host-facing calls
     double *d_line_sums = nullptr; // ny*nz
     double *d_plane_add = nullptr; // nz + 2

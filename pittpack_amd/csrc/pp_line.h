@@ -327,7 +327,10 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
     // position p = 2 (t + T m) + c % 2), no LDS scatter; otherwise element n = t + T c,
     // scattered through LDS below
     constexpr bool DIRECT_IN = (AX == 1) && (FWD || KIND == KIND_ND || KIND == KIND_DN);
-    double xin[2 * E];
+    // inputs held in their own precision (FP32 inputs are only ever scaled by 1): half the
+    // registers for the FP32 forms
+    using XT = typename std::conditional<std::is_same<TIn, float>::value, float, double>::type;
+    XT xin[2 * E];
     {
         const bool tin = (AX == 1) && (a.ztiled & ZT_IN);
         const long ib = tin ? tbase : base, is = tin ? tstride : stride;
@@ -340,7 +343,7 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
                 bool ng;
                 n = pos_elem<N, KIND>(2 * (t + T * (c >> 1)) + (c & 1), ng);
             }
-            xin[c] = (double)in[ib + (long)n * is];
+            xin[c] = (XT)in[ib + (long)n * is];
         }
     }
     // x inverse: the solution being accumulated into and the plane term are read here, with
@@ -384,20 +387,20 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
         const double cyz = dl.cyz, cx0 = dl.cx0, cxn = dl.cxn;
 #pragma unroll
         for (int c = 0; c < 2 * E; c++) {
-            xin[c] = xin[c] * a.in_scale;  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
-            xin[c] = xin[c] * a.in_scale;
+            xin[c] = (XT)(xin[c] * a.in_scale);  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
+            xin[c] = (XT)(xin[c] * a.in_scale);
         }
         // one branch per thread, not per element: y/z-face lines, then the x-face elements
         // n = 0 (thread 0, c = 0) and n = N - 1 (thread T - 1, c = 2E - 1)
         if (cyz != 0.0) {
 #pragma unroll
-            for (int c = 0; c < 2 * E; c++) xin[c] = xin[c] - gld(ex, t + T * c) * cyz;
+            for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] - gld(ex, t + T * c) * cyz);
         }
-        if (t == 0) xin[0] = xin[0] - cx0;
-        if (t == T - 1) xin[2 * E - 1] = xin[2 * E - 1] - cxn;
+        if (t == 0) xin[0] = (XT)(xin[0] - cx0);
+        if (t == T - 1) xin[2 * E - 1] = (XT)(xin[2 * E - 1] - cxn);
     } else {
 #pragma unroll
-        for (int c = 0; c < 2 * E; c++) xin[c] = xin[c] * a.in_scale;
+        for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] * a.in_scale);
     }
     double lsum = 0.0;
     if constexpr (!DIRECT_IN) {
@@ -721,6 +724,13 @@ constexpr int line_lines()
     return lines;
 }
 
+// FP32 y transforms of length 2M have a 32-column block (1024 threads at most, 144 KB of LDS)
+template <int M, int T = line_threads(M, 1)>
+constexpr bool line_tiled32()
+{
+    return T > 0 && 32 * T <= 1024 && 32 * line_slots<M>() * 2 * (int)sizeof(float) <= 144 * 1024;
+}
+
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 hipError_t launch_line_blocks(const LineArgs &a, hipStream_t s)
 {
@@ -757,6 +767,18 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
         // run faster
         if constexpr (AX == 1 && sizeof(R) == 8 && T >= 64 && LINES == 16) {
             if (!a.ztiled && !a.items) return launch_line_blocks<TIn, TOut, R, M, T, LINES / 2, AX, FWD, KIND>(a, s);
+        }
+        // FP32 z tiles are 32 columns wide (128-byte rows): 32-column blocks of up to 1024
+        // threads write or read whole tile rows
+        if constexpr (AX == 1 && sizeof(R) == 4 && line_tiled32<M, T>()) {
+            if (a.ztiled && a.tile_w == 32 && !a.items) {
+                // alternative: half the threads per column, twice the elements (one radix pass
+                // and one LDS exchange fewer, 512-thread blocks)
+                if constexpr (T % 2 == 0 && plan_ok(M, 2 * (M / T))) {
+                    if (a.variant) return launch_line_blocks<TIn, TOut, R, M, T / 2, 32, AX, FWD, KIND>(a, s);
+                }
+                return launch_line_blocks<TIn, TOut, R, M, T, 32, AX, FWD, KIND>(a, s);
+            }
         }
         return launch_line_blocks<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, s);
     }
@@ -816,6 +838,37 @@ inline bool line_compiled(int n) { return n % 2 == 0 && line_threads(n / 2, 0) >
 
 // the FP64 transforms of length N have an alternative launch shape
 inline bool line_has_variant(int n) { return line_compiled(n) && line_threads_alt(n / 2) > 0; }
+
+// the FP32 y transform of length N can write / read 32-column z tiles in whole rows
+inline bool line_tiled32_f32(int N)
+{
+    switch (line_compiled(N) ? N / 2 : 0) {
+#define PPM(m) \
+    case m: return line_tiled32<m>();
+        PP_FFT_LENGTHS(PPM)
+#undef PPM
+    default: return false;
+    }
+}
+
+template <int M>
+constexpr bool tiled32_has_variant()
+{
+    constexpr int T = line_threads(M, 1);
+    return line_tiled32<M>() && T % 2 == 0 && plan_ok(M, 2 * (M / (T > 0 ? T : 1)));
+}
+
+// ... and has the alternative shape (half the threads per column) for those tiles
+inline bool line_tiled32_f32_variant(int N)
+{
+    switch (line_compiled(N) ? N / 2 : 0) {
+#define PPM(m) \
+    case m: return tiled32_has_variant<m>();
+        PP_FFT_LENGTHS(PPM)
+#undef PPM
+    default: return false;
+    }
+}
 
 // lines (x) or columns (y) per block of the launch for line length N, precision R
 template <typename R>

@@ -271,14 +271,15 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
             q.hy2 = q.hy;
         }
     };
-    // software pipelined march: the loads of plane z + 1 are issued before plane z is
-    // computed and stored, so their latency overlaps this plane's arithmetic
+    // software pipelined march: the loads of planes z + 1 and z + 2 are in flight while plane
+    // z is computed and stored, so two planes of latency overlap this plane's arithmetic
     const int zend = min(z0 + RES_ZC, a.nz);
-    Plane cur;
+    Plane cur, nxt;
     fetch(z0, cur);
+    fetch(min(z0 + 1, zend - 1), nxt);
     for (int z = z0; z < zend; z++) {
-        Plane nxt;
-        fetch(min(z + 1, zend - 1), nxt);
+        Plane nx2;
+        fetch(min(z + 2, zend - 1), nx2);
         const int b = (z - z0) & 1;
         tile[b][w][lane] = uc;
         __syncthreads();  // one barrier per plane: the other buffer was last read before it
@@ -306,6 +307,7 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
         um = uc;
         uc = up;
         cur = nxt;
+        nxt = nx2;
     }
 }
 

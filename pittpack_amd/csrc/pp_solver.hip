@@ -168,6 +168,8 @@ int stage_var(const pp_solver *h, int i, int N)
 void free_mixed(pp_solver *h)
 {
     if (h->d_w) (void)hipFree(h->d_w);
+    if (h->d_wt) (void)hipFree(h->d_wt);
+    h->d_wt = nullptr;
     if (h->d_aux) (void)hipFree(h->d_aux);
     if (h->d_line_sums) (void)hipFree(h->d_line_sums);
     if (h->d_plane_add) (void)hipFree(h->d_plane_add);
@@ -621,11 +623,36 @@ ZArgs z_args(pp_solver *h, void *data)
     return z;
 }
 
+// FP32 z tiles of the mixed form: the y transforms write and read 32-column tiles (128-byte
+// rows, whole rows per block) of 16 J rows, so the FP32 z solve streams contiguous tiles
+// instead of one 128-byte row from each of nz planes.  Compiled y lengths whose 32-column
+// block fits, columns up to 512 rows (the FP64 re-solve of near-singular columns works on
+// tiles up to there).
+bool use_ztiles_f32(const pp_solver *h)
+{
+    return h->ztiles && h->wtiles && !h->g.zper && line_tiled32_f32(h->g.ny) && zfix_tiles_supported(h->g.nz);
+}
+
+// y shape of the FP32 tiled stages: PP_OPT_LINE_VARIANT when forced, else the tuner's
+int mixed_yvar(const pp_solver *h) { return h->force_var >= 0 ? h->force_var : h->wvar; }
+
+int ztile_rows_f32(int nz) { return (nz + 15) / 16 * 16; }
+
 // Buffers and the near-singular column list of the mixed form.  The list uses the same
 // eigenvalue arithmetic as the kernels (no contraction) so both sides agree on it.
 int ensure_mixed(pp_solver *h)
 {
     if (h->mixed_ready) return PP_SUCCESS;
+    if (use_ztiles_f32(h) && !h->d_wt) {
+        const size_t elems = ((size_t)h->g.nx + 31) / 32 * 32 * h->g.ny * ztile_rows_f32(h->g.nz);
+        if (hipMalloc((void **)&h->d_wt, elems * sizeof(float)) != hipSuccess) {
+            h->d_wt = nullptr;
+            return PP_ALLOCATION_FAIL;
+        }
+        // padding rows and columns are never written by the transforms; zero keeps them finite
+        // through the z solve (identity rows)
+        PP_HIP_CHECK(hipMemset(h->d_wt, 0, elems * sizeof(float)));
+    }
     // size-dependent buffers: allocated once per handle (sizes are fixed at pp_create) and
     // freed only by pp_destroy, so no pointer handed to a solve can be released under it
     const size_t N = (size_t)h->g.nx * h->g.ny * h->g.nz;
@@ -712,15 +739,29 @@ int mixed_inner(pp_solver *h)
 {
     hipStream_t s = h->stream;
     int rc;
+    const bool zt = use_ztiles_f32(h) && h->d_wt;
+    float *zdata = zt ? h->d_wt : h->d_w;
+    const int trows = ztile_rows_f32(h->g.nz);
     {
         LineArgs a = line_args(h, 1, nullptr);
         a.in = h->d_w;
-        a.out = h->d_w;
+        a.out = zdata;
+        if (zt) {
+            a.ztiled = ZT_OUT;
+            a.tnz = trows;
+            a.tile_w = 32;
+            a.variant = mixed_yvar(h);
+        }
         PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
     {
-        ZArgs z = z_args(h, h->d_w);
+        ZArgs z = z_args(h, zdata);
+        if (zt) {
+            z.tiled = 1;
+            z.tile_w = 32;
+            z.tile_rows = trows;
+        }
         z.tau = h->tau;
         z.pinned_zero = 1;
         PP_HIP_CHECK(launch_zsolve_pcr_f32(h->g.nz, z, s));
@@ -738,8 +779,14 @@ int mixed_inner(pp_solver *h)
     if ((rc = mark(h))) return rc;
     {
         LineArgs a = line_args(h, 1, nullptr);
-        a.in = h->d_w;
+        a.in = zdata;
         a.out = h->d_w;
+        if (zt) {
+            a.ztiled = ZT_IN;
+            a.tnz = trows;
+            a.tile_w = 32;
+            a.variant = mixed_yvar(h);
+        }
         PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[1], h->g.ny, 1, 0, a, s));
     }
     return mark(h);
@@ -833,6 +880,81 @@ int run_mixed(pp_solver *h, const double *f, double *u)
     return PP_SUCCESS;
 }
 
+// Autotuner of the mixed form: FP32 z tiles between the y transforms and the z solve or the
+// work field in place, and with tiles the FP32 y shape (32 threads or half of them per
+// column).  Whole mixed solves on scratch fields, best of three, timed round-robin; the
+// default (tiles, default shape) is kept unless another is more than 0.5% faster.  Cached per
+// process and shape.
+std::map<std::array<int, 7>, std::array<int, 2>> g_mtune;
+
+int autotune_mixed(pp_solver *h)
+{
+    const std::array<int, 7> key = {h->g.nx, h->g.ny, h->g.nz, h->g.kind[0], h->g.kind[1], h->g.zlo_d, h->g.zhi_d};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_mtune.find(key);
+        if (it != g_mtune.end()) {
+            h->wtiles = it->second[0];
+            h->wvar = it->second[1];
+            h->mtuned = true;
+            return PP_SUCCESS;
+        }
+    }
+    h->wtiles = 1;
+    h->wvar = 0;
+    std::vector<std::array<int, 2>> cand = {{1, 0}};
+    if (use_ztiles_f32(h)) {
+        if (line_tiled32_f32_variant(h->g.ny)) cand.push_back({1, 1});
+        cand.push_back({0, 0});
+    }
+    const size_t n = (size_t)h->g.nx * h->g.ny * h->g.nz;
+    double *f = nullptr, *u = nullptr;
+    if (cand.size() < 2 || hipMalloc((void **)&f, n * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&u, n * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (f) (void)hipFree(f);
+        h->mtuned = true;  // nothing to choose, or no room for scratch fields: the default
+        return PP_SUCCESS;
+    }
+    hipEvent_t e0, e1;
+    int rc = PP_SUCCESS;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PP_HIP_ERROR;
+    if (!rc && launch_fill_uniform(f, (long)n, 0x6d74ull, 0, h->stream) != hipSuccess) rc = PP_HIP_ERROR;
+    std::vector<double> best(cand.size(), 1e30);
+    for (int round = 0; round < 4 && !rc; round++)
+        for (size_t c = 0; c < cand.size() && !rc; c++) {
+            h->wtiles = cand[c][0];
+            h->wvar = cand[c][1];
+            h->mixed_ready = false;  // the tile buffer follows the choice (set up untimed)
+            if ((rc = ensure_mixed(h))) break;
+            if (round == 0 && (rc = run_mixed(h, f, u))) break;  // warm-up
+            if (hipEventRecord(e0, h->stream) != hipSuccess) rc = PP_HIP_ERROR;
+            if (!rc) rc = run_mixed(h, f, u);
+            if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                rc = PP_HIP_ERROR;
+            float t = 0.f;
+            if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = PP_HIP_ERROR;
+            best[c] = std::min(best[c], (double)t);
+        }
+    size_t bc = 0;
+    for (size_t c = 1; c < cand.size(); c++)
+        if (best[c] < best[bc]) bc = c;
+    if (best[bc] > 0.995 * best[0]) bc = 0;
+    h->wtiles = cand[bc][0];
+    h->wvar = cand[bc][1];
+    h->mixed_ready = false;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(f);
+    (void)hipFree(u);
+    if (rc) return rc;
+    h->mtuned = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_mtune[key] = cand[bc];
+    return PP_SUCCESS;
+}
+
 // dispatch on the precision form; rhs and out may alias (the mixed form then works on a copy)
 int run_solve(pp_solver *h, const double *rhs, double *out);
 
@@ -842,7 +964,8 @@ int run_solve(pp_solver *h, const double *rhs, double *out);
 int solve_graphed(pp_solver *h, const double *rhs, double *out)
 {
     const bool graphable = h->graph_mode && !h->timing && h->stream != nullptr &&
-                           !(h->precision == 0 && h->autotune && !h->tuned) && !(h->precision == 1 && rhs == out);
+                           !(h->precision == 0 && h->autotune && !h->tuned) &&
+                           !(h->precision == 1 && h->autotune && !h->mtuned) && !(h->precision == 1 && rhs == out);
     if (!graphable) {
         if (h->graph) drop_graph(h);
         return run_solve(h, rhs, out);
@@ -887,6 +1010,10 @@ int run_solve(pp_solver *h, const double *rhs, double *out)
             if (rc) return rc;
         }
         return run_fp64(h, rhs, out);
+    }
+    if (h->autotune && !h->mtuned) {
+        const int rc = autotune_mixed(h);
+        if (rc) return rc;
     }
     if (rhs == out) {
         const size_t bytes = (size_t)h->g.nx * h->g.ny * h->g.nz * sizeof(double);
@@ -1092,13 +1219,19 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_ZLAYOUT:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->ztiles = (int)value;
+        h->mixed_ready = false;
         return PP_SUCCESS;
     case PP_OPT_AUTOTUNE:
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->autotune = (int)value;
         h->tuned = false;
-        if (!value)
+        h->mtuned = false;
+        if (!value) {
             for (int i = 0; i < 4; i++) h->line_var[i] = 0;
+            h->wtiles = 1;
+            h->wvar = 0;
+            h->mixed_ready = false;
+        }
         return PP_SUCCESS;
     case PP_OPT_LINE_VARIANT:
         if (value < -1 || value > 1) return PP_INPUT_ARGS;
@@ -1146,6 +1279,11 @@ int pp_get_tuning(pp_handle h, int *out, int n)
     out[1] = h->zsolver;
     out[2] = use_ztiles(h) ? 1 : 0;
     for (int i = 0; i < 4; i++) out[3 + i] = stage_var(h, i, i == 0 || i == 3 ? h->g.nx : h->g.ny);
+    if (n >= 9) {
+        // mixed form: FP32 z tiles in use, and the tiled y shape
+        out[7] = use_ztiles_f32(h) ? 1 : 0;
+        out[8] = out[7] && line_tiled32_f32_variant(h->g.ny) ? mixed_yvar(h) : 0;
+    }
     return PP_SUCCESS;
 }
 

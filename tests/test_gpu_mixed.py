@@ -187,3 +187,43 @@ def test_mixed_converges_to_stencil_solution(bc):
     assert np.abs(u64 - ref).max() <= 1e-14 * scale         # FP64 form: the reference's arithmetic
     assert np.abs(ref - exact).max() >= 3e-13 * scale        # ... which is off the exact solution
     assert np.abs(um - exact).max() <= 1e-13 * scale         # mixed form: the exact solution
+
+
+ZT_SHAPES = [(16, 64, 48), (40, 128, 96), (33, 48, 40), (200, 96, 64), (24, 512, 32)]
+
+
+@pytest.mark.parametrize("shape", ZT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", ["PPPPNN", "DDNDDD", "NNNNDD", "NDDNNN"])
+def test_mixed_ztile_layouts(bc, shape):
+    """the mixed form's FP32 z tiles (32-column y blocks, both y shapes) and the work field in
+    place: each meets the bar against the oracle"""
+    print(f"mixed z layouts {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=len(bc) + shape[1])
+    nz, ny, nx = shape
+    for zero_mode in (0, 1):
+        ref = oracle.solve(f, bc, zero_mode=zero_mode)
+        for zlayout, var in ((1, 0), (1, 1), (0, 0)):
+            got = pa.solve(f, bc, zero_mode=zero_mode, precision=1, zlayout=zlayout, line_variant=var)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err <= TOL, f"{bc} {shape} zm={zero_mode} zlayout={zlayout} variant={var}: {err:.3e}"
+
+
+def test_mixed_autotune():
+    """the mixed-form tuner picks a z layout / y shape and the result meets the bar"""
+    shape = (64, 128, 96)
+    f = prng.uniform_pm1(shape, seed=77)
+    ref = oracle.solve(f, "DDNDDD", zero_mode=1)
+    nz, ny, nx = shape
+    with pa.PoissonHIP(nx, ny, nz) as M:
+        M.set_option(pa.OPT_PRECISION, 1)
+        M.set_option(pa.OPT_AUTOTUNE, 1)
+        M.assignBoundary("DDNDDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.assignRhs(f)
+        M.pittPack()
+        u = M.getSolution()
+        t = M.tuning()
+    print("mixed tuning", t, flush=True)
+    assert t["mixed_ztiles"] in (0, 1) and t["mixed_y_variant"] in (0, 1)
+    assert np.abs(u - ref).max() <= TOL * np.abs(ref).max()

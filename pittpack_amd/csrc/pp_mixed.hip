@@ -15,6 +15,8 @@
 //                  closures the spectral solve diagonalises (P wrap, N mirror, D antisym).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pp_line.h"
 #include "pp_mixed.h"
 #include "pp_zsolve.h"
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(1024) k_mode00_batch(const Mode00Item *items, 
     mode00_body(it.line_sums, it.ny, it.nz, it.b0, it.bn, it.inv_scale, it.plane_add, it.upart, it.nparts, correct);
 }
 
-constexpr int RES_ZC = 32;  // planes marched per thread
+constexpr int RES_ZC = 30;  // planes marched per thread (a multiple of the 3-plane unroll)
 constexpr int RES_TX = 64;  // block tile: one 64-lane wave per row ...
 constexpr int RES_TY = 4;   // ... four rows
 
@@ -272,42 +274,71 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
         }
     };
     // software pipelined march: the loads of planes z + 1 and z + 2 are in flight while plane
-    // z is computed and stored, so two planes of latency overlap this plane's arithmetic
+    // z is computed and stored, so two planes of latency overlap this plane's arithmetic.  The
+    // loop body has no branch around a memory access (neighbours are read from LDS
+    // unconditionally and selected; a tile wholly inside the grid stores unconditionally), and
+    // a block of RES_ZC planes runs unrolled by three over three plane buffers, so no loaded
+    // register is copied (a copy would wait for its load) and the compiler can count the loads
+    // in flight instead of waiting for all of them.
     const int zend = min(z0 + RES_ZC, a.nz);
-    Plane cur, nxt;
-    fetch(z0, cur);
-    fetch(min(z0 + 1, zend - 1), nxt);
-    for (int z = z0; z < zend; z++) {
-        Plane nx2;
-        fetch(min(z + 2, zend - 1), nx2);
+    const bool full_tile = (bx + 1) * RES_TX <= a.nx && (by + 1) * RES_TY <= a.ny;
+    auto step = [&](int z, const Plane &cur, auto all_valid) {
+        constexpr bool ALLV = decltype(all_valid)::value;
         const int b = (z - z0) & 1;
         tile[b][w][lane] = uc;
         __syncthreads();  // one barrier per plane: the other buffer was last read before it
         const long o = sz * z;
+        const double t_xm = tile[b][w][lane > 0 ? lane - 1 : 0];
+        const double t_xp = tile[b][w][lane < 63 ? lane + 1 : 63];
+        const double t_ym = tile[b][w > 0 ? w - 1 : 0][lane];
+        const double t_yp = tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
         const double up = (z + 1 < a.nz || ztop_wrap) ? cur.up : (a.bc[5] == 'N' ? uc : -uc);
-        const double uxm = hxm ? gxm * cur.hx : tile[b][w][lane > 0 ? lane - 1 : 0];
-        const double uxp = hxp ? gxp * (bothx ? cur.hx2 : cur.hx) : tile[b][w][lane < 63 ? lane + 1 : 63];
-        const double uym = hym ? gym * cur.hy : tile[b][w > 0 ? w - 1 : 0][lane];
-        const double uyp = hyp ? gyp * (bothy ? cur.hy2 : cur.hy) : tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
+        const double uxm = hxm ? gxm * cur.hx : t_xm;
+        const double uxp = hxp ? gxp * (bothx ? cur.hx2 : cur.hx) : t_xp;
+        const double uym = hym ? gym * cur.hy : t_ym;
+        const double uyp = hyp ? gyp * (bothy ? cur.hy2 : cur.hy) : t_yp;
         // second differences as differences of first differences: neighbouring values are
         // close, so each first difference is exact (Sterbenz) and the rounding scales with the
-        // differences, not with u -- on cells with dz >> dx, dy the factors Cx, Cy are large
-        // and eps |u| Cy would otherwise set the floor the refinement converges to
+        // differences, not with u
         const double Lu = ((uxp - uc) - (uc - uxm)) * a.Cx + ((uyp - uc) - (uc - uym)) * a.Cy + ((up - uc) - (uc - um));
         // effective right-hand side: rhs * dz^2 minus the Dirichlet boundary term
         double v = cur.f;
         v = v * a.c3;
         v = v * a.c3;
-        if (cxy != 0.0) v = v - cxy * ezs[z - z0];
+        v = v - cxy * ezs[z - z0];  // cxy = 0 away from x / y Dirichlet faces
         if (z == 0) v = v - czlo;
         if (z == a.nz - 1) v = v - czhi;
         double res = v - Lu;
         if (z == 0) res += defect;
-        if (valid) r_p[o] = (TOut)res;
+        if (ALLV || valid) r_p[o] = (TOut)res;
         um = uc;
         uc = up;
-        cur = nxt;
-        nxt = nx2;
+    };
+    if (full_tile && zend - z0 == RES_ZC) {
+        Plane A, B, C;
+        fetch(z0, A);
+        fetch(z0 + 1, B);
+        // fully unrolled: no loop back edge, so the counts of loads in flight stay exact
+#pragma clang loop unroll(full)
+        for (int g = 0; g < RES_ZC / 3; g++) {
+            const int z = z0 + 3 * g;
+            fetch(min(z + 2, zend - 1), C);
+            step(z, A, std::true_type{});
+            fetch(min(z + 3, zend - 1), A);
+            step(z + 1, B, std::true_type{});
+            fetch(min(z + 4, zend - 1), B);
+            step(z + 2, C, std::true_type{});
+        }
+    } else {
+        // edge tiles and the last, shorter block of planes
+        Plane cur;
+        fetch(z0, cur);
+        for (int z = z0; z < zend; z++) {
+            Plane nxt;
+            fetch(min(z + 1, zend - 1), nxt);
+            step(z, cur, std::false_type{});
+            cur = nxt;
+        }
     }
 }
 

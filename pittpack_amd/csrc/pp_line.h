@@ -731,6 +731,13 @@ constexpr bool line_tiled32()
     return T > 0 && 32 * T <= 1024 && 32 * line_slots<M>() * 2 * (int)sizeof(float) <= 144 * 1024;
 }
 
+// ... only in the shape with half the threads per column (and twice the elements)
+template <int M, int T = line_threads(M, 1)>
+constexpr bool tiled32_half()
+{
+    return T > 0 && !line_tiled32<M, T>() && T % 2 == 0 && line_tiled32<M, T / 2>() && plan_ok(M, 2 * (M / T));
+}
+
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 hipError_t launch_line_blocks(const LineArgs &a, hipStream_t s)
 {
@@ -779,6 +786,11 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
                 }
                 return launch_line_blocks<TIn, TOut, R, M, T, 32, AX, FWD, KIND>(a, s);
             }
+        } else if constexpr (AX == 1 && sizeof(R) == 4 && tiled32_half<M, T>()) {
+            // lines too long for 32 columns of the default shape (1024 points): the
+            // half-thread shape is the 32-column block
+            if (a.ztiled && a.tile_w == 32 && !a.items)
+                return launch_line_blocks<TIn, TOut, R, M, T / 2, 32, AX, FWD, KIND>(a, s);
         }
         return launch_line_blocks<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, s);
     }
@@ -844,7 +856,7 @@ inline bool line_tiled32_f32(int N)
 {
     switch (line_compiled(N) ? N / 2 : 0) {
 #define PPM(m) \
-    case m: return line_tiled32<m>();
+    case m: return line_tiled32<m>() || tiled32_half<m>();
         PP_FFT_LENGTHS(PPM)
 #undef PPM
     default: return false;

@@ -57,8 +57,15 @@ __global__ void __launch_bounds__(256) k_zfix64(ZArgs a0, const int4 *cols)
     }
     double *col = zf;           // nz values
     double *cp = zf + a.nz;     // nz values
-    const long zs = (long)a.SX * a.SY;
-    const long off = (long)sc.y * a.SX + sc.x;
+    // element i of the column: in place (stride SX*SY) or in its z tile (stride W)
+    long zs = (long)a.SX * a.SY;
+    long off = (long)sc.y * a.SX + sc.x;
+    if (a.tiled) {
+        const int W = a.tile_w;
+        const int rows = a.tile_rows ? a.tile_rows : (a.nz + 15) / 16 * 16;
+        off = (((long)(sc.x / W) * a.SY + sc.y) * rows) * W + sc.x % W;
+        zs = W;
+    }
     R *data = reinterpret_cast<R *>(a.data);
     const int n = a.nz;
     for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = (double)data[off + (long)i * zs];

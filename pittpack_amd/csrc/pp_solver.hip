@@ -626,11 +626,12 @@ ZArgs z_args(pp_solver *h, void *data)
 // FP32 z tiles of the mixed form: the y transforms write and read 32-column tiles (128-byte
 // rows, whole rows per block) of 16 J rows, so the FP32 z solve streams contiguous tiles
 // instead of one 128-byte row from each of nz planes.  Compiled y lengths whose 32-column
-// block fits, columns up to 512 rows (the FP64 re-solve of near-singular columns works on
-// tiles up to there).
+// block fits (1024-point lines in their half-thread shape), columns up to 1024 rows (the FP64
+// re-solve of near-singular columns works on 16-column tile halves up to 512 rows and column
+// by column beyond).
 bool use_ztiles_f32(const pp_solver *h)
 {
-    return h->ztiles && h->wtiles && !h->g.zper && line_tiled32_f32(h->g.ny) && zfix_tiles_supported(h->g.nz);
+    return h->ztiles && h->wtiles && !h->g.zper && line_tiled32_f32(h->g.ny) && zsolve_supported_f32(h->g.nz);
 }
 
 // y shape of the FP32 tiled stages: PP_OPT_LINE_VARIANT when forced, else the tuner's

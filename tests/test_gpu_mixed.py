@@ -189,7 +189,7 @@ def test_mixed_converges_to_stencil_solution(bc):
     assert np.abs(um - exact).max() <= 1e-13 * scale         # mixed form: the exact solution
 
 
-ZT_SHAPES = [(16, 64, 48), (40, 128, 96), (33, 48, 40), (200, 96, 64), (24, 512, 32)]
+ZT_SHAPES = [(16, 64, 48), (40, 128, 96), (33, 48, 40), (200, 96, 64), (24, 512, 32), (700, 32, 48), (20, 1024, 32)]
 
 
 @pytest.mark.parametrize("shape", ZT_SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -200,12 +200,19 @@ def test_mixed_ztile_layouts(bc, shape):
     print(f"mixed z layouts {bc} {shape}", flush=True)
     f = prng.uniform_pm1(shape, seed=len(bc) + shape[1])
     nz, ny, nx = shape
+    # cubic cells: the refinement converges to the exact 7-point solution, which on cells much
+    # thinner than dz differs from the spectral solution by its eigenvalue rounding
+    # (test_mixed_converges_to_stencil_solution); with Neumann z faces the lowest modes of a
+    # long axis carry the bound 4 eps (n / pi)^2 of that rounding
+    h = 1.0 / max(shape)
+    box = [0, nx * h, 0, ny * h, 0, nz * h]
+    tol = TOL if "D" in bc[4:] else max(TOL, 4 * np.finfo(float).eps * (max(shape) / np.pi) ** 2)
     for zero_mode in (0, 1):
-        ref = oracle.solve(f, bc, zero_mode=zero_mode)
+        ref = oracle.solve(f, bc, box, zero_mode=zero_mode)
         for zlayout, var in ((1, 0), (1, 1), (0, 0)):
-            got = pa.solve(f, bc, zero_mode=zero_mode, precision=1, zlayout=zlayout, line_variant=var)
+            got = pa.solve(f, bc, box, zero_mode=zero_mode, precision=1, zlayout=zlayout, line_variant=var)
             err = np.abs(got - ref).max() / np.abs(ref).max()
-            assert err <= TOL, f"{bc} {shape} zm={zero_mode} zlayout={zlayout} variant={var}: {err:.3e}"
+            assert err <= tol, f"{bc} {shape} zm={zero_mode} zlayout={zlayout} variant={var}: {err:.3e}"
 
 
 def test_mixed_autotune():

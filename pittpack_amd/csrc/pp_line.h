@@ -210,6 +210,10 @@ struct LineAddr {
     long base, stride, tbase, tstride;
     bool valid;
     int ly, lz;  // x lines: the line's y and z
+    // y columns: base = ubase + ucol and tbase = tubase + tcol, split into the block's uniform
+    // offset and the lane's (small, non-negative) column offset
+    long ubase, tubase;
+    int ucol, tcol;
 };
 
 template <int LINES, int AX>
@@ -225,24 +229,28 @@ __device__ __forceinline__ LineAddr line_addr(const LineArgs &a, int N, int bx, 
         d.ly = (int)(line % a.ny);
         d.lz = (int)(line / a.ny);
     } else {
-        long col = (long)bx * LINES + l;
-        d.valid = col < a.ncols;
-        if (!d.valid) col = 0;
-        d.base = (long)by * N * a.ncols + col;
+        // columns past the grid read the last column (in bounds)
+        const int col0 = bx * LINES;
+        d.valid = col0 + l < a.ncols;
+        d.ubase = (long)by * N * a.ncols + col0;
+        d.ucol = d.valid ? l : a.ncols - 1 - col0;
+        d.base = d.ubase + d.ucol;
         d.stride = a.ncols;
         if (a.ztiled) {
             if (a.tile_w == 0 || a.tile_w == LINES) {
                 // tile (bx, n) holds rows z of LINES columns: ((bx * ny + n) * tnz + z) * LINES + l
-                d.tbase = ((long)bx * N * a.tnz + by) * LINES + l;
+                d.tubase = ((long)bx * N * a.tnz + by) * LINES;
                 d.tstride = (long)a.tnz * LINES;
             } else {
                 // tiles W = tile_w columns wide (a multiple of LINES): this block fills LINES of
                 // tile bx / (W / LINES); neighbouring blocks, which complete each row, are
                 // numbered onto one XCD (W * 8 bytes is one 128-byte line)
                 const int W = a.tile_w, sub = W / LINES;
-                d.tbase = ((long)(bx / sub) * N * a.tnz + by) * W + (bx % sub) * LINES + l;
+                d.tubase = ((long)(bx / sub) * N * a.tnz + by) * W + (bx % sub) * LINES;
                 d.tstride = (long)a.tnz * W;
             }
+            d.tcol = l;
+            d.tbase = d.tubase + d.tcol;
         }
     }
     return d;
@@ -272,40 +280,93 @@ __device__ __forceinline__ DirLine dirichlet_line(const LineArgs &a, int ly, int
     return d;
 }
 
-template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
-// power-of-two lengths from 512 (8 elements per thread): at least 4 waves per SIMD (<= 128
-// VGPRs), so two 512-thread blocks fit a CU; other lengths keep their registers, and so does
-// the FP32 -> FP64 x inverse of 1024-point lines (its prefetched solution would spill)
-__global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
-                                            !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512))
-                                               ? 4
-                                               : 1) k_line(LineArgs a0)
+// Inputs are held in their own precision (FP32 inputs are only ever scaled by 1): half the
+// registers for the FP32 forms
+template <typename TIn>
+using line_xt = typename std::conditional<std::is_same<TIn, float>::value, float, double>::type;
+
+// y forward and y quarter-wave (either direction): lanes are columns, so every row is a
+// coalesced load and each thread reads the rows of its own FFT inputs directly (real view
+// position p = 2 (t + T m) + c % 2), no LDS scatter; otherwise element n = t + T c,
+// scattered through LDS after the load
+template <int AX, int FWD, int KIND>
+constexpr bool line_direct_in()
 {
-    LineArgs a = a0;
-    int bx, by;
-    line_block<TIn, TOut, R, LINES, AX>(a0, a, bx, by);
+    return (AX == 1) && (FWD || KIND == KIND_ND || KIND == KIND_DN);
+}
+
+// Minimum waves per SIMD a line kernel is compiled for (its register bound).  Power-of-two
+// lengths from 512 (8 elements per thread): at least 4 (<= 128 VGPRs), so two 512-thread
+// blocks fit a CU; other lengths keep their registers, and so does the FP32 -> FP64 x inverse
+// of 1024-point lines (its prefetched solution would spill).  FP32 y blocks of 1024 threads
+// whose LDS slice fits a CU twice: 8 (<= 64 VGPRs), so two blocks share a CU and one
+// block's loads overlap the other's transform.
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD>
+constexpr int line_min_waves()
+{
+    if (AX == 1 && sizeof(R) == 4 && T * LINES == 1024 && 2 * LINES * line_slots<M>() * 8 <= 160 * 1024) return 8;
+    return (M / T == 8 && M >= 256 && !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512)) ? 4 : 1;
+}
+
+// Issue the coalesced global loads of one line (column) of a block
+template <typename TIn, typename R, int M, int T, int AX, int FWD, int KIND>
+__device__ __forceinline__ void line_load(const LineArgs &a, const LineAddr &ad, int t, line_xt<TIn> (&xin)[2 * (M / T)])
+{
     constexpr int N = 2 * M;
     constexpr int E = M / T;
-    constexpr int LS = line_slots<M>();
-    constexpr bool SYNC = true;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
-
-    const int tid = threadIdx.x;
-    int l, t;
-    if constexpr (AX == 0) {
-        t = tid % T;
-        l = tid / T;
-    } else {
-        l = tid % LINES;
-        t = tid / LINES;
-    }
-    cpx<R> *L = lds + l * LS;
-
     const TIn *in = reinterpret_cast<const TIn *>(a.in);
-    TOut *out = reinterpret_cast<TOut *>(a.out);
+    if constexpr (AX == 1) {
+        // element n(c, t) is affine in the thread's row t: the address is a block-uniform
+        // pointer (row n(c, t) at its smallest over t, in scalar registers) plus a 32-bit lane
+        // byte offset (< 2^28 for lines up to 1024 and strides up to 32768), so no 64-bit
+        // address per load is held in vector registers.  The Makhoul order of NN / DD inputs
+        // read directly with an odd element count per thread is only piecewise affine (its
+        // two halves meet inside a row of threads): those rows take row 0 as the base.
+        constexpr bool AFFINE = !(line_direct_in<AX, FWD, KIND>() && (KIND == KIND_NN || KIND == KIND_DD) && E % 2);
+        const bool tin = a.ztiled & ZT_IN;
+        const long ub = tin ? ad.tubase : ad.ubase, is = tin ? ad.tstride : ad.stride;
+        const int lc = tin ? ad.tcol : ad.ucol;
+        auto row = [](int c, int tt) {
+            int n = tt + T * c;
+            if constexpr (line_direct_in<AX, FWD, KIND>()) {
+                bool ng;
+                n = pos_elem<N, KIND>(2 * (tt + T * (c >> 1)) + (c & 1), ng);
+            }
+            return n;
+        };
+#pragma unroll
+        for (int c = 0; c < 2 * E; c++) {
+            const int nmin = AFFINE ? min(row(c, 0), row(c, T - 1)) : 0;
+            const TIn *pu = in + ub + (long)nmin * is;
+            const unsigned lane_bytes = (unsigned)(((row(c, t) - nmin) * (int)is + lc) * (int)sizeof(TIn));
+            xin[c] = (line_xt<TIn>)*reinterpret_cast<const TIn *>(reinterpret_cast<const char *>(pu) + lane_bytes);
+        }
+        return;
+    }
+    const long ib = ad.base, is = ad.stride;
+#pragma unroll
+    for (int c = 0; c < 2 * E; c++) {
+        // invalid lanes read line 0 (in bounds): lines are transformed independently and
+        // their results are not stored, so no branch or select per load
+        int n = t + T * c;
+        if constexpr (line_direct_in<AX, FWD, KIND>()) {
+            bool ng;
+            n = pos_elem<N, KIND>(2 * (t + T * (c >> 1)) + (c & 1), ng);
+        }
+        xin[c] = (line_xt<TIn>)in[ib + (long)n * is];
+    }
+}
 
-    const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
+// Transform one line (column) whose inputs are loaded, and store it: L is the line's LDS
+// slice, bx the block's line index (x line sums)
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+__device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad, int t, int l, int bx, cpx<R> *L,
+                                          line_xt<TIn> (&xin)[2 * (M / T)])
+{
+    constexpr int N = 2 * M;
+    constexpr int E = M / T;
+    constexpr bool SYNC = true;
+    TOut *out = reinterpret_cast<TOut *>(a.out);
     const long base = ad.base, stride = ad.stride, tbase = ad.tbase, tstride = ad.tstride;
     const bool valid = ad.valid;
     const int lz = ad.lz;
@@ -320,32 +381,9 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
         if (AX == 1 && (a.ztiled & ZT_OUT)) out[tbase + (long)n * tstride] = (TOut)val;
         else out[base + (long)n * stride] = (TOut)val;
     };
+    constexpr bool DIRECT_IN = line_direct_in<AX, FWD, KIND>();
+    using XT = line_xt<TIn>;
 
-    // ---- issue the coalesced global loads first ----
-    // y forward and y quarter-wave (either direction): lanes are columns, so every row is a
-    // coalesced load and each thread reads the rows of its own FFT inputs directly (real view
-    // position p = 2 (t + T m) + c % 2), no LDS scatter; otherwise element n = t + T c,
-    // scattered through LDS below
-    constexpr bool DIRECT_IN = (AX == 1) && (FWD || KIND == KIND_ND || KIND == KIND_DN);
-    // inputs held in their own precision (FP32 inputs are only ever scaled by 1): half the
-    // registers for the FP32 forms
-    using XT = typename std::conditional<std::is_same<TIn, float>::value, float, double>::type;
-    XT xin[2 * E];
-    {
-        const bool tin = (AX == 1) && (a.ztiled & ZT_IN);
-        const long ib = tin ? tbase : base, is = tin ? tstride : stride;
-#pragma unroll
-        for (int c = 0; c < 2 * E; c++) {
-            // invalid lanes read line 0 (in bounds): lines are transformed independently and
-            // their results are not stored, so no branch or select per load
-            int n = t + T * c;
-            if constexpr (DIRECT_IN) {
-                bool ng;
-                n = pos_elem<N, KIND>(2 * (t + T * (c >> 1)) + (c & 1), ng);
-            }
-            xin[c] = (XT)in[ib + (long)n * is];
-        }
-    }
     // x inverse: the solution being accumulated into and the plane term are read here, with
     // the input, so their latency overlaps the transform instead of sitting before each store
     // (FP32 -> FP64 form only: the FP64 form never accumulates, and holding 2E more doubles
@@ -648,6 +686,34 @@ __global__ void __launch_bounds__(T *LINES, (M / T == 8 && M >= 256 &&
     }
 }
 
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+__global__ void __launch_bounds__(T *LINES, (line_min_waves<TIn, TOut, R, M, T, LINES, AX, FWD>())) k_line(LineArgs a0)
+{
+    LineArgs a = a0;
+    int bx, by;
+    line_block<TIn, TOut, R, LINES, AX>(a0, a, bx, by);
+    constexpr int N = 2 * M;
+    constexpr int E = M / T;
+    constexpr int LS = line_slots<M>();
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
+
+    const int tid = threadIdx.x;
+    int l, t;
+    if constexpr (AX == 0) {
+        t = tid % T;
+        l = tid / T;
+    } else {
+        l = tid % LINES;
+        t = tid / LINES;
+    }
+    cpx<R> *L = lds + l * LS;
+    const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
+    line_xt<TIn> xin[2 * E];
+    line_load<TIn, R, M, T, AX, FWD, KIND>(a, ad, t, xin);
+    line_body<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, ad, t, l, bx, L, xin);
+}
+
 // ---------------------------------------------------------------------------------------
 // launch configuration per (R, M, AX)
 // ---------------------------------------------------------------------------------------
@@ -770,8 +836,10 @@ hipError_t launch_line_cfg(const LineArgs &a, hipStream_t s)
         // y keeps the default's columns per block in either shape (the z-tile width)
         constexpr int LINES = (V && AX == 1) ? line_lines<R, M, AX>() : line_lines<R, M, AX, T>();
         // the 16-column FP64 blocks of 1024-point y lines (one block per CU) pay off where they
-        // write or read whole z-tile rows; on the field in place two 8-column blocks per CU
-        // run faster
+        // write or read whole z-tile rows (two 8-column blocks per CU, each writing half of
+        // every tile row, and the same block loading a second plane's columns while it
+        // transforms the first both measured slower at 1024^3); on the field in place two
+        // 8-column blocks per CU run faster
         if constexpr (AX == 1 && sizeof(R) == 8 && T >= 64 && LINES == 16) {
             if (!a.ztiled && !a.items) return launch_line_blocks<TIn, TOut, R, M, T, LINES / 2, AX, FWD, KIND>(a, s);
         }

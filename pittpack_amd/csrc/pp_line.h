@@ -358,9 +358,9 @@ __device__ __forceinline__ void line_load(const LineArgs &a, const LineAddr &ad,
 }
 
 // Transform one line (column) whose inputs are loaded, and store it: L is the line's LDS
-// slice, bx the block's line index (x line sums)
+// slice
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
-__device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad, int t, int l, int bx, cpx<R> *L,
+__device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad, int t, cpx<R> *L,
                                           line_xt<TIn> (&xin)[2 * (M / T)])
 {
     constexpr int N = 2 * M;
@@ -457,7 +457,7 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
             // the T threads of a line are adjacent lanes of one wave: butterfly reduction
 #pragma unroll
             for (int off = T / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, T);
-            if (t == 0 && valid) a.line_sums[(long)bx * LINES + l] = lsum;
+            if (t == 0 && valid) a.line_sums[(long)ad.lz * a.ny + ad.ly] = lsum;
         }
     }
     (void)lsum;
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(T *LINES, (line_min_waves<TIn, TOut, R, M, T, 
     const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
     line_xt<TIn> xin[2 * E];
     line_load<TIn, R, M, T, AX, FWD, KIND>(a, ad, t, xin);
-    line_body<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, ad, t, l, bx, L, xin);
+    line_body<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, ad, t, L, xin);
 }
 
 // ---------------------------------------------------------------------------------------

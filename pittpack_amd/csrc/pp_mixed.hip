@@ -1,9 +1,10 @@
 // pp_mixed.hip -- kernels of the mixed-precision form: FP32 transforms and z solves, with
 // FP64 where FP32 cannot deliver FP64 accuracy after refinement.
 //
-//  * k_zfix64    : FP64 Thomas for the near-singular (kx, ky) columns (|eig+2| < tau); the
-//                  FP32 z solve leaves those columns untouched.  Same algebra as
-//                  thomasLowMem without the pinning branch (src/triDiag.cpp:1693-1822).
+//  * k_zfix64    : FP64 solve of the near-singular (kx, ky) columns (|eig+2| < tau) one
+//                  column per block; the FP32 z solve leaves those columns untouched.  The
+//                  system is thomasLowMem's without the pinning branch (src/triDiag.cpp:
+//                  1693-1822), solved by parallel cyclic reduction in LDS.
 //  * k_mode00    : exact FP64 path of the pinned (0,0) column (x and y periodic or Neumann,
 //                  z Neumann-Neumann, or the reference row-0 rule): the column's right-hand
 //                  side is the xy-sum of the effective RHS per plane, the pinned system
@@ -55,8 +56,8 @@ __global__ void __launch_bounds__(256) k_zfix64(ZArgs a0, const int4 *cols)
         a.lamx = sv.lamx;
         a.lamy = sv.lamy;
     }
-    double *col = zf;           // nz values
-    double *cp = zf + a.nz;     // nz values
+    // the column's rows in LDS: a, b, c, d (nz each)
+    double *ra = zf, *rb = zf + a.nz, *rc = zf + 2 * a.nz, *rd = zf + 3 * a.nz;
     // element i of the column: in place (stride SX*SY) or in its z tile (stride W)
     long zs = (long)a.SX * a.SY;
     long off = (long)sc.y * a.SX + sc.x;
@@ -68,26 +69,49 @@ __global__ void __launch_bounds__(256) k_zfix64(ZArgs a0, const int4 *cols)
     }
     R *data = reinterpret_cast<R *>(a.data);
     const int n = a.nz;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = (double)data[off + (long)i * zs];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sc.x], a.Cx)), __dmul_rn(a.lamy[sc.y], a.Cy));
-        const double b0 = a.zlo_dirichlet ? eig - 1. : eig + 1.;
-        const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
-        Row3d r = zrow(0, n, eig, b0, bn);
-        double inv = 1.0 / r.b;
-        cp[0] = r.c * inv;
-        col[0] = col[0] * inv;
-        for (int i = 1; i < n; i++) {
-            r = zrow(i, n, eig, b0, bn);
-            inv = 1.0 / (r.b - r.a * cp[i - 1]);
-            cp[i] = r.c * inv;
-            col[i] = (col[i] - r.a * col[i - 1]) * inv;
-        }
-        for (int i = n - 2; i >= 0; i--) col[i] = col[i] - cp[i] * col[i + 1];
+    const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sc.x], a.Cx)), __dmul_rn(a.lamy[sc.y], a.Cy));
+    const double b0 = a.zlo_dirichlet ? eig - 1. : eig + 1.;
+    const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const Row3d r = zrow(i, n, eig, b0, bn);
+        ra[i] = r.a;
+        rb[i] = r.b;
+        rc[i] = r.c;
+        rd[i] = (double)data[off + (long)i * zs];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) data[off + (long)i * zs] = (R)col[i];
+    // parallel cyclic reduction: step S eliminates rows i - S and i + S from row i (rows past
+    // either end are identity rows); after the step with S >= n / 2 every row stands alone
+    constexpr int RPT = 4;  // rows per thread (n <= 4 * 256)
+    for (int S = 1; S < n; S <<= 1) {
+        double na[RPT], nb[RPT], nc[RPT], nd[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const int i = threadIdx.x + k * (int)blockDim.x;
+            if (i < n) {
+                const bool lo = i - S >= 0, hi = i + S < n;
+                const double k1 = lo ? ra[i] / rb[i - S] : 0.0;
+                const double k2 = hi ? rc[i] / rb[i + S] : 0.0;
+                na[k] = lo ? -ra[i - S] * k1 : 0.0;
+                nc[k] = hi ? -rc[i + S] * k2 : 0.0;
+                nb[k] = rb[i] - (lo ? rc[i - S] * k1 : 0.0) - (hi ? ra[i + S] * k2 : 0.0);
+                nd[k] = rd[i] - (lo ? rd[i - S] * k1 : 0.0) - (hi ? rd[i + S] * k2 : 0.0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const int i = threadIdx.x + k * (int)blockDim.x;
+            if (i < n) {
+                ra[i] = na[k];
+                rb[i] = nb[k];
+                rc[i] = nc[k];
+                rd[i] = nd[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) data[off + (long)i * zs] = (R)(rd[i] / rb[i]);
 }
 
 // One block, FP64, for the pinned (0,0) column.  R(z) = sum_y line_sums[y + ny z] is the
@@ -352,7 +376,8 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
 hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s)
 {
     if (ncols == 0) return hipSuccess;
-    const size_t lds = 2 * (size_t)nz_max * sizeof(double);
+    if (nz_max > 4 * 256) return hipErrorInvalidValue;  // four rows per thread
+    const size_t lds = 4 * (size_t)nz_max * sizeof(double);
     hipLaunchKernelGGL(k_zfix64<float>, dim3(ncols), dim3(256), lds, s, a, cols);
     return hipGetLastError();
 }

@@ -66,7 +66,9 @@ enum {
     /* 0 (default): FP64 throughout.  1: mixed form -- FP32 transforms and z solves with
      * FP64 twiddles and eigenvalues, FP64 solves of the near-singular columns
      * (|eig+2| < tau), the pinned (0,0) column exactly in FP64, then PP_OPT_REFINE steps of
-     * FP64-residual refinement.  Columns up to 1024 rows (FP64: 512). */
+     * FP64-residual refinement.  Columns up to 1024 rows.  Periodic z: the circulant z
+     * solve computes in FP64 on the FP32 field, and the singular column follows the
+     * reference's averaged row 0 (shermanMorrisonThomasV1) from FP64 plane sums. */
     PP_OPT_PRECISION = 4,
     PP_OPT_REFINE = 5, /* refinement steps of the mixed form (default 2) */
     PP_OPT_TAU = 6,    /* pp_set_option_double: near-singular threshold (default 0.05) */
@@ -274,7 +276,9 @@ typedef struct pp_batch_s *pp_batch;
 
 int pp_batch_create(pp_batch *b);
 int pp_batch_destroy(pp_batch b);
-/* member with the reference's set-up arguments (sizes, bc[6], box X[6]); *id = its index */
+/* member with the reference's set-up arguments (sizes, bc[6], box X[6]); *id = its index.
+ * Any boundary combination the single-solve handle takes, periodic z included (FP64 members
+ * with periodic z keep their field in place instead of z tiles); columns up to 1024 rows. */
 int pp_batch_add(pp_batch b, int nx, int ny, int nz, const char *bc, const double *X, int *id);
 int pp_batch_finalize(pp_batch b);
 int pp_batch_count(pp_batch b, int *members, long long *points);

@@ -88,23 +88,28 @@ int BatchCore::add(int nx, int ny, int nz, const char *bc, const double *X, int 
 {
     if (!bc || !X || !id) return PP_INPUT_ARGS;
     if (planned) return PP_CALL_ORDER;
-    const bool zok = precision == 1 ? zsolve_supported_f32(nz) : zsolve_supported_f64(nz);
-    if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
-    if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
-    if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
-    // FP64 members hand the z solve 16-column tiles, which y blocks of 16 columns fill alone
-    // and narrower y blocks (a divisor of 16) fill together
-    if (precision == 0) {
-        const int ly = line_lines_any<double>(ny, 1);
-        if (ly <= 0 || zsolve_cpb<double>() % ly != 0) return PP_UNSUPPORTED_SIZE;
-    }
     BatchMember m;
     m.g.nx = nx;
     m.g.ny = ny;
     m.g.nz = nz;
+    const bool zper = bc[4] == 'P';
+    // periodic z: any length the single-solve handle takes (the mixed form from 3 rows);
+    // otherwise FP64 columns up to 1024 rows (513..1024 by the long-column kernel, one launch
+    // per member) and FP32 columns up to 1024
+    bool zok;
+    if (zper) zok = (zperiodic_supported(nz) || line_size_supported(nz)) && (precision == 0 || nz >= 3);
+    else zok = precision == 1 ? zsolve_supported_f32(nz) : (zsolve_supported_f64(nz) || zsolve_long_supported_f64(nz));
+    if (!line_size_supported(nx) || !line_size_supported(ny) || !zok) return PP_UNSUPPORTED_SIZE;
+    if (!(X[1] > X[0] && X[3] > X[2] && X[5] > X[4])) return PP_INPUT_ARGS;
+    if (16.0 * nx * ny * sizeof(double) >= 4294967296.0) return PP_UNSUPPORTED_SIZE;
+    // FP64 members hand the z solve 16-column tiles, which y blocks of 16 columns fill alone
+    // and narrower y blocks (a divisor of 16) fill together; periodic z stays in place
+    if (precision == 0 && !zper) {
+        const int ly = line_lines_any<double>(ny, 1);
+        if (ly <= 0 || zsolve_cpb<double>() % ly != 0) return PP_UNSUPPORTED_SIZE;
+    }
     const int rc = setup_boundary(m.g, bc);
     if (rc) return rc;
-    if (m.g.zper) return PP_UNSUPPORTED_BC;  // periodic z: single-solve handle only
     for (int i = 0; i < 6; i++) m.g.X[i] = X[i];
     setup_coords(m.g);
     *id = (int)mem.size();
@@ -163,10 +168,54 @@ int BatchCore::plan_lines(int axis, std::vector<HostClass> &out)
 
 void BatchCore::plan_z()
 {
-    std::map<int, std::vector<ZItem>> cls;
+    std::map<int, std::vector<ZItem>> cls, pcls;
     const int cpb = precision == 1 ? zsolve_cpb<float>() : zsolve_cpb<double>();
+    zpl_.clear();
+    zlong_.clear();
+    smt_.clear();
+    zc2_.clear();
+    zc2_off_.clear();
+    smt_nz_max_ = 0;
     for (int m = 0; m < (int)mem.size(); m++) {
         const SolveSetup &g = mem[m].g;
+        if (g.zper) {
+            if (!zc2_off_.count(g.nz)) {
+                zc2_off_[g.nz] = (long)zc2_.size();
+                const std::vector<double> c2 = zper_cos2(g.nz);
+                zc2_.insert(zc2_.end(), c2.begin(), c2.end());
+            }
+            if (zperiodic_supported(g.nz)) {
+                auto &items = pcls[g.nz];
+                const int lines = zper_lines(g.nz);
+                const int nbx = (g.nx + lines - 1) / lines;
+                for (int sy = 0; sy < g.ny; sy++)
+                    for (int bx = 0; bx < nbx; bx++) items.push_back({m, bx, sy});
+            } else {
+                zpl_.push_back(m);
+            }
+            // FP64: the singular column by the reference's branch (the mixed form solves it
+            // from plane sums)
+            if (precision == 0) {
+                const std::vector<double> lx = eig_factors(g, 0), ly = eig_factors(g, 1);
+                const double Cx = (g.dxyz[2] / g.dxyz[0]) * (g.dxyz[2] / g.dxyz[0]);
+                const double Cy = (g.dxyz[2] / g.dxyz[1]) * (g.dxyz[2] / g.dxyz[1]);
+                bool found = false;
+                for (int sy = 0; sy < g.ny && !found; sy++)
+                    for (int sx = 0; sx < g.nx && !found; sx++) {
+                        const double eig = column_eig(lx, ly, sx, sy, Cx, Cy);
+                        if (std::fabs(eig + 2.) < 1.e-10) {
+                            smt_.push_back({mem[m].offset, g.nx, g.ny, g.nz, sx, sy, eig});
+                            smt_nz_max_ = std::max(smt_nz_max_, g.nz);
+                            found = true;
+                        }
+                    }
+            }
+            continue;
+        }
+        if (precision == 0 && zsolve_long_supported_f64(g.nz)) {
+            zlong_.push_back(m);
+            continue;
+        }
         auto &items = cls[(g.nz + 15) / 16];
         const int nbx = (g.nx + cpb - 1) / cpb;
         for (int sy = 0; sy < g.ny; sy++)
@@ -179,6 +228,14 @@ void BatchCore::plan_z()
         c.nitems = (int)kv.second.size();
         c.zi = std::move(kv.second);
         lz_.push_back(std::move(c));
+    }
+    lzp_.clear();
+    for (auto &kv : pcls) {
+        HostClass c;
+        c.key = kv.first;
+        c.nitems = (int)kv.second.size();
+        c.zi = std::move(kv.second);
+        lzp_.push_back(std::move(c));
     }
 }
 
@@ -214,7 +271,8 @@ int BatchCore::plan_mixed()
                     m.pinned = true;
                     continue;
                 }
-                if (std::fabs(eig + 2.) < tau) fix_.push_back(make_int4(sx, sy, (int)i, 0));
+                // (periodic z computes every column in FP64: nothing to re-solve)
+                if (!g.zper && std::fabs(eig + 2.) < tau) fix_.push_back(make_int4(sx, sy, (int)i, 0));
             }
         if (m.pinned) {
             Mode00Item it{};
@@ -224,6 +282,7 @@ int BatchCore::plan_mixed()
             it.bn = g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
             it.inv_scale = 1. / g.scale;
             it.nparts = residual_parts(g.nx, g.ny);
+            it.zper = g.zper ? 1 : 0;
             m00_.push_back(it);
             m00_mem_.push_back((int)i);
         }
@@ -292,6 +351,8 @@ int BatchCore::plan()
     ntiles_ = 0;
     if (precision == 0) {
         for (BatchMember &m : mem) {
+            m.toff = -1;
+            if (m.g.zper) continue;  // periodic z stays in place
             m.toff = ntiles_;
             ntiles_ += (long)((m.g.nx + 15) / 16) * 16 * m.g.ny * ((m.g.nz + 15) / 16) * 16;
         }
@@ -305,7 +366,7 @@ int BatchCore::plan()
     plan_z();
     launches[0] = (int)lx_.size();
     launches[1] = (int)ly_.size();
-    launches[2] = (int)lz_.size();
+    launches[2] = (int)(lz_.size() + lzp_.size() + 3 * zpl_.size() + zlong_.size() + (smt_.empty() ? 0 : 1));
     ls_.assign(mem.size(), LineSolve{});
     zs_.assign(mem.size(), ZSolve{});
     // region layout
@@ -324,6 +385,9 @@ int BatchCore::plan()
             if (!h.li_alt.empty()) h.off_alt = take(c, h.li_alt.size() * sizeof(LineItem));
         }
     for (HostClass &h : lz_) h.off = take(c, h.zi.size() * sizeof(ZItem));
+    for (HostClass &h : lzp_) h.off = take(c, h.zi.size() * sizeof(ZItem));
+    o_zc2_ = take(c, zc2_.size() * sizeof(double));
+    o_smt_ = take(c, smt_.size() * sizeof(SmtItem));
     if (precision == 1) {
         o_sums_ = take(c, (size_t)nsums_ * sizeof(double));
         o_plane_ = take(c, (size_t)nplane_ * sizeof(double));
@@ -350,11 +414,13 @@ int BatchCore::bind(char *base, TableCache *tc)
     d_dtab_ = (double *)(base + o_dtab_);
     d_ls_ = (LineSolve *)(base + o_ls_);
     d_zs_ = (ZSolve *)(base + o_zs_);
-    for (auto *v : {&lx_, &ly_, &lz_})
+    for (auto *v : {&lx_, &ly_, &lz_, &lzp_})
         for (HostClass &h : *v) {
             h.d_items = base + h.off;
             h.d_items_alt = h.li_alt.empty() ? nullptr : base + h.off_alt;
         }
+    d_zc2_ = (double *)(base + o_zc2_);
+    d_smt_ = (SmtItem *)(base + o_smt_);
     int rc;
     for (auto *v : {&lx_, &ly_})
         for (HostClass &h : *v) {
@@ -362,6 +428,18 @@ int BatchCore::bind(char *base, TableCache *tc)
             if ((rc = tc->get(h.key, t))) return rc;
             tabs_[h.key] = t;
         }
+    // periodic z: the complex FFT of length nz uses the twiddles of real lines of 2 nz; other
+    // lengths take the real line tables of length nz
+    for (HostClass &h : lzp_) {
+        AxisDev t;
+        if ((rc = tc->get(2 * h.key, t))) return rc;
+        tabs_[2 * h.key] = t;
+    }
+    for (int m : zpl_) {
+        AxisDev t;
+        if ((rc = tc->get(mem[m].g.nz, t))) return rc;
+        tabs_[mem[m].g.nz] = t;
+    }
     if (precision == 1) {
         d_sums_ = (double *)(base + o_sums_);
         d_plane_ = (double *)(base + o_plane_);
@@ -387,9 +465,9 @@ int BatchCore::bind(char *base, TableCache *tc)
         l.sums_off = m.sums_off;
         l.plane_off = m.plane_off;
         l.toffset = m.toff;
-        l.tnz = (g.nz + 15) / 16 * 16;
+        l.tnz = (precision == 0 && m.toff < 0) ? 0 : (g.nz + 15) / 16 * 16;
         ZSolve &z = zs_[i];
-        z.offset = precision == 0 ? m.toff : m.offset;  // FP64 solves the z tiles
+        z.offset = (precision == 0 && m.toff >= 0) ? m.toff : m.offset;  // FP64 solves the z tiles
         z.SX = g.nx;
         z.SY = g.ny;
         z.nz = g.nz;
@@ -432,7 +510,7 @@ int BatchCore::upload(hipStream_t s)
     if (!bound) return PP_CALL_ORDER;
     int rc;
     // tile padding (rows from nz, columns from nx) is never written by the transforms
-    if (precision == 0) PPB_CHECK(hipMemsetAsync(d_tiles, 0, (size_t)ntiles_ * sizeof(double), s));
+    if (precision == 0 && ntiles_) PPB_CHECK(hipMemsetAsync(d_tiles, 0, (size_t)ntiles_ * sizeof(double), s));
     if ((rc = put(d_lam_, lam_, s)) || (rc = put(d_dtab_, dtab_, s)) || (rc = put(d_ls_, ls_, s)) ||
         (rc = put(d_zs_, zs_, s)))
         return rc;
@@ -441,8 +519,10 @@ int BatchCore::upload(hipStream_t s)
             if ((rc = put((LineItem *)h.d_items, h.li, s)) ||
                 (!h.li_alt.empty() && (rc = put((LineItem *)h.d_items_alt, h.li_alt, s))))
                 return rc;
-    for (HostClass &h : lz_)
-        if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
+    for (auto *v : {&lz_, &lzp_})
+        for (HostClass &h : *v)
+            if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
+    if ((rc = put(d_zc2_, zc2_, s)) || (rc = put(d_smt_, smt_, s))) return rc;
     if (precision == 1) {
         // members without a pinned column add zeros in the inverse-x store
         PPB_CHECK(hipMemsetAsync(d_plane_, 0, (size_t)nplane_ * sizeof(double), s));
@@ -573,6 +653,76 @@ int BatchCore::run_z_stage(hipStream_t s, bool f32)
             PPB_CHECK(launch_zsolve_pcr_f64(16 * c.key, z, s));
         }
     }
+    // FP64 columns of 513..1024 rows: the long-column kernel on the member's z tiles
+    for (int m : zlong_) {
+        const ZSolve &zs = zs_[m];
+        ZArgs z{};
+        z.data = d_tiles + zs.offset;
+        z.tiled = 1;
+        z.tile_w = zsolve_cpb<double>();
+        z.tile_rows = (zs.nz + 15) / 16 * 16;
+        z.SX = zs.SX;
+        z.SY = zs.SY;
+        z.nz = zs.nz;
+        z.lamx = d_lam_ + mem[m].lam_off[0];
+        z.lamy = d_lam_ + mem[m].lam_off[1];
+        z.Cx = zs.Cx;
+        z.Cy = zs.Cy;
+        z.zlo_dirichlet = zs.zlo_dirichlet;
+        z.zhi_dirichlet = zs.zhi_dirichlet;
+        z.zero_mode = zero_mode;
+        PPB_CHECK(launch_zsolve_long_f64(zs.nz, z, s));
+    }
+    // periodic z, in place on the solution arena (FP64) or the work arena (FP32 storage, FP64
+    // arithmetic, singular column zeroed)
+    void *pdata = f32 ? (void *)d_w : (void *)d_sol;
+    for (const HostClass &c : lzp_) {
+        ZPerArgs zp{};
+        zp.data = pdata;
+        zp.nz = c.key;
+        zp.tw = tabs_.at(2 * c.key).tw;
+        zp.c2 = d_zc2_ + zc2_off_.at(c.key);
+        zp.pinned_zero = f32 ? 1 : 0;
+        zp.items = (const ZItem *)c.d_items;
+        zp.solves = d_zs_;
+        PPB_CHECK(f32 ? launch_zperiodic_f32_batch(zp, c.nitems, s) : launch_zperiodic_f64_batch(zp, c.nitems, s));
+    }
+    for (int m : zpl_) {
+        const SolveSetup &g = mem[m].g;
+        const AxisDev &t = tabs_.at(g.nz);
+        const long off = mem[m].offset;
+        void *data = f32 ? (void *)(d_w + off) : (void *)(d_sol + off);
+        ZPerArgs zp{};
+        zp.data = data;
+        zp.SX = g.nx;
+        zp.SY = g.ny;
+        zp.nz = g.nz;
+        zp.lamx = d_lam_ + mem[m].lam_off[0];
+        zp.lamy = d_lam_ + mem[m].lam_off[1];
+        zp.Cx = zs_[m].Cx;
+        zp.Cy = zs_[m].Cy;
+        zp.c2 = d_zc2_ + zc2_off_.at(g.nz);
+        zp.pinned_zero = f32 ? 1 : 0;
+        // z lines as the columns of a [nz][ny*nx] array: R2HC, divide, HC2R
+        LineArgs a{};
+        a.in = data;
+        a.out = data;
+        a.tw = t.tw;
+        a.A = t.A;
+        a.B = t.B;
+        a.C = t.C;
+        a.D = t.D;
+        a.in_scale = 1.0;
+        a.out_scale = 1.0;
+        a.nlines = (long)g.nx * g.ny;
+        a.ncols = g.nx * g.ny;
+        a.nplanes = 1;
+        const int form = f32 ? FORM_F32 : FORM_F64;
+        PPB_CHECK(launch_line(form, KIND_PP, g.nz, 1, 1, a, s));
+        PPB_CHECK(f32 ? launch_zper_divide_f32(zp, s) : launch_zper_divide(zp, s));
+        PPB_CHECK(launch_line(form, KIND_PP, g.nz, 1, 0, a, s));
+    }
+    if (!f32) PPB_CHECK(launch_smt_v1_batch(d_sol, d_smt_, (int)smt_.size(), smt_nz_max_, s));
     return PP_SUCCESS;
 }
 

@@ -11,6 +11,7 @@
 #include <map>
 #include <vector>
 
+#include "pp_launch.h"
 #include "pp_line.h"
 #include "pp_mixed.h"
 #include "pp_setup.h"
@@ -45,7 +46,7 @@ struct BatchMember {
     long dtab_off = 0;
     bool pinned = false;  // mixed form: pinned (0,0) column solved in FP64 from plane sums
     long sums_off = 0, plane_off = 0, upart_off = 0;
-    long toff = 0;  // FP64: offset of the member's z tiles in the tile arena
+    long toff = 0;  // FP64: offset of the member's z tiles in the tile arena (-1: none)
 };
 
 class BatchCore {
@@ -117,6 +118,16 @@ class BatchCore {
     std::vector<LineSolve> ls_;
     std::vector<ZSolve> zs_;
     std::vector<HostClass> lx_, ly_, lz_;
+    // periodic z: one-pass classes (key nz, a compiled complex plan; items {member, column
+    // block, y row}), members of other lengths (real z-line transforms, one launch each), the
+    // singular columns of FP64 members, and 2 cos(2 pi k / nz) per distinct periodic length
+    std::vector<HostClass> lzp_;
+    std::vector<int> zpl_;
+    std::vector<SmtItem> smt_;
+    std::vector<double> zc2_;
+    std::map<int, long> zc2_off_;
+    // FP64 members with 513..1024-row columns: the long-column kernel, one launch each
+    std::vector<int> zlong_;
     std::vector<int4> fix_;
     std::vector<HostClass> lfix_;  // 16-column tiles holding near-singular columns, per J
     std::vector<Mode00Item> m00_;
@@ -127,7 +138,7 @@ class BatchCore {
     int nz_max_ = 0;
     long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)
-    size_t o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
+    size_t o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
            o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0;
     std::map<int, AxisDev> tabs_;
     // device pointers of the tables
@@ -135,6 +146,9 @@ class BatchCore {
     LineSolve *d_ls_ = nullptr;
     ZSolve *d_zs_ = nullptr;
     int4 *d_fix_ = nullptr, *d_ritems_ = nullptr;
+    double *d_zc2_ = nullptr;
+    SmtItem *d_smt_ = nullptr;
+    int smt_nz_max_ = 0;
 
     Mode00Item *d_m00_ = nullptr;
     ResArgs *d_res_ = nullptr;

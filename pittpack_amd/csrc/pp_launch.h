@@ -33,22 +33,43 @@ bool zsolve_supported_f32(int nz);
 bool zfix_tiles_supported(int nz);
 hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s);
 
-// periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64; the singular column by
-// the reference's own branch
+// periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64 arithmetic on FP64 or FP32
+// storage; the singular column by the reference's own branch
 struct ZPerArgs {
-    double *data;
+    void *data;  // [nz][SY][SX], in place; double (f64 launchers) or float (f32 launchers)
     int SX, SY, nz;
     const double *lamx, *lamy;
     double Cx, Cy;
     const double *tw;
     const double *c2;
+    // mixed form: the singular column is written as zeros (its contribution is solved from
+    // plane sums in FP64 elsewhere); otherwise it is left for the reference's branch
+    int pinned_zero;
+    // batched launch of the one-pass kernel (null: one solve, grid = column blocks x SY):
+    // block b works on columns bx * zper_lines(nz) ... of row sy of solve items[b].solve
+    // (items[b] = {solve, bx, sy}); every listed solve has length nz
+    const ZItem *items;
+    const ZSolve *solves;
 };
 // one-pass kernel: nz with a compiled complex plan (zperiodic_supported); other nz: real
 // z-line transforms around launch_zper_divide
 bool zperiodic_supported(int nz);
+int zper_lines(int nz);  // columns per block of the one-pass kernel
 hipError_t launch_zperiodic_f64(const ZPerArgs &a, hipStream_t s);
+hipError_t launch_zperiodic_f32(const ZPerArgs &a, hipStream_t s);
+// batched launches: grid = nitems
+hipError_t launch_zperiodic_f64_batch(const ZPerArgs &a, int nitems, hipStream_t s);
+hipError_t launch_zperiodic_f32_batch(const ZPerArgs &a, int nitems, hipStream_t s);
 hipError_t launch_zper_divide(const ZPerArgs &a, hipStream_t s);
+hipError_t launch_zper_divide_f32(const ZPerArgs &a, hipStream_t s);
 hipError_t launch_smt_v1(double *data, int SX, int SY, int nz, int sx, int sy, double diag, hipStream_t s);
+// the singular column of several solves (batched form): one block per item
+struct SmtItem {
+    long offset;  // element offset of the solve's field in the data arena
+    int SX, SY, nz, sx, sy;
+    double diag;
+};
+hipError_t launch_smt_v1_batch(double *data, const SmtItem *items, int nitems, int nz_max, hipStream_t s);
 
 // counter-generator right-hand side (pp_solver.hip): out[i] = 2 u - 1,
 // u = (splitmix64(seed * 2^32 + i) >> 11) * 2^-53, optionally rounded to FP32

@@ -45,7 +45,7 @@ struct LineSolve {
     long sums_off;   // offset of the solve's ny*nz line sums in line_sums
     long plane_off;  // offset of the solve's nz (+2) plane values in plane_add
     long toffset;    // z-tiled launches: offset of the solve's z tiles on the tiled side
-    int tnz;         // rows per z tile
+    int tnz;         // rows per z tile; 0: the solve has no z tiles (its field stays in place)
 };
 struct LineItem {
     int solve, bx, by;
@@ -179,8 +179,18 @@ __device__ __forceinline__ void line_block(const LineArgs &a0, LineArgs &a, int 
     if (a0.items) {
         const LineItem it = a0.items[blockIdx.x];
         const LineSolve sv = a0.solves[it.solve];
-        a.in = reinterpret_cast<const TIn *>(a0.in) + ((a0.ztiled & ZT_IN) ? sv.toffset : sv.offset);
-        a.out = reinterpret_cast<TOut *>(a0.out) + ((a0.ztiled & ZT_OUT) ? sv.toffset : sv.offset);
+        if (a0.ztiled && sv.tnz == 0) {
+            // a solve without z tiles (periodic z) keeps its field in place: the y transform
+            // reads and writes the launch's other arena (tiled launches are FP64 -> FP64)
+            TOut *f = (a0.ztiled & ZT_OUT) ? reinterpret_cast<TOut *>(const_cast<void *>(a0.in))
+                                           : reinterpret_cast<TOut *>(a0.out);
+            a.in = f + sv.offset;
+            a.out = f + sv.offset;
+            a.ztiled = 0;
+        } else {
+            a.in = reinterpret_cast<const TIn *>(a0.in) + ((a0.ztiled & ZT_IN) ? sv.toffset : sv.offset);
+            a.out = reinterpret_cast<TOut *>(a0.out) + ((a0.ztiled & ZT_OUT) ? sv.toffset : sv.offset);
+        }
         a.tnz = sv.tnz;
         a.nx = sv.nx;
         a.ny = sv.ny;

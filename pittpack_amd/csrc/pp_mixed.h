@@ -21,8 +21,8 @@ struct ResArgs {
     // the solution leaves a plane-constant residual d0/(NxNy) at z = 0 that belongs to no
     // other mode; it is removed here (null: no pinned column)
     const double *pin_defect;
-    // pinned (0,0) column: per-block partial sums of u over plane 0 (null: not wanted);
-    // residual_parts(nx, ny) entries
+    // pinned (0,0) column: per-block partial sums of u over plane 0, or over planes 0 and
+    // nz-1 for periodic z (null: not wanted); residual_parts(nx, ny) entries
     double *upart;
 };
 
@@ -43,6 +43,7 @@ struct Mode00Item {
     double *plane_add;
     const double *upart;
     int nparts;
+    int zper;  // periodic z: the reference's singular cyclic column (row 0 averaged)
 };
 
 // cols[i] = {sx, sy, member, 0}; nz_max bounds every listed column's length
@@ -53,7 +54,7 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
 bool residual_two_halo(int nx, int ny);
 hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s);
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
-                         double *plane_add, const double *upart, int nparts, int correct, hipStream_t s);
+                         double *plane_add, const double *upart, int nparts, int correct, int zper, hipStream_t s);
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s);
 
 }  // namespace pp

@@ -123,8 +123,14 @@ __global__ void __launch_bounds__(256) k_zfix64(ZArgs a0, const int4 *cols)
 //  correction (correct == 1): the refinement residual's plane sums give rows z >= 1 of
 //      P du = R; row 0 is S0 - (plane-0 sum of the current u), assembled from the residual
 //      kernel's per-block partial sums upart[0..nparts).  plane_add[z] = du(z)/(NxNy).
+// Periodic z (zper): the column is the reference's singular cyclic one
+// (shermanMorrisonThomasV1, src/triDiag.cpp:2112-2255), i.e. the cyclic operator with row 0
+// replaced by u0 + u_{N-1} = (r0 + r_{N-1})/2, solved by the same Sherman-Morrison sweeps as
+// k_smt_v1; plane_add[nz] = (u_{N-1} - 2 u0 + u1 - S0)/(NxNy) and plane_add[nz+1] = the
+// row-0 target (S0 + S_{N-1})/2; in a correction, row 0 is that target minus the sum of the
+// current u over planes 0 and N-1 (the residual kernel's partial sums take both planes).
 __device__ void mode00_body(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
-                            double *plane_add, const double *upart, int nparts, int correct)
+                            double *plane_add, const double *upart, int nparts, int correct, int zper)
 {
     extern __shared__ double m0[];
     double *r = m0;
@@ -144,7 +150,42 @@ __device__ void mode00_body(const double *line_sums, int ny, int nz, double b0, 
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && zper) {
+        const double s0 = r[0];
+        const double target = correct ? plane_add[nz + 1] : (r[0] + r[nz - 1]) * 0.5;
+        r[0] = correct ? target - red[0] : target;
+        // thomasLowMemNoBCV1 sweeps of shermanMorrisonThomasV1 (b = {1, -2, -2}, gamma = -1,
+        // alpha = beta = 1): row 0 decoupled with diagonal 2, interior 1 -2 1, last row -1
+        const double bb[3] = {2.0, -2.0, -1.0};
+        auto sweep = [&](double *x) {
+            double bet = bb[0];
+            x[0] = x[0] / bet;
+            cp[1] = 0.0;
+            bet = bb[1];
+            x[1] = (x[1] - x[0]) / bet;
+            for (int j = 2; j < nz - 1; j++) {
+                cp[j] = 1.0 / bet;
+                bet = bb[1] - cp[j];
+                x[j] = (x[j] - x[j - 1]) / bet;
+            }
+            cp[nz - 1] = 1.0 / bet;
+            bet = bb[2] - cp[nz - 1];
+            x[nz - 1] = (x[nz - 1] - x[nz - 2]) / bet;
+            for (int j = nz - 2; j >= 0; j--) x[j] -= cp[j + 1] * x[j + 1];
+        };
+        double *q = red;  // the Sherman-Morrison vector (the reduction is done with red)
+        sweep(r);
+        for (int i = 0; i < nz; i++) q[i] = 0.0;
+        q[0] = -1.0;
+        q[nz - 1] = 1.0;
+        sweep(q);
+        const double fact = (r[0] - r[nz - 1]) / (1. + q[0] - q[nz - 1]);
+        for (int i = 0; i < nz; i++) r[i] -= fact * q[i];
+        if (!correct) {
+            plane_add[nz] = (r[nz - 1] - 2.0 * r[0] + r[1] - s0) * inv_scale;
+            plane_add[nz + 1] = target;
+        }
+    } else if (threadIdx.x == 0) {
         const double s0 = correct ? plane_add[nz + 1] : r[0];
         if (correct) r[0] = s0 - red[0];
         cp[0] = 0.0;
@@ -167,16 +208,17 @@ __device__ void mode00_body(const double *line_sums, int ny, int nz, double b0, 
 
 __global__ void __launch_bounds__(1024) k_mode00(const double *line_sums, int ny, int nz, double b0, double bn,
                                                  double inv_scale, double *plane_add, const double *upart, int nparts,
-                                                 int correct)
+                                                 int correct, int zper)
 {
-    mode00_body(line_sums, ny, nz, b0, bn, inv_scale, plane_add, upart, nparts, correct);
+    mode00_body(line_sums, ny, nz, b0, bn, inv_scale, plane_add, upart, nparts, correct, zper);
 }
 
 // batched: block b handles pinned member items[b]
 __global__ void __launch_bounds__(1024) k_mode00_batch(const Mode00Item *items, int correct)
 {
     const Mode00Item it = items[blockIdx.x];
-    mode00_body(it.line_sums, it.ny, it.nz, it.b0, it.bn, it.inv_scale, it.plane_add, it.upart, it.nparts, correct);
+    mode00_body(it.line_sums, it.ny, it.nz, it.b0, it.bn, it.inv_scale, it.plane_add, it.upart, it.nparts, correct,
+                it.zper);
 }
 
 constexpr int RES_ZC = 30;  // planes marched per thread (a multiple of the 3-plane unroll)
@@ -228,8 +270,8 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     gdouble *f_p = (gdouble *)(a.f + cell);
     __shared__ double red[256];
     if (a.upart && bz == 0) {
-        // plane-0 sum of u, one partial per (x, y) tile
-        red[threadIdx.x] = valid ? uc_p[0] : 0.0;
+        // plane-0 sum of u (periodic z: planes 0 and nz-1), one partial per (x, y) tile
+        red[threadIdx.x] = valid ? uc_p[0] + (a.bc[4] == 'P' ? uc_p[sz * (a.nz - 1)] : 0.0) : 0.0;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
             if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -383,17 +425,19 @@ hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz
 }
 
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
-                         double *plane_add, const double *upart, int nparts, int correct, hipStream_t s)
+                         double *plane_add, const double *upart, int nparts, int correct, int zper, hipStream_t s)
 {
+    if (nz > 1024 || (zper && nz < 3)) return hipErrorInvalidValue;  // red[] holds the periodic column's vector
     const size_t lds = 2 * (size_t)nz * sizeof(double);
     hipLaunchKernelGGL(k_mode00, dim3(1), dim3(1024), lds, s, line_sums, ny, nz, b0, bn, inv_scale, plane_add, upart,
-                       nparts, correct);
+                       nparts, correct, zper);
     return hipGetLastError();
 }
 
 hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, int correct, hipStream_t s)
 {
     if (nitems == 0) return hipSuccess;
+    if (nz_max > 1024) return hipErrorInvalidValue;
     const size_t lds = 2 * (size_t)nz_max * sizeof(double);
     hipLaunchKernelGGL(k_mode00_batch, dim3(nitems), dim3(1024), lds, s, items, correct);
     return hipGetLastError();

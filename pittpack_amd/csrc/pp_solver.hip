@@ -278,11 +278,11 @@ int ensure_ready(pp_solver *h)
 {
     if (!h->bc_set || !h->coords_set) return PP_CALL_ORDER;
     if (h->g.zper) {
-        // periodic z: FP64 form only, any z length from 2 to 1024
-        if (h->precision == 1) return PP_UNSUPPORTED_BC;
+        // periodic z: any z length from 2 to 1024 (the mixed form's singular column: from 3)
         if (!zperiodic_supported(h->g.nz) && !line_size_supported(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+        if (h->precision == 1 && h->g.nz < 3) return PP_UNSUPPORTED_SIZE;
     }
-    if (h->precision == 1 && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
+    if (h->precision == 1 && !h->g.zper && !zsolve_supported_f32(h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (h->precision == 0 && !h->g.zper && !fp64_z_supported(h->zsolver, h->g.nz)) return PP_UNSUPPORTED_SIZE;
     if (!h->tables_ready) {
         h->mixed_ready = false;
@@ -353,6 +353,49 @@ int ensure_ztiles(pp_solver *h)
     return PP_SUCCESS;
 }
 
+// Periodic z solve of the field in place: the one-pass kernel for lengths with a compiled
+// complex plan, else real z-line transforms (the columns of a [nz][ny*nx] array: R2HC,
+// divide, HC2R).  f32: the mixed form's FP32 work field, singular column written as zeros.
+int run_zperiodic(pp_solver *h, void *data, bool f32)
+{
+    hipStream_t s = h->stream;
+    ZPerArgs zp{};
+    zp.data = data;
+    zp.SX = h->g.nx;
+    zp.SY = h->g.ny;
+    zp.nz = h->g.nz;
+    zp.lamx = h->d_lamx;
+    zp.lamy = h->d_lamy;
+    zp.Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+    zp.Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+    zp.tw = h->d_ztw;
+    zp.c2 = h->d_zc2;
+    zp.pinned_zero = f32 ? 1 : 0;
+    if (zperiodic_supported(h->g.nz)) {
+        PP_HIP_CHECK(f32 ? launch_zperiodic_f32(zp, s) : launch_zperiodic_f64(zp, s));
+        return PP_SUCCESS;
+    }
+    LineArgs a{};
+    a.in = data;
+    a.out = data;
+    const DevTables &t = h->tabz;
+    a.tw = (const double *)t.tw;
+    a.A = (const double *)t.A;
+    a.B = (const double *)t.B;
+    a.C = (const double *)t.C;
+    a.D = (const double *)t.D;
+    a.in_scale = 1.0;
+    a.out_scale = 1.0;
+    a.nlines = (long)h->g.nx * h->g.ny;
+    a.ncols = h->g.nx * h->g.ny;
+    a.nplanes = 1;
+    const int form = f32 ? FORM_F32 : FORM_F64;
+    PP_HIP_CHECK(launch_line(form, KIND_PP, h->g.nz, 1, 1, a, s));
+    PP_HIP_CHECK(f32 ? launch_zper_divide_f32(zp, s) : launch_zper_divide(zp, s));
+    PP_HIP_CHECK(launch_line(form, KIND_PP, h->g.nz, 1, 0, a, s));
+    return PP_SUCCESS;
+}
+
 // the five stages, FP64: rhs -> field (x forward), then on field (and the z tiles)
 int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
 {
@@ -401,39 +444,7 @@ int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
         z.zhi_dirichlet = h->g.zhi_d;
         z.zero_mode = h->zero_mode;
         if (h->g.zper) {
-            ZPerArgs zp{};
-            zp.data = field;
-            zp.SX = h->g.nx;
-            zp.SY = h->g.ny;
-            zp.nz = h->g.nz;
-            zp.lamx = h->d_lamx;
-            zp.lamy = h->d_lamy;
-            zp.Cx = z.Cx;
-            zp.Cy = z.Cy;
-            zp.tw = h->d_ztw;
-            zp.c2 = h->d_zc2;
-            if (zperiodic_supported(h->g.nz)) {
-                PP_HIP_CHECK(launch_zperiodic_f64(zp, s));
-            } else {
-                // z lines as the columns of a [nz][ny*nx] array: R2HC, divide, HC2R
-                LineArgs a{};
-                a.in = field;
-                a.out = field;
-                const DevTables &t = h->tabz;
-                a.tw = (const double *)t.tw;
-                a.A = (const double *)t.A;
-                a.B = (const double *)t.B;
-                a.C = (const double *)t.C;
-                a.D = (const double *)t.D;
-                a.in_scale = 1.0;
-                a.out_scale = 1.0;
-                a.nlines = (long)h->g.nx * h->g.ny;
-                a.ncols = h->g.nx * h->g.ny;
-                a.nplanes = 1;
-                PP_HIP_CHECK(launch_line(FORM_F64, KIND_PP, h->g.nz, 1, 1, a, s));
-                PP_HIP_CHECK(launch_zper_divide(zp, s));
-                PP_HIP_CHECK(launch_line(FORM_F64, KIND_PP, h->g.nz, 1, 0, a, s));
-            }
+            if ((rc = run_zperiodic(h, field, false))) return rc;
             if (h->zsing_x >= 0)
                 PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
         } else {
@@ -688,7 +699,8 @@ int ensure_mixed(pp_solver *h)
                 h->pinned00 = true;
                 continue;
             }
-            if (std::fabs(eig + 2.) < h->tau) fix.push_back(make_int4(sx, sy, 0, 0));
+            // (periodic z computes every column in FP64: nothing to re-solve)
+            if (!h->g.zper && std::fabs(eig + 2.) < h->tau) fix.push_back(make_int4(sx, sy, 0, 0));
         }
     h->nfix = (int)fix.size();
     if (h->nfix) {
@@ -756,7 +768,11 @@ int mixed_inner(pp_solver *h)
         PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[1], h->g.ny, 1, 1, a, s));
     }
     if ((rc = mark(h))) return rc;
-    {
+    if (h->g.zper) {
+        // periodic z: FP64 arithmetic on the FP32 field, singular column zeroed
+        if ((rc = run_zperiodic(h, h->d_w, true))) return rc;
+        if ((rc = mark(h))) return rc;
+    } else {
         ZArgs z = z_args(h, zdata);
         if (zt) {
             z.tiled = 1;
@@ -817,7 +833,8 @@ int run_mixed(pp_solver *h, const double *f, double *u)
     if (h->pinned00) {
         const double b0 = h->g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
         const double bn = h->g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
-        PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, nullptr, 0, 0, s));
+        PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, nullptr, 0, 0,
+                                   h->g.zper, s));
     }
     {
         LineArgs a = line_args(h, 0, nullptr);
@@ -864,7 +881,7 @@ int run_mixed(pp_solver *h, const double *f, double *u)
             const double b0 = h->g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
             const double bn = h->g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
             PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, h->d_upart,
-                                       residual_parts(h->g.nx, h->g.ny), 1, s));
+                                       residual_parts(h->g.nx, h->g.ny), 1, h->g.zper, s));
         }
         {
             LineArgs a = line_args(h, 0, nullptr);
@@ -1012,7 +1029,7 @@ int run_solve(pp_solver *h, const double *rhs, double *out)
         }
         return run_fp64(h, rhs, out);
     }
-    if (h->autotune && !h->mtuned) {
+    if (h->autotune && !h->mtuned && !h->g.zper) {
         const int rc = autotune_mixed(h);
         if (rc) return rc;
     }

@@ -119,6 +119,45 @@ def test_batch_1024_point_y_lines(precision):
             assert err <= TOL, f"member {i} {bc} ({nx},{ny},{nz}): rel max diff {err:.3e}"
 
 
+# periodic z next to Dirichlet / Neumann z in one batch: lengths with a compiled complex plan
+# (one launch per length over all such members), other lengths (real z-line transforms per
+# member), the singular column (x and y in {P, N}); FP64 columns of 513..1024 rows
+ZPER_MEMBERS = [
+    (32, 48, 64, "PPPPPP", [0, 1, 0, 1, 0, 1]),
+    (20, 12, 22, "NNNNPP", [0, 1, 0, 1, 0, 2]),
+    (48, 32, 64, "DDDDPP", [0, 2, 0, 1, 0, 1]),
+    (24, 20, 101, "NNPPPP", [0, 1, 0, 1, 0, 3]),
+    (33, 16, 8, "NDDNPP", [0, 1, 0, 1, 0, 1]),
+    (16, 24, 128, "DDPPPP", [0, 1, 0, 1, 0, 4]),
+    (32, 32, 40, "PPPPNN", [0, 1, 0, 1, 0, 1]),
+    (16, 16, 600, "NNNNDD", [0, 1, 0, 1, 0, 6]),
+    (16, 32, 1024, "DDNDDD", [0, 1, 0, 1, 0, 8]),
+    (64, 48, 17, "DDDDNN", [0, 1, 0, 1, 0, 1]),
+]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_periodic_z_and_long_columns(precision):
+    print(f"batch periodic z / long columns precision={precision}", flush=True)
+    with pa.Batch(precision=precision) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in ZPER_MEMBERS]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, ZPER_MEMBERS):
+            f = prng.uniform_pm1((nz, ny, nx), seed=300 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, ZPER_MEMBERS):
+            got = B.get_solution(i)
+            if precision == 0:
+                single = pa.solve(rhs[i], bc, box)
+                assert np.array_equal(got, single), f"member {i} {bc}: batched != single ({rel_maxdiff(got, single):.2e})"
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+            err = rel_maxdiff(got, ref)
+            assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
+
+
 def test_batch_repeated_and_device_fill():
     """device generator == Python generator; the right-hand sides are kept, so a second solve
     reproduces the first"""

@@ -37,6 +37,63 @@ def test_mixed_parity(bc, shape):
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
+# periodic z: the z stage computes in FP64 on the FP32 work field (one-pass circulant solve for
+# lengths with a compiled complex plan, real z-line transforms around a division otherwise);
+# the singular column (x and y in {P, N}) follows the reference's averaged row 0
+# (shermanMorrisonThomasV1) from FP64 plane sums, in the initial solve and every correction
+ZPER_MIXED = ["PPPPPP", "NNNNPP", "NNPPPP", "DDPPPP", "DDDDPP", "DDNNPP", "NDDNPP", "DNNNPP"]
+
+
+@pytest.mark.parametrize("nz", [4, 5, 8, 22, 40, 64, 101, 128, 512, 600])
+@pytest.mark.parametrize("bc", ZPER_MIXED)
+def test_mixed_periodic_z(bc, nz):
+    shape = (nz, 12, 20)
+    print(f"mixed periodic z {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=nz * 3 + sum(map(ord, bc)))
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, nz / 16.0]
+    ref = oracle.solve(f, bc, box)
+    got = pa.solve(f, bc, box, precision=1)
+    err = rel_maxdiff(got, ref)
+    assert err <= TOL, f"{bc} nz={nz}: rel max diff {err:.3e}"
+
+
+@pytest.mark.parametrize("bc", ["PPPPPP", "NNNNPP"])
+def test_mixed_periodic_z_contracts(bc):
+    """periodic z on a larger grid: FP32 accuracy without refinement, the FP64 bar after two
+    steps"""
+    shape = (96, 64, 128)
+    print(f"mixed periodic z refine {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=41)
+    ref = oracle.solve(f, bc)
+    errs = [rel_maxdiff(pa.solve(f, bc, precision=1, refine=r), ref) for r in (0, 1, 2)]
+    assert errs[0] <= 1e-5, errs
+    assert errs[1] <= 1e-10, errs
+    assert errs[2] <= TOL, errs
+
+
+@pytest.mark.parametrize("bc", ["PPPPPP", "NNNNPP"])
+def test_mixed_periodic_z_thin_cells(bc):
+    """cells six times thinner in z than in x and y make the low (kx, ky) columns
+    ill-conditioned (|eig + 2| ~ 1e-5): the refinement converges to the exact 7-point solution
+    (residual at rounding level, below the oracle's own), while the reference's spectral FP64
+    solve, and so the oracle, carries cancellation error of about eps / |eig + 2| there"""
+    shape = (96, 64, 128)
+    box = [0.0, 2.0, 0.0, 1.0, 0.0, 0.25]
+    print(f"mixed periodic z thin cells {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=41)
+    fe = oracle.effective_rhs(f, bc, box)
+    ref = oracle.solve(f, bc, box)
+    got = pa.solve(f, bc, box, precision=1)
+
+    def resid(u):
+        r = fe - oracle.apply_operator(u, bc, box)
+        r[0] -= r[0].mean()  # the gauge row's plane-0 defect (row 0 replaced by the averaged rule)
+        return float(np.abs(r).max() / np.abs(fe).max())
+
+    assert resid(got) <= 2e-13 and resid(got) <= resid(ref), (resid(got), resid(ref))
+    assert rel_maxdiff(got, ref) <= 1e-11
+
+
 @pytest.mark.parametrize("bc", ["PPPPNN", "NNNNDD", "DDNDDD"])
 def test_mixed_refinement_contracts(bc):
     """FP32 accuracy without refinement, FP64 accuracy after one or two steps"""

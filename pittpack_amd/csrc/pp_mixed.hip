@@ -415,6 +415,195 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     }
 }
 
+// Residual with two adjacent x cells per lane (grids with even nx): a block tile is 128
+// columns x 4 rows, u and f are read as one 16-byte load per lane (one 1 KiB row per wave and
+// field), r is stored as one 8-byte pair, and the u(z) tile in LDS holds pairs -- half the
+// load and LDS instructions per cell of k_residual, whose address processing bounds it.
+// Each lane takes its left and right halo value from memory (a lane with no halo in a
+// direction reads its own cell); the rest is k_residual's march.  Plane-0 partial sums of u
+// are written per 64-column half tile, in k_residual's layout.  TWOY: ny = 1 (mod 4).
+typedef double rp_d2 __attribute__((ext_vector_type(2)));
+typedef float rp_f2 __attribute__((ext_vector_type(2)));
+constexpr int RP_CX = 2 * RES_TX;  // columns per block tile
+
+template <bool TWOY>
+__global__ void __launch_bounds__(256) k_residual_pair(ResArgs a)
+{
+    const int lin = xcd_logical(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                gridDim.x * gridDim.y * gridDim.z);
+    const int bx = lin % gridDim.x, by = (lin / gridDim.x) % gridDim.y, bz = lin / (gridDim.x * gridDim.y);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = bx * RP_CX + 2 * lane;  // first cell of the lane's pair
+    const int y = by * RES_TY + w;
+    const int z0 = bz * RES_ZC;
+    const bool valid = x < a.nx && y < a.ny;
+    const long sy = a.nx, sz = (long)a.nx * a.ny;
+    using gdouble = const __attribute__((address_space(1))) double;
+    using gd2 = const __attribute__((address_space(1))) rp_d2;
+    using gf2 = __attribute__((address_space(1))) rp_f2;
+    // lanes outside the grid work on the clamped pair (in bounds) and store nothing; nx is
+    // even, so a pair is wholly inside or wholly outside and every pair starts 16-byte aligned
+    const int xc = min(x, a.nx - 2), yc = min(y, a.ny - 1);
+    const long cell = xc + sy * yc;
+    // every offset below counts doubles (floats for r) from the pair's first cell; pairs are
+    // read and written through a vector cast of the element address
+    gdouble *uc_p = (gdouble *)(a.u + cell);
+    const double *u_c = a.u + cell, *f_c = a.f + cell;
+    auto ldu2 = [&](long o) { return *(gd2 *)(u_c + o); };
+    auto ldf2 = [&](long o) { return *(gd2 *)(f_c + o); };
+    __shared__ double red[256];
+    if (a.upart && bz == 0) {
+        // plane-0 (periodic z: and plane nz-1) sums of u per 64-column half tile
+        double sum = 0.0;
+        if (valid) {
+            const rp_d2 v = ldu2(0);
+            sum = v.x + v.y;
+            if (a.bc[4] == 'P') {
+                const rp_d2 q = ldu2(sz * (a.nz - 1));
+                sum += q.x + q.y;
+            }
+        }
+        red[(lane >> 5) * 128 + w * 32 + (lane & 31)] = sum;
+        __syncthreads();
+        for (int st = 64; st > 0; st >>= 1) {
+            if ((int)(threadIdx.x & 127) < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        const int g64 = (a.nx + RES_TX - 1) / RES_TX;
+        if (threadIdx.x == 0) a.upart[by * g64 + 2 * bx] = red[0];
+        if (threadIdx.x == 128 && 2 * bx + 1 < g64) a.upart[by * g64 + 2 * bx + 1] = red[128];
+    }
+    const double *ex = a.dtab, *ey = a.dtab + a.nx, *ez = a.dtab + a.nx + a.ny;
+    const double *ef = a.dtab + a.nx + a.ny + a.nz;
+    // halo offsets from the pair's first cell: left neighbour of cell xc, right neighbour of
+    // cell xc + 1; at the grid's faces the ghost closure (P wraps, N mirrors, D antisymmetric)
+    auto ghost = [](char bc, long wrap, long self, long &off, double &sg) {
+        off = bc == 'P' ? wrap : self;
+        sg = bc == 'D' ? -1.0 : 1.0;
+    };
+    const bool hxm = lane == 0, hxp = lane == RES_TX - 1 || xc + 1 == a.nx - 1;
+    const bool hym = w == 0, hyp = w == RES_TY - 1 || yc == a.ny - 1;
+    long oxm = -1, oxp = 2, oym = -sy, oyp = sy;
+    double gxm = 1.0, gxp = 1.0, gym = 1.0, gyp = 1.0;
+    if (xc == 0) ghost(a.bc[0], a.nx - 1, 0, oxm, gxm);
+    if (xc + 1 == a.nx - 1) ghost(a.bc[1], -(long)(a.nx - 2), 1, oxp, gxp);
+    if (yc == 0) ghost(a.bc[2], sy * (a.ny - 1), 0, oym, gym);
+    if (yc == a.ny - 1) ghost(a.bc[3], -sy * (a.ny - 1), 0, oyp, gyp);
+    const long ohl = hxm ? oxm : 0, ohr = hxp ? oxp : 0;
+    const long ohy = hym ? oym : (hyp ? oyp : 0);
+    const bool bothy = TWOY && hym && hyp;
+    const long ohy2 = bothy ? oyp : 0;
+    __shared__ double ezs[RES_ZC];
+    __shared__ rp_d2 tile[2][RES_TY][RES_TX];
+    if (threadIdx.x < RES_ZC) ezs[threadIdx.x] = z0 + (int)threadIdx.x < a.nz ? gld(ez, z0 + threadIdx.x) : 0.0;
+    // Dirichlet terms per cell: x and y faces a per-(x, y) constant times ez[z], z faces a
+    // constant on the first and last plane
+    rp_d2 cxy = {0.0, 0.0}, czlo = {0.0, 0.0}, czhi = {0.0, 0.0};
+    if (a.dmask) {
+        const double ex0 = ex[xc], ex1 = ex[xc + 1], eyc = ey[yc];
+        if ((a.dmask & 1) && xc == 0) cxy.x += 2. * ef[0] * eyc;
+        if ((a.dmask & 2) && xc + 1 == a.nx - 1) cxy.y += 2. * ef[1] * eyc;
+        if ((a.dmask & 4) && yc == 0) {
+            cxy.x += 2. * ex0 * ef[2];
+            cxy.y += 2. * ex1 * ef[2];
+        }
+        if ((a.dmask & 8) && yc == a.ny - 1) {
+            cxy.x += 2. * ex0 * ef[3];
+            cxy.y += 2. * ex1 * ef[3];
+        }
+        if (a.dmask & 16) {
+            czlo.x = 2. * ex0 * eyc * ef[4];
+            czlo.y = 2. * ex1 * eyc * ef[4];
+        }
+        if (a.dmask & 32) {
+            czhi.x = 2 * ex0 * eyc * ef[5];
+            czhi.y = 2 * ex1 * eyc * ef[5];
+        }
+    }
+    const double defect = a.pin_defect ? a.pin_defect[0] : 0.0;
+    float *r_c = reinterpret_cast<float *>(a.r) + cell;
+
+    rp_d2 uc = ldu2(sz * z0);
+    rp_d2 um;
+    if (z0 > 0) um = ldu2(sz * (z0 - 1));
+    else um = a.bc[4] == 'P' ? ldu2(sz * (a.nz - 1)) : (a.bc[4] == 'N' ? uc : -uc);
+    struct Plane {
+        rp_d2 up, f, hy, hy2;
+        double hl, hr;
+    };
+    const bool ztop_wrap = a.bc[5] == 'P';
+    auto fetch = [&](int z, Plane &q) {
+        const long o = sz * z;
+        q.up = ldu2(z + 1 < a.nz ? o + sz : (ztop_wrap ? 0 : o));
+        q.f = ldf2(o);
+        q.hl = uc_p[o + ohl];
+        q.hr = uc_p[o + ohr];
+        q.hy = ldu2(o + ohy);
+        if constexpr (TWOY) q.hy2 = ldu2(o + ohy2);
+        else q.hy2 = q.hy;
+    };
+    const int zend = min(z0 + RES_ZC, a.nz);
+    const bool full_tile = (bx + 1) * RP_CX <= a.nx && (by + 1) * RES_TY <= a.ny;
+    auto step = [&](int z, const Plane &cur, auto all_valid) {
+        constexpr bool ALLV = decltype(all_valid)::value;
+        const int b = (z - z0) & 1;
+        tile[b][w][lane] = uc;
+        __syncthreads();  // one barrier per plane: the other buffer was last read before it
+        const double t_l = tile[b][w][lane > 0 ? lane - 1 : 0].y;
+        const double t_r = tile[b][w][lane < RES_TX - 1 ? lane + 1 : RES_TX - 1].x;
+        const rp_d2 t_ym = tile[b][w > 0 ? w - 1 : 0][lane];
+        const rp_d2 t_yp = tile[b][w < RES_TY - 1 ? w + 1 : w][lane];
+        const rp_d2 up = (z + 1 < a.nz || ztop_wrap) ? cur.up : (a.bc[5] == 'N' ? uc : -uc);
+        const double uxl = hxm ? gxm * cur.hl : t_l;
+        const double uxr = hxp ? gxp * cur.hr : t_r;
+        const rp_d2 uym = hym ? gym * cur.hy : t_ym;
+        const rp_d2 uyp = hyp ? gyp * (bothy ? cur.hy2 : cur.hy) : t_yp;
+        // second differences as differences of first differences (as k_residual)
+        const double Lu0 = ((uc.y - uc.x) - (uc.x - uxl)) * a.Cx + ((uyp.x - uc.x) - (uc.x - uym.x)) * a.Cy +
+                           ((up.x - uc.x) - (uc.x - um.x));
+        const double Lu1 = ((uxr - uc.y) - (uc.y - uc.x)) * a.Cx + ((uyp.y - uc.y) - (uc.y - uym.y)) * a.Cy +
+                           ((up.y - uc.y) - (uc.y - um.y));
+        rp_d2 v = cur.f;
+        v = v * a.c3;
+        v = v * a.c3;
+        v = v - cxy * ezs[z - z0];
+        if (z == 0) v = v - czlo;
+        if (z == a.nz - 1) v = v - czhi;
+        double res0 = v.x - Lu0, res1 = v.y - Lu1;
+        if (z == 0) {
+            res0 += defect;
+            res1 += defect;
+        }
+        if (ALLV || valid) *(gf2 *)(r_c + sz * z) = rp_f2{(float)res0, (float)res1};
+        um = uc;
+        uc = up;
+    };
+    if (full_tile && zend - z0 == RES_ZC) {
+        Plane A, B, C;
+        fetch(z0, A);
+        fetch(z0 + 1, B);
+#pragma clang loop unroll(full)
+        for (int g = 0; g < RES_ZC / 3; g++) {
+            const int z = z0 + 3 * g;
+            fetch(min(z + 2, zend - 1), C);
+            step(z, A, std::true_type{});
+            fetch(min(z + 3, zend - 1), A);
+            step(z + 1, B, std::true_type{});
+            fetch(min(z + 4, zend - 1), B);
+            step(z + 2, C, std::true_type{});
+        }
+    } else {
+        Plane cur;
+        fetch(z0, cur);
+        for (int z = z0; z < zend; z++) {
+            Plane nxt;
+            fetch(min(z + 1, zend - 1), nxt);
+            step(z, cur, std::false_type{});
+            cur = nxt;
+        }
+    }
+}
+
 hipError_t launch_zfix64_f32(const ZArgs &a, const int4 *cols, int ncols, int nz_max, hipStream_t s)
 {
     if (ncols == 0) return hipSuccess;
@@ -457,6 +646,13 @@ bool residual_two_halo(int nx, int ny) { return nx % RES_TX == 1 || ny % RES_TY 
 
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
 {
+    if (a.nx % 2 == 0 && a.nx >= 2) {
+        const dim3 grid((unsigned)((a.nx + RP_CX - 1) / RP_CX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
+                        (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
+        if (a.ny % RES_TY == 1) hipLaunchKernelGGL(k_residual_pair<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_residual_pair<false>, grid, dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((a.nx + RES_TX - 1) / RES_TX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
               (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
     if (residual_two_halo(a.nx, a.ny)) hipLaunchKernelGGL((k_residual<float, true>), grid, dim3(256), 0, s, a, nullptr, nullptr);

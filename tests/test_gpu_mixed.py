@@ -94,6 +94,18 @@ def test_mixed_periodic_z_thin_cells(bc):
     assert rel_maxdiff(got, ref) <= 1e-11
 
 
+# residual tiles whose first pair of columns is the grid's last (nx = 2 mod 128) and whose last
+# row is a tile's first (ny = 1 mod 4): both neighbours of one direction come from memory
+@pytest.mark.parametrize("shape", [(12, 9, 130), (7, 5, 258), (20, 13, 66)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", ["PPPPNN", "DDDDDD", "NNNNDD", "DDNDDD", "NNPPPP"])
+def test_mixed_residual_tile_edges(bc, shape):
+    print(f"mixed residual tile edges {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=sum(map(ord, bc)) + shape[2])
+    ref = oracle.solve(f, bc, zero_mode=1)
+    got = pa.solve(f, bc, zero_mode=1, precision=1)
+    assert rel_maxdiff(got, ref) <= TOL
+
+
 @pytest.mark.parametrize("bc", ["PPPPNN", "NNNNDD", "DDNDDD"])
 def test_mixed_refinement_contracts(bc):
     """FP32 accuracy without refinement, FP64 accuracy after one or two steps"""

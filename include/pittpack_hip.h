@@ -105,7 +105,14 @@ enum {
      * under a stream of small ones */
     PP_OPT_MAX_OVERTAKE = 13,
     /* pp_io file format: PP_IO_HDF5 (default) or PP_IO_RAW */
-    PP_OPT_IO_FORMAT = 14
+    PP_OPT_IO_FORMAT = 14,
+    /* pp_set_option_double, FP64 form with z solver 0 or 1, columns up to 512 rows: columns
+     * with |eig+2| below this threshold (not the pinned one) are solved by the reference's
+     * serial Thomas (thomasLowMem, src/triDiag.cpp:1693-1822) instead of the PCR steps, which
+     * lose up to ten times more digits on nearly singular columns (strongly anisotropic cells,
+     * long Neumann axes).  0 (default) = off; 1e-3 covers the columns where the two differ
+     * beyond 1e-13. */
+    PP_OPT_ZTHOMAS_TAU = 15
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

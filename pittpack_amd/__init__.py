@@ -47,6 +47,7 @@ OPT_AUTOTUNE = 11
 OPT_LINE_VARIANT = 12
 OPT_MAX_OVERTAKE = 13
 OPT_IO_FORMAT = 14
+OPT_ZTHOMAS_TAU = 15
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (

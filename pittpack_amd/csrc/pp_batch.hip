@@ -248,7 +248,9 @@ int BatchCore::plan_mixed()
     m00_.clear();
     m00_mem_.clear();
     ritems_.clear();
+    rpitems_.clear();
     res_two_ = false;
+    resp_two_ = false;
     for (size_t i = 0; i < mem.size(); i++) {
         BatchMember &m = mem[i];
         const SolveSetup &g = m.g;
@@ -286,14 +288,17 @@ int BatchCore::plan_mixed()
             m00_.push_back(it);
             m00_mem_.push_back((int)i);
         }
-        res_two_ = res_two_ || residual_two_halo(g.nx, g.ny);
+        // the single-solve handle's residual kernel for this member's nx (results bitwise alike)
+        const bool pair = residual_pairs(g.nx);
+        if (pair) resp_two_ = resp_two_ || g.ny % residual_tile_y() == 1;
+        else res_two_ = res_two_ || residual_two_halo(g.nx, g.ny);
         const int zp = residual_zplanes();
-        const int tx = residual_tile_x(), ty = residual_tile_y();
+        const int tx = residual_tile_x_of(g.nx), ty = residual_tile_y();
         const int gx = (g.nx + tx - 1) / tx, gy = (g.ny + ty - 1) / ty, gz = (g.nz + zp - 1) / zp;
-        if ((long)gx * gy * gz != residual_blocks(g.nx, g.ny, g.nz)) return PP_INPUT_ARGS;
+        std::vector<int4> &ri = pair ? rpitems_ : ritems_;
         for (int bz = 0; bz < gz; bz++)
             for (int by = 0; by < gy; by++)
-                for (int bx = 0; bx < gx; bx++) ritems_.push_back(make_int4((int)i, bx, by, bz));
+                for (int bx = 0; bx < gx; bx++) ri.push_back(make_int4((int)i, bx, by, bz));
     }
     nsums_ = so;
     nplane_ = po;
@@ -397,6 +402,7 @@ int BatchCore::plan()
         o_m00_ = take(c, m00_.size() * sizeof(Mode00Item));
         o_res_ = take(c, res_.size() * sizeof(ResArgs));
         o_ritems_ = take(c, ritems_.size() * sizeof(int4));
+        o_rpitems_ = take(c, rpitems_.size() * sizeof(int4));
     }
     region_bytes = (c + 255) / 256 * 256;
     planned = true;
@@ -449,6 +455,7 @@ int BatchCore::bind(char *base, TableCache *tc)
         d_m00_ = (Mode00Item *)(base + o_m00_);
         d_res_ = (ResArgs *)(base + o_res_);
         d_ritems_ = (int4 *)(base + o_ritems_);
+        d_rpitems_ = (int4 *)(base + o_rpitems_);
     }
     for (size_t i = 0; i < mem.size(); i++) {
         const BatchMember &m = mem[i];
@@ -529,7 +536,7 @@ int BatchCore::upload(hipStream_t s)
         for (HostClass &h : lfix_)
             if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
         if ((rc = put(d_fix_, fix_, s)) || (rc = put(d_m00_, m00_, s)) || (rc = put(d_res_, res_, s)) ||
-            (rc = put(d_ritems_, ritems_, s)))
+            (rc = put(d_ritems_, ritems_, s)) || (rc = put(d_rpitems_, rpitems_, s)))
             return rc;
     }
     return PP_SUCCESS;
@@ -785,6 +792,7 @@ int BatchCore::run_mixed(hipStream_t s, Marks *mk)
     if ((rc = mark(s, mk))) return rc;
     for (int it = 0; it < refine; it++) {
         PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), res_two_, s));
+        PPB_CHECK(launch_residual_pair_f32_batch(d_res_, d_rpitems_, (int)rpitems_.size(), resp_two_, s));
         if ((rc = mark(s, mk))) return rc;
         if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0))) return rc;
         if ((rc = mark(s, mk))) return rc;

@@ -133,19 +133,21 @@ class BatchCore {
     std::vector<Mode00Item> m00_;
     std::vector<int> m00_mem_;  // member of each m00_ entry
     std::vector<ResArgs> res_;
-    std::vector<int4> ritems_;
-    bool res_two_ = false;  // some member's residual tiles need the second halo load
+    std::vector<int4> ritems_;   // residual tiles of odd-nx members (64 columns)
+    std::vector<int4> rpitems_;  // residual tiles of even-nx members (two cells per lane, 128 columns)
+    bool res_two_ = false;   // some odd-nx member's residual tiles need the second halo load
+    bool resp_two_ = false;  // some even-nx member has ny = 1 (mod 4)
     int nz_max_ = 0;
     long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)
     size_t o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
-           o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0;
+           o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0, o_rpitems_ = 0;
     std::map<int, AxisDev> tabs_;
     // device pointers of the tables
     double *d_lam_ = nullptr, *d_dtab_ = nullptr, *d_sums_ = nullptr, *d_plane_ = nullptr, *d_upart_ = nullptr;
     LineSolve *d_ls_ = nullptr;
     ZSolve *d_zs_ = nullptr;
-    int4 *d_fix_ = nullptr, *d_ritems_ = nullptr;
+    int4 *d_fix_ = nullptr, *d_ritems_ = nullptr, *d_rpitems_ = nullptr;
     double *d_zc2_ = nullptr;
     SmtItem *d_smt_ = nullptr;
     int smt_nz_max_ = 0;

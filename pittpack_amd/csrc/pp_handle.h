@@ -46,6 +46,12 @@ struct pp_solver {
     hipEvent_t *ev_force = nullptr;             // tuner's own events for the launch marks
     int mg_cycles = 200;                        // multigrid z solver: V-cycle limit
     double mg_tol = 1e-16;                      //   and relative max-norm residual tolerance
+    // FP64 form: columns with |eig+2| below zthomas_tau (not pinned) are solved by the
+    // reference's serial Thomas after the PCR kernel (0: off); their list, built with the tables
+    double zthomas_tau = 0.0;
+    int4 *d_zthomas = nullptr;
+    int nzthomas = 0;
+    bool zthomas_ready = false;
     int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any
     double zsing_eig = -2.0;
     std::vector<double> lamx_h, lamy_h;  // host copies of the eigenvalue factors

@@ -33,6 +33,10 @@ bool zsolve_supported_f32(int nz);
 bool zfix_tiles_supported(int nz);
 hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s);
 
+// the reference's serial Thomas (thomasLowMem) on the listed columns cols[i] = {sx, sy, 0, 0}
+// of FP64 data in place or on z tiles (a's layout), one block per column; nz <= 1024
+hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, int ncols, hipStream_t s);
+
 // periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64 arithmetic on FP64 or FP32
 // storage; the singular column by the reference's own branch
 struct ZPerArgs {

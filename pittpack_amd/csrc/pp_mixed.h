@@ -53,6 +53,13 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
 // (residual_two_halo), which takes a second halo load per direction
 bool residual_two_halo(int nx, int ny);
 hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s);
+// grids with even nx take the two-cells-per-lane kernel (residual_tile_x_of(nx) = 128 columns
+// per tile, partial sums in the same layout); its batched launch: items {member, bx, by, bz}
+// over 128-column tiles, twoy: some member has ny = 1 (mod residual_tile_y())
+bool residual_pairs(int nx);
+int residual_tile_x_of(int nx);
+hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy,
+                                          hipStream_t s);
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
                          double *plane_add, const double *upart, int nparts, int correct, int zper, hipStream_t s);
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s);

@@ -427,11 +427,20 @@ typedef float rp_f2 __attribute__((ext_vector_type(2)));
 constexpr int RP_CX = 2 * RES_TX;  // columns per block tile
 
 template <bool TWOY>
-__global__ void __launch_bounds__(256) k_residual_pair(ResArgs a)
+__global__ void __launch_bounds__(256) k_residual_pair(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
+    ResArgs a = a0;
     const int lin = xcd_logical(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
                                 gridDim.x * gridDim.y * gridDim.z);
-    const int bx = lin % gridDim.x, by = (lin / gridDim.x) % gridDim.y, bz = lin / (gridDim.x * gridDim.y);
+    int bx = lin % gridDim.x, by = (lin / gridDim.x) % gridDim.y, bz = lin / (gridDim.x * gridDim.y);
+    if (items) {
+        // batched: block b works on items[b] = {member, bx, by, bz} (128-column tiles)
+        const int4 it = items[lin];
+        a = solves[it.x];
+        bx = it.y;
+        by = it.z;
+        bz = it.w;
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int x = bx * RP_CX + 2 * lane;  // first cell of the lane's pair
     const int y = by * RES_TY + w;
@@ -632,6 +641,19 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
     return hipGetLastError();
 }
 
+bool residual_pairs(int nx) { return nx % 2 == 0 && nx >= 2; }
+
+int residual_tile_x_of(int nx) { return residual_pairs(nx) ? RP_CX : RES_TX; }
+
+hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy, hipStream_t s)
+{
+    if (nitems == 0) return hipSuccess;
+    ResArgs a{};
+    if (twoy) hipLaunchKernelGGL(k_residual_pair<true>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
+    else hipLaunchKernelGGL(k_residual_pair<false>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
+    return hipGetLastError();
+}
+
 int residual_parts(int nx, int ny) { return ((nx + RES_TX - 1) / RES_TX) * ((ny + RES_TY - 1) / RES_TY); }
 
 int residual_tile_x() { return RES_TX; }
@@ -646,11 +668,11 @@ bool residual_two_halo(int nx, int ny) { return nx % RES_TX == 1 || ny % RES_TY 
 
 hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
 {
-    if (a.nx % 2 == 0 && a.nx >= 2) {
+    if (residual_pairs(a.nx)) {
         const dim3 grid((unsigned)((a.nx + RP_CX - 1) / RP_CX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
                         (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
-        if (a.ny % RES_TY == 1) hipLaunchKernelGGL(k_residual_pair<true>, grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_residual_pair<false>, grid, dim3(256), 0, s, a);
+        if (a.ny % RES_TY == 1) hipLaunchKernelGGL(k_residual_pair<true>, grid, dim3(256), 0, s, a, nullptr, nullptr);
+        else hipLaunchKernelGGL(k_residual_pair<false>, grid, dim3(256), 0, s, a, nullptr, nullptr);
         return hipGetLastError();
     }
     dim3 grid((unsigned)((a.nx + RES_TX - 1) / RES_TX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),

@@ -188,8 +188,17 @@ void free_mixed(pp_solver *h)
     h->mixed_ready = false;
 }
 
+void free_zthomas(pp_solver *h)
+{
+    if (h->d_zthomas) (void)hipFree(h->d_zthomas);
+    h->d_zthomas = nullptr;
+    h->nzthomas = 0;
+    h->zthomas_ready = false;
+}
+
 void free_tables(pp_solver *h)
 {
+    free_zthomas(h);
     for (DevTables *t : {&h->tab64[0], &h->tab64[1], &h->tabz}) {
         for (void **p : {&t->tw, &t->A, &t->B, &t->C, &t->D}) {
             if (*p) (void)hipFree(*p);
@@ -396,6 +405,44 @@ int run_zperiodic(pp_solver *h, void *data, bool f32)
     return PP_SUCCESS;
 }
 
+// PP_OPT_ZTHOMAS_TAU applies: FP64 PCR -> Thomas or partitioned Thomas on columns of at
+// most 512 rows, non-periodic z
+bool zthomas_on(const pp_solver *h)
+{
+    return h->zthomas_tau > 0.0 && !h->g.zper && (h->zsolver == 0 || h->zsolver == 1) && zsolve_supported_f64(h->g.nz);
+}
+
+// The columns the serial Thomas takes: |eig+2| < zthomas_tau with the kernels' eigenvalue
+// rounding, the pinned column excluded (it keeps the row-0 rule); built once per tables,
+// threshold and zero-mode rule
+int ensure_zthomas(pp_solver *h)
+{
+    if (h->zthomas_ready) return PP_SUCCESS;
+    if (h->d_zthomas) {
+        PP_HIP_CHECK(hipStreamSynchronize(h->stream));
+        PP_HIP_CHECK(hipFree(h->d_zthomas));
+        h->d_zthomas = nullptr;
+    }
+    h->nzthomas = 0;
+    const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
+    const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+    std::vector<int4> cols;
+    for (int sy = 0; sy < h->g.ny; sy++)
+        for (int sx = 0; sx < h->g.nx; sx++) {
+            const double eig = column_eig(h->lamx_h, h->lamy_h, sx, sy, Cx, Cy);
+            bool pin = std::fabs(eig + 2.) < 1.e-10;
+            if (pin && h->zero_mode == 1 && (h->g.zlo_d || h->g.zhi_d)) pin = false;
+            if (!pin && std::fabs(eig + 2.) < h->zthomas_tau) cols.push_back(make_int4(sx, sy, 0, 0));
+        }
+    if (!cols.empty()) {
+        if (hipMalloc((void **)&h->d_zthomas, cols.size() * sizeof(int4)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_zthomas, cols.data(), cols.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    h->nzthomas = (int)cols.size();
+    h->zthomas_ready = true;
+    return PP_SUCCESS;
+}
+
 // the five stages, FP64: rhs -> field (x forward), then on field (and the z tiles)
 int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
 {
@@ -403,6 +450,8 @@ int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
     const bool zt = use_ztiles(h);
     int rc;
     if (zt && (rc = ensure_ztiles(h))) return rc;
+    const bool zth = zthomas_on(h);
+    if (zth && (rc = ensure_zthomas(h))) return rc;
     double *zdata = zt ? h->d_zt : field;
     if ((rc = begin_timing(h))) return rc;
     // 1. forward x: rhs * dz^2, Dirichlet boundary term, x transform
@@ -448,6 +497,8 @@ int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
             if (h->zsing_x >= 0)
                 PP_HIP_CHECK(launch_smt_v1(field, h->g.nx, h->g.ny, h->g.nz, h->zsing_x, h->zsing_y, h->zsing_eig, s));
         } else {
+            // nearly singular columns left to the reference's serial Thomas (PP_OPT_ZTHOMAS_TAU)
+            if (zth) z.tau = h->zthomas_tau;
             switch (h->zsolver) {
             case 1: PP_HIP_CHECK(launch_zsolve_scan_f64(h->g.nz, z, s)); break;
             case 2: PP_HIP_CHECK(launch_zsolve_mg_f64(z, h->mg_cycles, h->mg_tol, s)); break;
@@ -455,6 +506,7 @@ int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
             case 4: PP_HIP_CHECK(launch_zsolve_cr_f64(z, s)); break;
             default: PP_HIP_CHECK(launch_zsolve_pcr_f64(h->g.nz, z, s)); break;
             }
+            if (zth) PP_HIP_CHECK(launch_zthomas_f64(z, h->d_zthomas, h->nzthomas, s));
         }
     }
     if ((rc = mark(h))) return rc;
@@ -1229,6 +1281,7 @@ int pp_set_option(pp_handle h, int key, long long value)
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         h->zero_mode = (int)value;
         h->mixed_ready = false;
+        h->zthomas_ready = false;
         return PP_SUCCESS;
     case PP_OPT_ZSOLVER:
         if (value < 0 || value > 4) return PP_INPUT_ARGS;
@@ -1353,6 +1406,12 @@ int pp_set_option_double(pp_handle h, int key, double value)
         if (!(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
         h->tau = value;
         h->mixed_ready = false;
+        return PP_SUCCESS;
+    }
+    if (key == PP_OPT_ZTHOMAS_TAU) {
+        if (!(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
+        h->zthomas_tau = value;
+        h->zthomas_ready = false;
         return PP_SUCCESS;
     }
     return PP_INPUT_ARGS;

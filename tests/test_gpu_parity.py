@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 import pittpack_amd as pa
+from pittpack_amd import rng as prng
 
 pytestmark = pytest.mark.gpu
 
@@ -227,6 +228,51 @@ def test_call_order_and_errors():
         with pytest.raises(pa.PittPackError) as e:
             M.pittPack()
         assert e.value.code == pa.UNSUPPORTED_SIZE
+
+
+def _solve_opts(f, bc, box, zthomas=0.0, zsolver=None, zlayout=None):
+    nz, ny, nx = f.shape
+    with pa.PoissonHIP(nx, ny, nz) as M:
+        M.assignBoundary(bc)
+        M.setBox(box)
+        M.setCoords(2)
+        if zsolver is not None:
+            M.set_option(pa.OPT_ZSOLVER, zsolver)
+        if zlayout is not None:
+            M.set_option(pa.OPT_ZLAYOUT, zlayout)
+        if zthomas:
+            M.set_option_double(pa.OPT_ZTHOMAS_TAU, zthomas)
+        M.assignRhs(f)
+        M.pittPack()
+        return M.getSolution()
+
+
+def test_zthomas_thin_cells():
+    """cells six times thinner in z than in x and y with Neumann z faces: the low (kx, ky)
+    columns have |eig + 2| ~ 1e-5, where the PCR steps lose about ten times more digits than
+    the reference's Thomas (2e-12 against the oracle); PP_OPT_ZTHOMAS_TAU hands those columns to
+    the reference's serial Thomas and restores the FP64 bar"""
+    print("zthomas thin cells NNNNNN (96, 64, 128)", flush=True)
+    f = prng.uniform_pm1((96, 64, 128), seed=41)
+    box = [0.0, 2.0, 0.0, 1.0, 0.0, 0.25]
+    ref = oracle.solve(f, "NNNNNN", box, zero_mode=1)
+    e_pcr = rel_maxdiff(_solve_opts(f, "NNNNNN", box), ref)
+    e_thomas = rel_maxdiff(_solve_opts(f, "NNNNNN", box, zthomas=1e-3), ref)
+    print(f"  PCR {e_pcr:.2e}  with Thomas columns {e_thomas:.2e}", flush=True)
+    assert e_thomas <= TOL and e_thomas <= e_pcr, (e_pcr, e_thomas)
+
+
+@pytest.mark.parametrize("zsolver,zlayout", [(0, 1), (0, 0), (1, 0)])
+@pytest.mark.parametrize("bc", ["DDNDDD", "NNNNDD", "PPPPNN", "NNNNNN"])
+def test_zthomas_paths(bc, zsolver, zlayout):
+    """the Thomas columns on z tiles and in place, after either direct z solver, both zero-mode
+    rules; a threshold of 0.5 sends many columns there"""
+    print(f"zthomas {bc} zsolver={zsolver} zlayout={zlayout}", flush=True)
+    f = prng.uniform_pm1((40, 48, 64), seed=sum(map(ord, bc)) + zsolver)
+    box = [0, 1, 0, 1, 0, 1]
+    ref = oracle.solve(f, bc, box, zero_mode=1)
+    got = _solve_opts(f, bc, box, zthomas=0.5, zsolver=zsolver, zlayout=zlayout)
+    assert rel_maxdiff(got, ref) <= TOL
 
 
 # periodic z: circulant columns by z FFT on the GPU, the singular column by the reference's

@@ -40,6 +40,11 @@ MIXED_STEP = [("residual_f64", 20), ("x_forward_f32", 8), ("y_forward_f32", 8), 
               ("z_fix_f64", 0), ("y_inverse_f32", 8), ("x_inverse_accumulate", 20)]
 
 
+def progress(msg: str):
+    """one line per finished phase on stderr (the JSON result goes to stdout)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def kernel_table(precision: int, refine: int):
     if precision == 0:
         return FP64_KERNELS
@@ -455,6 +460,7 @@ def main():
     tuning = M.tuning()
     u64 = u.clone()
     M.close()
+    progress(f"FP64 {n}^3: {ms_per_step:.3f} ms per solve")
 
     # the mixed form on the same workload, its throughput ratio and its error vs the FP64 form
     mixed = None
@@ -475,6 +481,7 @@ def main():
         }
         del diff
         M.close()
+        progress(f"mixed {n}^3: {mms:.3f} ms per solve")
     # host copy of the FP64 solution for the comparison with the oracle (CPU baseline leg)
     u64_host = u64.cpu().numpy() if not args.no_cpu_baseline else None
     del u64
@@ -534,15 +541,19 @@ def main():
     if args.large > 0:
         ls = max(1, min(args.steps, 5))
         out["large_fp64"], u_large = bench_large(pa, torch, args.large, 0, args.refine, ls)
+        progress(f"FP64 {args.large}^3 done")
         out["large_mixed"], u_mixed = bench_large(pa, torch, args.large, 1, args.refine, ls, ref=u_large)
+        progress(f"mixed {args.large}^3 done")
         del u_large, u_mixed
         torch.cuda.empty_cache()
     if args.batch_count > 0:
         out["batched"] = bench_batched(pa, args.batch_count, max(1, min(args.steps, 5)), args.refine,
                                        autotune=not args.no_autotune)
+        progress("batched form done")
     if args.serve_requests > 0:
         rates = [float(v) for v in args.serve_rates.split(",") if v.strip()]
         out["serving"] = bench_serving_sweep(pa, args.serve_requests, rates)
+        progress("serving sweep done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(u64_host, n=n)
     print(json.dumps(out), flush=True)

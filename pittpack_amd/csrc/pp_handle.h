@@ -50,6 +50,7 @@ struct pp_solver {
     // reference's serial Thomas after the PCR kernel (0: off); their list, built with the tables
     double zthomas_tau = 0.0;
     int4 *d_zthomas = nullptr;
+    double *d_zthomas_tab = nullptr;  // their Thomas factors (zthomas_factors), 2 nz per column
     int nzthomas = 0;
     bool zthomas_ready = false;
     int zsing_x = -1, zsing_y = -1;             // periodic z: the singular column, if any

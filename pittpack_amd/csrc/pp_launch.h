@@ -34,8 +34,24 @@ bool zfix_tiles_supported(int nz);
 hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s);
 
 // the reference's serial Thomas (thomasLowMem) on the listed columns cols[i] = {sx, sy, 0, 0}
-// of FP64 data in place or on z tiles (a's layout), one block per column; nz <= 1024
-hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, int ncols, hipStream_t s);
+// of FP64 data in place or on z tiles (a's layout), one block per column; nz <= 1024;
+// tab + 2 nz i = {1 / bet_j, tmp_j} of column i (zthomas_factors)
+hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, const double *tab, int ncols, hipStream_t s);
+
+// thomasLowMem's factors of one column (src/triDiag.cpp:1693-1822): out[j] = 1 / bet_j,
+// out[n + j] = tmp_j (tmp_0 unused), with b0 / bn the end rows' diagonals
+inline void zthomas_factors(double eig, double b0, double bn, int n, double *out)
+{
+    double bet = b0;
+    out[0] = 1.0 / bet;
+    out[n] = 0.0;
+    for (int j = 1; j < n; j++) {
+        const double tmp = 1.0 / bet;
+        bet = (j == n - 1 ? bn : eig) - tmp;
+        out[j] = 1.0 / bet;
+        out[n + j] = tmp;
+    }
+}
 
 // periodic z (pp_zperiodic.hip): circulant columns by z FFT, FP64 arithmetic on FP64 or FP32
 // storage; the singular column by the reference's own branch

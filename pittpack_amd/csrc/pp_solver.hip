@@ -191,7 +191,9 @@ void free_mixed(pp_solver *h)
 void free_zthomas(pp_solver *h)
 {
     if (h->d_zthomas) (void)hipFree(h->d_zthomas);
+    if (h->d_zthomas_tab) (void)hipFree(h->d_zthomas_tab);
     h->d_zthomas = nullptr;
+    h->d_zthomas_tab = nullptr;
     h->nzthomas = 0;
     h->zthomas_ready = false;
 }
@@ -418,12 +420,15 @@ bool zthomas_on(const pp_solver *h)
 int ensure_zthomas(pp_solver *h)
 {
     if (h->zthomas_ready) return PP_SUCCESS;
-    if (h->d_zthomas) {
+    if (h->d_zthomas || h->d_zthomas_tab) {
         PP_HIP_CHECK(hipStreamSynchronize(h->stream));
-        PP_HIP_CHECK(hipFree(h->d_zthomas));
+        if (h->d_zthomas) PP_HIP_CHECK(hipFree(h->d_zthomas));
+        if (h->d_zthomas_tab) PP_HIP_CHECK(hipFree(h->d_zthomas_tab));
         h->d_zthomas = nullptr;
+        h->d_zthomas_tab = nullptr;
     }
     h->nzthomas = 0;
+    int rc;
     const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
     const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
     std::vector<int4> cols;
@@ -435,8 +440,16 @@ int ensure_zthomas(pp_solver *h)
             if (!pin && std::fabs(eig + 2.) < h->zthomas_tau) cols.push_back(make_int4(sx, sy, 0, 0));
         }
     if (!cols.empty()) {
+        const int n = h->g.nz;
+        std::vector<double> tab(cols.size() * 2 * (size_t)n);
+        for (size_t i = 0; i < cols.size(); i++) {
+            const double eig = column_eig(h->lamx_h, h->lamy_h, cols[i].x, cols[i].y, Cx, Cy);
+            zthomas_factors(eig, h->g.zlo_d ? eig - 1. : eig + 1., h->g.zhi_d ? eig - 1. : eig + 1., n,
+                            tab.data() + i * 2 * n);
+        }
         if (hipMalloc((void **)&h->d_zthomas, cols.size() * sizeof(int4)) != hipSuccess) return PP_ALLOCATION_FAIL;
         PP_HIP_CHECK(hipMemcpy(h->d_zthomas, cols.data(), cols.size() * sizeof(int4), hipMemcpyHostToDevice));
+        if ((rc = upload((void **)&h->d_zthomas_tab, tab))) return rc;
     }
     h->nzthomas = (int)cols.size();
     h->zthomas_ready = true;
@@ -506,7 +519,7 @@ int run_fp64_stages(pp_solver *h, const double *rhs, double *field)
             case 4: PP_HIP_CHECK(launch_zsolve_cr_f64(z, s)); break;
             default: PP_HIP_CHECK(launch_zsolve_pcr_f64(h->g.nz, z, s)); break;
             }
-            if (zth) PP_HIP_CHECK(launch_zthomas_f64(z, h->d_zthomas, h->nzthomas, s));
+            if (zth) PP_HIP_CHECK(launch_zthomas_f64(z, h->d_zthomas, h->d_zthomas_tab, h->nzthomas, s));
         }
     }
     if ((rc = mark(h))) return rc;

@@ -132,15 +132,19 @@ hipError_t launch_zsolve_pcr_f32(int nz, const ZArgs &a, hipStream_t s)
 namespace pp {
 
 // The reference's Thomas sweep (thomasLowMem, src/triDiag.cpp:1693-1822, not the pinned
-// branch) on one listed FP64 column per block: the column is staged in LDS by the block,
-// swept serially by one thread in the reference's operation order (unit sub- and
-// super-diagonals; end rows eig - 1 for D, eig + 1 for N), and written back.  The PCR
-// kernel left these columns untouched (their |eig + 2| is below its tau).
-__global__ void __launch_bounds__(256) k_zthomas64(ZArgs a, const int4 *cols)
+// branch) on one listed FP64 column per block.  Its factors depend only on the column's
+// eigenvalue and end rows, so the host computes them in the reference's order (bet_0 = b0,
+// tmp_j = 1 / bet_{j-1}, bet_j = eig - tmp_j, the last row bn - tmp) and hands over
+// tab = {1 / bet_j, tmp_j} per column; the block stages the column and its factors in LDS and
+// one thread runs the two sweeps, each row one subtraction and one multiplication on the
+// serial chain.  The PCR kernel left these columns untouched (|eig + 2| below its tau).
+__global__ void __launch_bounds__(256) k_zthomas64(ZArgs a, const int4 *cols, const double *tab)
 {
     extern __shared__ double zth[];
-    double *rh = zth, *tmp = zth + a.nz;
+    const int n = a.nz;
+    double *rh = zth, *ib = zth + n, *tp = zth + 2 * n;
     const int4 sc = cols[blockIdx.x];
+    const double *t = tab + (long)blockIdx.x * 2 * n;
     // element i of the column: in place (stride SX*SY) or in its z tile (stride W)
     long zs = (long)a.SX * a.SY;
     long off = (long)sc.y * a.SX + sc.x;
@@ -151,34 +155,36 @@ __global__ void __launch_bounds__(256) k_zthomas64(ZArgs a, const int4 *cols)
         zs = W;
     }
     double *data = reinterpret_cast<double *>(a.data);
-    const int n = a.nz;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) rh[i] = data[off + (long)i * zs];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        rh[i] = data[off + (long)i * zs];
+        ib[i] = t[i];
+        tp[i] = t[n + i];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double eig = __dadd_rn(__dadd_rn(-2.0, __dmul_rn(a.lamx[sc.x], a.Cx)), __dmul_rn(a.lamy[sc.y], a.Cy));
-        const double b0 = a.zlo_dirichlet ? eig - 1. : eig + 1.;
-        const double bn = a.zhi_dirichlet ? eig - 1. : eig + 1.;
-        double bet;
-        rh[0] = rh[0] / (bet = b0);
-        for (int j = 1; j < n - 1; j++) {
-            tmp[j] = 1.0 / bet;
-            bet = eig - tmp[j];
-            rh[j] = (rh[j] - rh[j - 1]) / bet;
+        double x = rh[0] * ib[0];
+        rh[0] = x;
+#pragma unroll 8
+        for (int j = 1; j < n; j++) {
+            x = (rh[j] - x) * ib[j];
+            rh[j] = x;
         }
-        tmp[n - 1] = 1.0 / bet;
-        bet = bn - tmp[n - 1];
-        rh[n - 1] = (rh[n - 1] - rh[n - 2]) / bet;
-        for (int j = n - 2; j >= 0; j--) rh[j] -= tmp[j + 1] * rh[j + 1];
+#pragma unroll 8
+        for (int j = n - 2; j >= 0; j--) {
+            x = rh[j] - tp[j + 1] * x;
+            rh[j] = x;
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) data[off + (long)i * zs] = rh[i];
 }
 
-hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, int ncols, hipStream_t s)
+hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, const double *tab, int ncols, hipStream_t s)
 {
     if (ncols == 0) return hipSuccess;
     if (a.nz < 3 || a.nz > 1024 || a.items) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zthomas64, dim3((unsigned)ncols), dim3(256), 2 * (size_t)a.nz * sizeof(double), s, a, cols);
+    hipLaunchKernelGGL(k_zthomas64, dim3((unsigned)ncols), dim3(256), 3 * (size_t)a.nz * sizeof(double), s, a, cols,
+                       tab);
     return hipGetLastError();
 }
 

@@ -34,23 +34,31 @@ BC = "DDNDDD"
 
 # launch order of one solve and the algorithmic bytes per grid point of each kernel
 FP64_KERNELS = [("x_forward", 16), ("y_forward", 16), ("z_solve", 16), ("y_inverse", 16), ("x_inverse", 16)]
-MIXED_INIT = [("x_forward_f64_to_f32", 12), ("y_forward_f32", 8), ("z_solve_f32", 8), ("z_fix_f64", 0),
-              ("y_inverse_f32", 8), ("x_inverse_f32_to_f64", 12)]
-MIXED_STEP = [("residual_f64", 20), ("x_forward_f32", 8), ("y_forward_f32", 8), ("z_solve_f32", 8),
-              ("z_fix_f64", 0), ("y_inverse_f32", 8), ("x_inverse_accumulate", 20)]
-
-
 def progress(msg: str):
     """one line per finished phase on stderr (the JSON result goes to stdout)"""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+MIXED_INIT = [("x_forward_f64_to_f32", 12), ("y_forward_f32", 8), ("z_solve_f32", 8), ("z_fix_f64", 0),
+              ("y_inverse_f32", 8)]
+MIXED_STEP = [("x_forward_f32", 8), ("y_forward_f32", 8), ("z_solve_f32", 8), ("z_fix_f64", 0), ("y_inverse_f32", 8)]
+
+
 def kernel_table(precision: int, refine: int):
+    """launch order of one solve with the algorithmic bytes per grid point of each kernel.  The
+    mixed form holds the solution as FP32 parts while it is refined (u0, then the sum of the
+    corrections before the last); only the last store writes u in FP64."""
     if precision == 0:
         return FP64_KERNELS
     out = list(MIXED_INIT)
-    for r in range(refine):
-        out += [(f"{k}.{r + 1}", b) for k, b in MIXED_STEP]
+    out.append(("x_inverse_u0_f32", 8) if refine > 0 else ("x_inverse_f32_to_f64", 12))
+    for k in range(1, refine + 1):
+        out.append((f"residual_f64.{k}", 16 if k == 1 else 20))  # f, u0 (+ csum), r
+        out += [(f"{name}.{k}", b) for name, b in MIXED_STEP]
+        if k < refine:
+            out.append((f"x_inverse_csum_f32.{k}", 8 if k == 1 else 12))  # c (+ csum) -> csum
+        else:
+            out.append((f"x_inverse_final_f64.{k}", 16 if k == 1 else 20))  # c, u0 (+ csum) -> u
     return out
 
 

@@ -380,6 +380,9 @@ int BatchCore::plan()
     o_sol_ = take(c, (size_t)off * sizeof(double));
     if (precision == 0) o_tiles_ = take(c, (size_t)ntiles_ * sizeof(double));
     if (precision == 1) o_w_ = take(c, (size_t)off * sizeof(float));
+    // the solution's FP32 parts while refined (the refinement count may change after planning)
+    if (precision == 1) o_u0_ = take(c, (size_t)off * sizeof(float));
+    if (precision == 1) o_cacc_ = take(c, (size_t)off * sizeof(float));
     o_lam_ = take(c, lam_.size() * sizeof(double));
     o_dtab_ = take(c, dtab_.size() * sizeof(double));
     o_ls_ = take(c, ls_.size() * sizeof(LineSolve));
@@ -416,6 +419,8 @@ int BatchCore::bind(char *base, TableCache *tc)
     d_sol = (double *)(base + o_sol_);
     d_tiles = precision == 0 ? (double *)(base + o_tiles_) : nullptr;
     d_w = precision == 1 ? (float *)(base + o_w_) : nullptr;
+    d_u0 = precision == 1 ? (float *)(base + o_u0_) : nullptr;
+    d_cacc = precision == 1 ? (float *)(base + o_cacc_) : nullptr;
     d_lam_ = (double *)(base + o_lam_);
     d_dtab_ = (double *)(base + o_dtab_);
     d_ls_ = (LineSolve *)(base + o_ls_);
@@ -488,6 +493,8 @@ int BatchCore::bind(char *base, TableCache *tc)
             ResArgs &r = res_[i];
             r.f = d_rhs + m.offset;
             r.u = d_sol + m.offset;
+            r.u0 = d_u0 ? d_u0 + m.offset : nullptr;
+            r.u1 = d_cacc ? d_cacc + m.offset : nullptr;
             r.r = d_w + m.offset;
             r.nx = g.nx;
             r.ny = g.ny;
@@ -544,16 +551,19 @@ int BatchCore::upload(hipStream_t s)
 
 // one stage of line transforms over every class: in/out arenas, precision form, flags
 int BatchCore::run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
-                              void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled)
+                              void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled,
+                              const float *acc0, const float *acc1)
 {
     int rc;
     for (const HostClass &c : cls)
-        if ((rc = launch_class(s, c, form, axis, fwd, in, out, bflags, sums, plane, accumulate, ztiled))) return rc;
+        if ((rc = launch_class(s, c, form, axis, fwd, in, out, bflags, sums, plane, accumulate, ztiled, acc0, acc1)))
+            return rc;
     return PP_SUCCESS;
 }
 
 int BatchCore::launch_class(hipStream_t s, const HostClass &c, int form, int axis, int fwd, const void *in, void *out,
-                            int bflags, double *sums, const double *plane, int accumulate, int ztiled)
+                            int bflags, double *sums, const double *plane, int accumulate, int ztiled,
+                            const float *acc0, const float *acc1)
 {
     {
         const int v = (form == FORM_F64 && line_has_variant(c.key)) ? (force_var >= 0 ? force_var : c.var[fwd ? 0 : 1]) : 0;
@@ -577,6 +587,8 @@ int BatchCore::launch_class(hipStream_t s, const HostClass &c, int form, int axi
         a.line_sums = sums;
         a.plane_add = plane;
         a.accumulate = accumulate;
+        a.acc0 = acc0;
+        a.acc1 = acc1;
         a.ztiled = ztiled;
         // z tiles are 16 columns wide; y blocks of fewer columns fill them together
         a.tile_w = ztiled ? zsolve_cpb<double>() : 0;
@@ -777,28 +789,44 @@ int BatchCore::mixed_inner(hipStream_t s, Marks *mk)
     return mark(s, mk);
 }
 
-// the mixed form over the batch (same steps as the single-solve handle's)
+// the mixed form over the batch (same steps and solution parts as the single-solve handle's)
 int BatchCore::run_mixed(hipStream_t s, Marks *mk)
 {
     int rc;
     const int nm00 = (int)m00_.size();
     double *sums = nm00 ? d_sums_ : nullptr;
     const double *plane = nm00 ? d_plane_ : nullptr;
+    const int R = refine;
     if ((rc = run_line_stage(s, lx_, FORM_F64_TO_F32, 0, 1, d_rhs, d_w, BF_RHS, sums, nullptr, 0))) return rc;
     if ((rc = mark(s, mk))) return rc;
     if ((rc = mixed_inner(s, mk))) return rc;
     if (nm00) PPB_CHECK(launch_mode00_batch(d_m00_, nm00, nz_max_, 0, s));
-    if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 0))) return rc;
+    // u0 in FP32 while refinement follows, else the FP64 solution
+    if (R > 0) {
+        if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 0, d_w, d_u0, BF_SCALE, nullptr, plane, 0))) return rc;
+    } else {
+        if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 0))) return rc;
+    }
     if ((rc = mark(s, mk))) return rc;
-    for (int it = 0; it < refine; it++) {
-        PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), res_two_, s));
-        PPB_CHECK(launch_residual_pair_f32_batch(d_res_, d_rpitems_, (int)rpitems_.size(), resp_two_, s));
+    for (int it = 0; it < R; it++) {
+        const int src = it == 0 ? RES_U0 : RES_U0_U1;
+        PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), res_two_, src, s));
+        PPB_CHECK(launch_residual_pair_f32_batch(d_res_, d_rpitems_, (int)rpitems_.size(), resp_two_, src, s));
         if ((rc = mark(s, mk))) return rc;
         if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0))) return rc;
         if ((rc = mark(s, mk))) return rc;
         if ((rc = mixed_inner(s, mk))) return rc;
         if (nm00) PPB_CHECK(launch_mode00_batch(d_m00_, nm00, nz_max_, 1, s));
-        if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 1))) return rc;
+        if (it == R - 1) {
+            // u = u0 + csum + c, in FP64
+            if ((rc = run_line_stage(s, lx_, FORM_F32_TO_F64, 0, 0, d_w, d_sol, BF_SCALE, nullptr, plane, 0, 0, d_u0,
+                                     it > 0 ? d_cacc : nullptr)))
+                return rc;
+        } else {
+            // csum = c1, then csum += c
+            if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 0, d_w, d_cacc, BF_SCALE, nullptr, plane, it > 0 ? 1 : 0)))
+                return rc;
+        }
         if ((rc = mark(s, mk))) return rc;
     }
     return PP_SUCCESS;

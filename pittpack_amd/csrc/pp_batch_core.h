@@ -83,6 +83,7 @@ class BatchCore {
     double *d_rhs = nullptr, *d_sol = nullptr;
     double *d_tiles = nullptr;  // FP64: z tiles between the y transforms and the z solve
     float *d_w = nullptr;
+    float *d_u0 = nullptr, *d_cacc = nullptr;  // mixed form: the solution's FP32 parts while refined
     long arena_elems = 0;
     int launches[3] = {0, 0, 0};
 
@@ -101,13 +102,15 @@ class BatchCore {
         void *d_items_alt = nullptr;
     };
     int launch_class(hipStream_t s, const HostClass &c, int form, int axis, int fwd, const void *in, void *out,
-                     int bflags, double *sums, const double *plane, int accumulate, int ztiled);
+                     int bflags, double *sums, const double *plane, int accumulate, int ztiled,
+                     const float *acc0 = nullptr, const float *acc1 = nullptr);
     template <typename R>
     int plan_lines(int axis, std::vector<HostClass> &out);
     void plan_z();
     int plan_mixed();
     int run_line_stage(hipStream_t s, std::vector<HostClass> &cls, int form, int axis, int fwd, const void *in,
-                       void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled = 0);
+                       void *out, int bflags, double *sums, const double *plane, int accumulate, int ztiled = 0,
+                       const float *acc0 = nullptr, const float *acc1 = nullptr);
     int run_z_stage(hipStream_t s, bool f32);
     int run_fp64(hipStream_t s, Marks *mk);
     int mixed_inner(hipStream_t s, Marks *mk);
@@ -140,7 +143,7 @@ class BatchCore {
     int nz_max_ = 0;
     long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)
-    size_t o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
+    size_t o_u0_ = 0, o_cacc_ = 0, o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
            o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0, o_rpitems_ = 0;
     std::map<int, AxisDev> tabs_;
     // device pointers of the tables

@@ -71,6 +71,8 @@ struct pp_solver {
     int refine = 2;
     double tau = 0.05;
     float *d_w = nullptr;          // FP32 work field
+    float *d_u0 = nullptr;         // FP32 initial solution of the refinement (u = u0 + c1 + ... + cR)
+    float *d_cacc = nullptr;       // FP32 sum of the corrections before the last
     float *d_wt = nullptr;         // FP32 z tiles (32 columns x 16 J rows) between the y transforms
     int wtiles = 1;                // mixed form: hand the z solve FP32 z tiles where the sizes allow
     int wvar = 0;                  // mixed form, FP32 tiled y stages: 1 = half the threads per column

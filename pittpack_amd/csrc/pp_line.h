@@ -75,6 +75,10 @@ struct LineArgs {
     // x inverse only: out = (accumulate ? out : 0) + out_scale * value + plane_add[z]
     const double *plane_add;
     int accumulate;
+    // x inverse from FP32 to FP64 (the mixed form's last store): out = acc0 + acc1 +
+    // out_scale * value + plane_add[z], the FP32 parts of the solution summed in FP64 (acc1
+    // may be null; null acc0: no such parts)
+    const float *acc0, *acc1;
     // batched launch (null: one solve, grid = blocks): per-block items, per-solve fields,
     // and which per-solve scalings apply (BF_RHS, BF_SCALE)
     const LineItem *items;
@@ -208,6 +212,8 @@ __device__ __forceinline__ void line_block(const LineArgs &a0, LineArgs &a, int 
         a.out_scale = (a0.bflags & BF_SCALE) ? sv.inv_scale : 1.0;
         if (a0.line_sums) a.line_sums = a0.line_sums + sv.sums_off;
         if (a0.plane_add) a.plane_add = a0.plane_add + sv.plane_off;
+        if (a0.acc0) a.acc0 = a0.acc0 + sv.offset;
+        if (a0.acc1) a.acc1 = a0.acc1 + sv.offset;
         bx = it.bx;
         by = it.by;
     }
@@ -405,6 +411,12 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
         if (a.accumulate) {
 #pragma unroll
             for (int c = 0; c < 2 * E; c++) accv[c] = (double)out[base + (long)(t + T * c) * stride];
+        } else if (a.acc0) {
+#pragma unroll
+            for (int c = 0; c < 2 * E; c++) {
+                const long i = base + (long)(t + T * c) * stride;
+                accv[c] = (double)a.acc0[i] + (a.acc1 ? (double)a.acc1[i] : 0.0);
+            }
         }
         if (a.plane_add) padd = a.plane_add[lz];
     }
@@ -687,7 +699,7 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
             if constexpr (XINV) {
                 val = a.out_scale * val;
                 if (a.plane_add) val += padd;
-                if (a.accumulate) val += accv[c];
+                if (a.accumulate || a.acc0) val += accv[c];
                 out[base + (long)n * stride] = (TOut)val;
             } else {
                 put(n, val);

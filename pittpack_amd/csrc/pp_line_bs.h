@@ -135,6 +135,13 @@ __global__ void __launch_bounds__(bs_threads(L) * LINES) k_line_bs(LineArgs a0)
                 const int n = t + T * c;
                 accv[c] = n < N ? (double)out[ad.base + (long)n * ad.stride] : 0.0;
             }
+        } else if (a.acc0) {
+#pragma unroll
+            for (int c = 0; c < H; c++) {
+                const int n = t + T * c;
+                const long i = ad.base + (long)(n < N ? n : 0) * ad.stride;
+                accv[c] = n < N ? (double)a.acc0[i] + (a.acc1 ? (double)a.acc1[i] : 0.0) : 0.0;
+            }
         }
         if (a.plane_add) padd = a.plane_add[ad.lz];
     }
@@ -238,7 +245,7 @@ __global__ void __launch_bounds__(bs_threads(L) * LINES) k_line_bs(LineArgs a0)
         if constexpr (AX == 0 && !FWD) {
             if constexpr (XINV) {
                 if (a.plane_add) val += padd;
-                if (a.accumulate) val += accv[m];
+                if (a.accumulate || a.acc0) val += accv[m];
             } else {
                 if (a.plane_add) val += a.plane_add[ad.lz];
                 if (a.accumulate) val += (double)out[ad.base + (long)pos * ad.stride];

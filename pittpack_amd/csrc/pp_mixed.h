@@ -9,7 +9,9 @@ namespace pp {
 // arguments of the FP64 residual r = f_eff - L u
 struct ResArgs {
     const double *f;  // right-hand side as given (unscaled)
-    const double *u;  // current solution
+    const double *u;  // current solution, FP64 (the residual launch's source kind RES_U64)
+    // or its FP32 parts: u0 (RES_U0) or u0 + u1 (RES_U0_U1), summed in FP64
+    const float *u0, *u1;
     void *r;          // residual out (FP32)
     int nx, ny, nz;
     double c3;        // dz: f_eff = f*dz*dz - Dirichlet term
@@ -25,6 +27,9 @@ struct ResArgs {
     // nz-1 for periodic z (null: not wanted); residual_parts(nx, ny) entries
     double *upart;
 };
+
+// where the residual kernel reads the current solution from
+enum ResSource : int { RES_U64 = 0, RES_U0 = 1, RES_U0_U1 = 2 };
 
 // per-block partial sums of u over plane 0 written by the residual kernel
 int residual_parts(int nx, int ny);
@@ -52,16 +57,18 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
 // two: some member has nx = 1 (mod residual_tile_x()) or ny = 1 (mod residual_tile_y())
 // (residual_two_halo), which takes a second halo load per direction
 bool residual_two_halo(int nx, int ny);
-hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s);
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, int src,
+                                     hipStream_t s);
 // grids with even nx take the two-cells-per-lane kernel (residual_tile_x_of(nx) = 128 columns
 // per tile, partial sums in the same layout); its batched launch: items {member, bx, by, bz}
 // over 128-column tiles, twoy: some member has ny = 1 (mod residual_tile_y())
 bool residual_pairs(int nx);
 int residual_tile_x_of(int nx);
-hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy,
+hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy, int src,
                                           hipStream_t s);
 hipError_t launch_mode00(const double *line_sums, int ny, int nz, double b0, double bn, double inv_scale,
                          double *plane_add, const double *upart, int nparts, int correct, int zper, hipStream_t s);
-hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s);
+// src: ResSource (which of a.u / a.u0 / a.u1 hold the current solution)
+hipError_t launch_residual_f32(const ResArgs &a, int src, hipStream_t s);
 
 }  // namespace pp

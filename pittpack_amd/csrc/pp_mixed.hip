@@ -235,7 +235,7 @@ constexpr int RES_TY = 4;   // ... four rows
 // TWO: grids with nx = 1 (mod 64) or ny = 1 (mod 4), whose last column (row) is the first lane
 // (wave) of its tile and so needs both neighbours of that direction from memory: one more
 // halo load per direction and plane.
-template <typename TOut, bool TWO>
+template <typename TOut, bool TWO, int SRC>
 __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
     ResArgs a = a0;
@@ -266,12 +266,21 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     // lanes outside the grid work on the clamped cell (in bounds) and store nothing
     const int xc = min(x, a.nx - 1), yc = min(y, a.ny - 1);
     const long cell = xc + sy * yc;
-    gdouble *uc_p = (gdouble *)(a.u + cell);
+    // the current solution: FP64, or its FP32 parts summed in FP64 (SRC, ResSource)
+    using gfloat = const __attribute__((address_space(1))) float;
+    gdouble *uc_p = SRC == RES_U64 ? (gdouble *)(a.u + cell) : nullptr;
+    gfloat *u0_p = SRC != RES_U64 ? (gfloat *)(a.u0 + cell) : nullptr;
+    gfloat *u1_p = SRC == RES_U0_U1 ? (gfloat *)(a.u1 + cell) : nullptr;
+    auto ldu = [&](long o) -> double {
+        if constexpr (SRC == RES_U64) return uc_p[o];
+        else if constexpr (SRC == RES_U0) return (double)u0_p[o];
+        else return (double)u0_p[o] + (double)u1_p[o];
+    };
     gdouble *f_p = (gdouble *)(a.f + cell);
     __shared__ double red[256];
     if (a.upart && bz == 0) {
         // plane-0 sum of u (periodic z: planes 0 and nz-1), one partial per (x, y) tile
-        red[threadIdx.x] = valid ? uc_p[0] + (a.bc[4] == 'P' ? uc_p[sz * (a.nz - 1)] : 0.0) : 0.0;
+        red[threadIdx.x] = valid ? ldu(0) + (a.bc[4] == 'P' ? ldu(sz * (a.nz - 1)) : 0.0) : 0.0;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
             if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -322,10 +331,10 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     const double defect = a.pin_defect ? a.pin_defect[0] : 0.0;
     gout *r_p = (gout *)(reinterpret_cast<TOut *>(a.r) + cell);
 
-    double uc = uc_p[sz * z0];
+    double uc = ldu(sz * z0);
     double um;
-    if (z0 > 0) um = uc_p[sz * (z0 - 1)];
-    else um = a.bc[4] == 'P' ? uc_p[sz * (a.nz - 1)] : (a.bc[4] == 'N' ? uc : -uc);
+    if (z0 > 0) um = ldu(sz * (z0 - 1));
+    else um = a.bc[4] == 'P' ? ldu(sz * (a.nz - 1)) : (a.bc[4] == 'N' ? uc : -uc);
     // loads of one plane: u(z+1) (at the top face: plane 0 for P, else this plane, replaced
     // by the ghost below), the right-hand side and the halo value per direction
     struct Plane {
@@ -334,13 +343,13 @@ __global__ void __launch_bounds__(256) k_residual(ResArgs a0, const ResArgs *sol
     const bool ztop_wrap = a.bc[5] == 'P';
     auto fetch = [&](int z, Plane &q) {
         const long o = sz * z;
-        q.up = uc_p[z + 1 < a.nz ? o + sz : (ztop_wrap ? 0 : o)];
+        q.up = ldu(z + 1 < a.nz ? o + sz : (ztop_wrap ? 0 : o));
         q.f = f_p[o];
-        q.hx = uc_p[o + ohx];
-        q.hy = uc_p[o + ohy];
+        q.hx = ldu(o + ohx);
+        q.hy = ldu(o + ohy);
         if constexpr (TWO) {
-            q.hx2 = uc_p[o + ohx2];
-            q.hy2 = uc_p[o + ohy2];
+            q.hx2 = ldu(o + ohx2);
+            q.hy2 = ldu(o + ohy2);
         } else {
             q.hx2 = q.hx;
             q.hy2 = q.hy;
@@ -426,7 +435,7 @@ typedef double rp_d2 __attribute__((ext_vector_type(2)));
 typedef float rp_f2 __attribute__((ext_vector_type(2)));
 constexpr int RP_CX = 2 * RES_TX;  // columns per block tile
 
-template <bool TWOY>
+template <bool TWOY, int SRC>
 __global__ void __launch_bounds__(256) k_residual_pair(ResArgs a0, const ResArgs *solves, const int4 *items)
 {
     ResArgs a = a0;
@@ -456,9 +465,31 @@ __global__ void __launch_bounds__(256) k_residual_pair(ResArgs a0, const ResArgs
     const long cell = xc + sy * yc;
     // every offset below counts doubles (floats for r) from the pair's first cell; pairs are
     // read and written through a vector cast of the element address
-    gdouble *uc_p = (gdouble *)(a.u + cell);
-    const double *u_c = a.u + cell, *f_c = a.f + cell;
-    auto ldu2 = [&](long o) { return *(gd2 *)(u_c + o); };
+    // the current solution: FP64, or its FP32 parts summed in FP64 (SRC, ResSource)
+    using gfloat = const __attribute__((address_space(1))) float;
+    using gpf2 = const __attribute__((address_space(1))) rp_f2;
+    const double *u_c = SRC == RES_U64 ? a.u + cell : nullptr, *f_c = a.f + cell;
+    const float *u0_c = SRC != RES_U64 ? a.u0 + cell : nullptr;
+    const float *u1_c = SRC == RES_U0_U1 ? a.u1 + cell : nullptr;
+    auto ldu = [&](long o) -> double {
+        if constexpr (SRC == RES_U64) return *(gdouble *)(u_c + o);
+        else if constexpr (SRC == RES_U0) return (double)*(gfloat *)(u0_c + o);
+        else return (double)*(gfloat *)(u0_c + o) + (double)*(gfloat *)(u1_c + o);
+    };
+    auto ldu2 = [&](long o) -> rp_d2 {
+        if constexpr (SRC == RES_U64) {
+            return *(gd2 *)(u_c + o);
+        } else {
+            const rp_f2 p = *(gpf2 *)(u0_c + o);
+            rp_d2 v = {(double)p.x, (double)p.y};
+            if constexpr (SRC == RES_U0_U1) {
+                const rp_f2 q = *(gpf2 *)(u1_c + o);
+                v.x += (double)q.x;
+                v.y += (double)q.y;
+            }
+            return v;
+        }
+    };
     auto ldf2 = [&](long o) { return *(gd2 *)(f_c + o); };
     __shared__ double red[256];
     if (a.upart && bz == 0) {
@@ -545,8 +576,8 @@ __global__ void __launch_bounds__(256) k_residual_pair(ResArgs a0, const ResArgs
         const long o = sz * z;
         q.up = ldu2(z + 1 < a.nz ? o + sz : (ztop_wrap ? 0 : o));
         q.f = ldf2(o);
-        q.hl = uc_p[o + ohl];
-        q.hr = uc_p[o + ohr];
+        q.hl = ldu(o + ohl);
+        q.hr = ldu(o + ohr);
         q.hy = ldu2(o + ohy);
         if constexpr (TWOY) q.hy2 = ldu2(o + ohy2);
         else q.hy2 = q.hy;
@@ -645,13 +676,39 @@ bool residual_pairs(int nx) { return nx % 2 == 0 && nx >= 2; }
 
 int residual_tile_x_of(int nx) { return residual_pairs(nx) ? RP_CX : RES_TX; }
 
-hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy, hipStream_t s)
+template <bool TWOY>
+static hipError_t launch_pair_src(dim3 grid, const ResArgs &a, const ResArgs *solves, const int4 *items, int src,
+                                  hipStream_t s)
+{
+    switch (src) {
+    case RES_U0: hipLaunchKernelGGL((k_residual_pair<TWOY, RES_U0>), grid, dim3(256), 0, s, a, solves, items); break;
+    case RES_U0_U1: hipLaunchKernelGGL((k_residual_pair<TWOY, RES_U0_U1>), grid, dim3(256), 0, s, a, solves, items); break;
+    default: hipLaunchKernelGGL((k_residual_pair<TWOY, RES_U64>), grid, dim3(256), 0, s, a, solves, items); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool TWO>
+static hipError_t launch_res_src(dim3 grid, const ResArgs &a, const ResArgs *solves, const int4 *items, int src,
+                                 hipStream_t s)
+{
+    switch (src) {
+    case RES_U0: hipLaunchKernelGGL((k_residual<float, TWO, RES_U0>), grid, dim3(256), 0, s, a, solves, items); break;
+    case RES_U0_U1:
+        hipLaunchKernelGGL((k_residual<float, TWO, RES_U0_U1>), grid, dim3(256), 0, s, a, solves, items);
+        break;
+    default: hipLaunchKernelGGL((k_residual<float, TWO, RES_U64>), grid, dim3(256), 0, s, a, solves, items); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_pair_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool twoy, int src,
+                                          hipStream_t s)
 {
     if (nitems == 0) return hipSuccess;
     ResArgs a{};
-    if (twoy) hipLaunchKernelGGL(k_residual_pair<true>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
-    else hipLaunchKernelGGL(k_residual_pair<false>, dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
-    return hipGetLastError();
+    return twoy ? launch_pair_src<true>(dim3((unsigned)nitems), a, solves, items, src, s)
+                : launch_pair_src<false>(dim3((unsigned)nitems), a, solves, items, src, s);
 }
 
 int residual_parts(int nx, int ny) { return ((nx + RES_TX - 1) / RES_TX) * ((ny + RES_TY - 1) / RES_TY); }
@@ -666,29 +723,27 @@ int residual_blocks(int nx, int ny, int nz) { return residual_parts(nx, ny) * ((
 
 bool residual_two_halo(int nx, int ny) { return nx % RES_TX == 1 || ny % RES_TY == 1; }
 
-hipError_t launch_residual_f32(const ResArgs &a, hipStream_t s)
+hipError_t launch_residual_f32(const ResArgs &a, int src, hipStream_t s)
 {
     if (residual_pairs(a.nx)) {
         const dim3 grid((unsigned)((a.nx + RP_CX - 1) / RP_CX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
                         (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
-        if (a.ny % RES_TY == 1) hipLaunchKernelGGL(k_residual_pair<true>, grid, dim3(256), 0, s, a, nullptr, nullptr);
-        else hipLaunchKernelGGL(k_residual_pair<false>, grid, dim3(256), 0, s, a, nullptr, nullptr);
-        return hipGetLastError();
+        return a.ny % RES_TY == 1 ? launch_pair_src<true>(grid, a, nullptr, nullptr, src, s)
+                                  : launch_pair_src<false>(grid, a, nullptr, nullptr, src, s);
     }
     dim3 grid((unsigned)((a.nx + RES_TX - 1) / RES_TX), (unsigned)((a.ny + RES_TY - 1) / RES_TY),
               (unsigned)((a.nz + RES_ZC - 1) / RES_ZC));
-    if (residual_two_halo(a.nx, a.ny)) hipLaunchKernelGGL((k_residual<float, true>), grid, dim3(256), 0, s, a, nullptr, nullptr);
-    else hipLaunchKernelGGL((k_residual<float, false>), grid, dim3(256), 0, s, a, nullptr, nullptr);
-    return hipGetLastError();
+    return residual_two_halo(a.nx, a.ny) ? launch_res_src<true>(grid, a, nullptr, nullptr, src, s)
+                                         : launch_res_src<false>(grid, a, nullptr, nullptr, src, s);
 }
 
-hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, hipStream_t s)
+hipError_t launch_residual_f32_batch(const ResArgs *solves, const int4 *items, int nitems, bool two, int src,
+                                     hipStream_t s)
 {
     if (nitems == 0) return hipSuccess;
     ResArgs a{};
-    if (two) hipLaunchKernelGGL((k_residual<float, true>), dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
-    else hipLaunchKernelGGL((k_residual<float, false>), dim3((unsigned)nitems), dim3(256), 0, s, a, solves, items);
-    return hipGetLastError();
+    return two ? launch_res_src<true>(dim3((unsigned)nitems), a, solves, items, src, s)
+               : launch_res_src<false>(dim3((unsigned)nitems), a, solves, items, src, s);
 }
 
 }  // namespace pp

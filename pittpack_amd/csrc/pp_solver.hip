@@ -169,7 +169,10 @@ void free_mixed(pp_solver *h)
 {
     if (h->d_w) (void)hipFree(h->d_w);
     if (h->d_wt) (void)hipFree(h->d_wt);
+    if (h->d_u0) (void)hipFree(h->d_u0);
+    if (h->d_cacc) (void)hipFree(h->d_cacc);
     h->d_wt = nullptr;
+    h->d_u0 = h->d_cacc = nullptr;
     if (h->d_aux) (void)hipFree(h->d_aux);
     if (h->d_line_sums) (void)hipFree(h->d_line_sums);
     if (h->d_plane_add) (void)hipFree(h->d_plane_add);
@@ -734,6 +737,12 @@ int ensure_mixed(pp_solver *h)
     // freed only by pp_destroy, so no pointer handed to a solve can be released under it
     const size_t N = (size_t)h->g.nx * h->g.ny * h->g.nz;
     if (!h->d_w && hipMalloc((void **)&h->d_w, N * sizeof(float)) != hipSuccess) return PP_ALLOCATION_FAIL;
+    // the solution's FP32 parts while it is refined (u0, then the sum of the corrections
+    // before the last): the last store writes u = u0 + csum + c in FP64
+    if (h->refine > 0 && !h->d_u0 && hipMalloc((void **)&h->d_u0, N * sizeof(float)) != hipSuccess)
+        return PP_ALLOCATION_FAIL;
+    if (h->refine > 1 && !h->d_cacc && hipMalloc((void **)&h->d_cacc, N * sizeof(float)) != hipSuccess)
+        return PP_ALLOCATION_FAIL;
     if (!h->d_line_sums &&
         hipMalloc((void **)&h->d_line_sums, (size_t)h->g.ny * h->g.nz * sizeof(double)) != hipSuccess)
         return PP_ALLOCATION_FAIL;
@@ -876,13 +885,19 @@ int mixed_inner(pp_solver *h)
 
 // Mixed precision: FP32 transforms and z solves (FP64 twiddles, eigenvalues and
 // near-singular columns), the pinned (0,0) column exactly in FP64, then `refine` steps of
-// u += solve32(f_eff - L u) with the residual in FP64.  f and u must not alias.
+// u += solve32(f_eff - L u) with the residual in FP64.  The solution is held as FP32 parts
+// while it is refined -- u = u0 + csum + c_k, u0 the initial FP32 solve, csum the corrections
+// before the last (each summed into it in FP32), the residual reading the parts and adding them
+// in FP64 -- and only the last store writes u in FP64 (u0 + csum + c_R); csum's FP32 rounding
+// (eps32 |c| ~ 1e-14 |u| after the first correction) is what the next step corrects.  f and u
+// must not alias.
 int run_mixed(pp_solver *h, const double *f, double *u)
 {
     hipStream_t s = h->stream;
     int rc = ensure_mixed(h);
     if (rc) return rc;
     if ((rc = begin_timing(h))) return rc;
+    const int R = h->refine;
     // initial solve: x forward reads the FP64 right-hand side
     {
         LineArgs a = line_args(h, 0, nullptr);
@@ -902,21 +917,25 @@ int run_mixed(pp_solver *h, const double *f, double *u)
                                    h->g.zper, s));
     }
     {
+        // u0 in FP32 while refinement follows, else the FP64 solution
         LineArgs a = line_args(h, 0, nullptr);
         a.in = h->d_w;
-        a.out = u;
+        a.out = R > 0 ? (void *)h->d_u0 : (void *)u;
         a.out_scale = 1. / (h->g.scale);
         a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
         a.accumulate = 0;
-        PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
+        PP_HIP_CHECK(launch_line(R > 0 ? FORM_F32 : FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
     }
     if ((rc = mark(h))) return rc;
     const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
     const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
-    for (int it = 0; it < h->refine; it++) {
+    for (int it = 0; it < R; it++) {
+        const bool last = it == R - 1;
         ResArgs r{};
         r.f = f;
-        r.u = u;
+        r.u = nullptr;
+        r.u0 = h->d_u0;
+        r.u1 = h->d_cacc;
         r.r = h->d_w;
         r.nx = h->g.nx;
         r.ny = h->g.ny;
@@ -929,7 +948,7 @@ int run_mixed(pp_solver *h, const double *f, double *u)
         r.dmask = h->g.dmask;
         r.pin_defect = h->pinned00 ? h->d_plane_add + h->g.nz : nullptr;
         r.upart = h->pinned00 ? h->d_upart : nullptr;
-        PP_HIP_CHECK(launch_residual_f32(r, s));
+        PP_HIP_CHECK(launch_residual_f32(r, it == 0 ? RES_U0 : RES_U0_U1, s));
         if ((rc = mark(h))) return rc;
         {
             LineArgs a = line_args(h, 0, nullptr);
@@ -951,11 +970,21 @@ int run_mixed(pp_solver *h, const double *f, double *u)
         {
             LineArgs a = line_args(h, 0, nullptr);
             a.in = h->d_w;
-            a.out = u;
             a.out_scale = 1. / (h->g.scale);
             a.plane_add = h->pinned00 ? h->d_plane_add : nullptr;
-            a.accumulate = 1;
-            PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
+            if (last) {
+                // u = u0 + csum + c, in FP64
+                a.out = u;
+                a.accumulate = 0;
+                a.acc0 = h->d_u0;
+                a.acc1 = it > 0 ? h->d_cacc : nullptr;
+                PP_HIP_CHECK(launch_line(FORM_F32_TO_F64, h->g.kind[0], h->g.nx, 0, 0, a, s));
+            } else {
+                // csum = c1, then csum += c
+                a.out = h->d_cacc;
+                a.accumulate = it > 0 ? 1 : 0;
+                PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[0], h->g.nx, 0, 0, a, s));
+            }
         }
         if ((rc = mark(h))) return rc;
     }
@@ -1336,6 +1365,7 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_REFINE:
         if (value < 0 || value > 16) return PP_INPUT_ARGS;
         h->refine = (int)value;
+        h->mixed_ready = false;  // the FP32 solution parts it needs
         return PP_SUCCESS;
     case PP_OPT_TIMING: {
         if (value < 0 || value > 1024) return PP_INPUT_ARGS;

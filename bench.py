@@ -51,14 +51,15 @@ def kernel_table(precision: int, refine: int):
     if precision == 0:
         return FP64_KERNELS
     out = list(MIXED_INIT)
-    out.append(("x_inverse_u0_f32", 8) if refine > 0 else ("x_inverse_f32_to_f64", 12))
+    out.append(("x_inverse_f32", 8) if refine > 0 else ("x_inverse_final_f64", 12))  # c -> u0 (or u)
     for k in range(1, refine + 1):
-        out.append((f"residual_f64.{k}", 16 if k == 1 else 20))  # f, u0 (+ csum), r
+        # f, u0 (+ the correction sum), r
+        out.append((f"residual_u0.{k}", 16) if k == 1 else (f"residual_u0_csum.{k}", 20))
         out += [(f"{name}.{k}", b) for name, b in MIXED_STEP]
         if k < refine:
-            out.append((f"x_inverse_csum_f32.{k}", 8 if k == 1 else 12))  # c (+ csum) -> csum
+            out.append((f"x_inverse_f32.{k}", 8 if k == 1 else 12))  # c (+ sum) -> sum
         else:
-            out.append((f"x_inverse_final_f64.{k}", 16 if k == 1 else 20))  # c, u0 (+ csum) -> u
+            out.append((f"x_inverse_final_f64.{k}", 16 if k == 1 else 20))  # c, u0 (+ sum) -> u
     return out
 
 

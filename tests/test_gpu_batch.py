@@ -224,6 +224,34 @@ def test_batch_mixed_precision(zero_mode):
             assert rel_maxdiff(got, single) <= 1e-14, f"member {i}: batched vs single mixed"
 
 
+@pytest.mark.parametrize("refine", [1, 3])
+def test_batch_mixed_refine_counts(refine):
+    """the solution's FP32 parts with one refinement step (u0 + c, written in FP64 at once) and
+    with three (the second correction summed into the first in FP32): each member agrees with
+    the single-solve mixed handle (whose FP32 y stages may take 32-column z tiles: the same
+    answer to the refinement's accuracy) and with the oracle, at the FP64 bar after three steps
+    and at FP32 squared (1e-11) after one; the refinement count may change after finalize"""
+    print(f"batch mixed refine={refine}", flush=True)
+    with pa.Batch(precision=1) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in MEMBERS[:6]]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS[:6]):
+            f = prng.uniform_pm1((nz, ny, nx), seed=500 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        B.set_option(pa.OPT_REFINE, refine)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS[:6]):
+            got = B.get_solution(i)
+            bar = TOL if refine >= 2 else 1e-11
+            single = pa.solve(rhs[i], bc, box, precision=1, refine=refine)
+            assert rel_maxdiff(got, single) <= bar, f"member {i}: batched vs single {rel_maxdiff(got, single):.2e}"
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+            assert rel_maxdiff(got, ref) <= bar, f"member {i} {bc}: {rel_maxdiff(got, ref):.2e}"
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_graph_replays_match_eager(precision):
     """PP_OPT_GRAPH: the second solve captures the batch's launches into a HIP graph and later

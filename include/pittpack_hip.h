@@ -106,12 +106,13 @@ enum {
     PP_OPT_MAX_OVERTAKE = 13,
     /* pp_io file format: PP_IO_HDF5 (default) or PP_IO_RAW */
     PP_OPT_IO_FORMAT = 14,
-    /* pp_set_option_double, FP64 form with z solver 0 or 1, columns up to 512 rows: columns
-     * with |eig+2| below this threshold (not the pinned one) are solved by the reference's
-     * serial Thomas (thomasLowMem, src/triDiag.cpp:1693-1822) instead of the PCR steps, which
-     * lose up to ten times more digits on nearly singular columns (strongly anisotropic cells,
-     * long Neumann axes).  0 (default) = off; 1e-3 covers the columns where the two differ
-     * beyond 1e-13. */
+    /* pp_set_option_double / pp_batch_set_option_double (before finalize), FP64 form with z
+     * solver 0 or 1, columns up to 512 rows: columns with |eig+2| below this threshold (not the
+     * pinned one) are solved by the reference's Thomas sweeps (thomasLowMem,
+     * src/triDiag.cpp:1693-1822, its factors in the reference's order) instead of the PCR
+     * steps, which lose up to ten times more digits on nearly singular columns (long Neumann or
+     * periodic axes, cells thinner in z).  Default 1e-3 (the columns where the two differ beyond
+     * 1e-13); 0 = off. */
     PP_OPT_ZTHOMAS_TAU = 15
 };
 
@@ -274,7 +275,8 @@ int pp_copy_device(const double *d_in, double *d_out, long long n, int mode, voi
  * transform kind for x and y, rows per lane for z) over all members of the class.  A
  * member's FP64 result is bitwise the one the single-solve handle computes.
  * Order: pp_batch_create, [pp_batch_set_option(PP_OPT_PRECISION / ZERO_MODE),
- * pp_batch_set_option_double(PP_OPT_TAU)], pp_batch_add (each member), pp_batch_finalize,
+ * pp_batch_set_option_double(PP_OPT_TAU / PP_OPT_ZTHOMAS_TAU)], pp_batch_add (each member),
+ * pp_batch_finalize,
  * fill the right-hand sides (pp_batch_set_rhs / pp_batch_field / pp_batch_fill_uniform),
  * pp_batch_solve (asynchronous on the batch's stream; right-hand sides kept), read
  * (pp_batch_get_solution / pp_batch_solution).

@@ -343,7 +343,7 @@ class Batch:
 
     def __init__(self, precision: int = 0, refine: int | None = None, tau: float | None = None,
                  zero_mode: int | None = None, graph: int | None = None, autotune: int | None = None,
-                 line_variant: int | None = None):
+                 line_variant: int | None = None, zthomas: float | None = None):
         self._L = load_library()
         self._h = ctypes.c_void_p()
         _check(self._L.pp_batch_create(ctypes.byref(self._h)), "pp_batch_create")
@@ -360,6 +360,9 @@ class Batch:
             self.set_option(OPT_REFINE, refine)
         if tau is not None:
             _check(self._L.pp_batch_set_option_double(self._h, OPT_TAU, float(tau)), "batch_set_option_double")
+        if zthomas is not None:
+            _check(self._L.pp_batch_set_option_double(self._h, OPT_ZTHOMAS_TAU, float(zthomas)),
+                   "batch_set_option_double")
         if zero_mode is not None:
             self.set_option(OPT_ZERO_MODE, zero_mode)
 

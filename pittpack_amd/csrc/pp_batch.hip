@@ -172,6 +172,8 @@ void BatchCore::plan_z()
     const int cpb = precision == 1 ? zsolve_cpb<float>() : zsolve_cpb<double>();
     zpl_.clear();
     zlong_.clear();
+    zth_.clear();
+    zth_tab_.clear();
     smt_.clear();
     zc2_.clear();
     zc2_off_.clear();
@@ -215,6 +217,23 @@ void BatchCore::plan_z()
         if (precision == 0 && zsolve_long_supported_f64(g.nz)) {
             zlong_.push_back(m);
             continue;
+        }
+        if (precision == 0 && zthomas_tau > 0.0) {
+            // nearly singular columns, as the single-solve handle lists them
+            const std::vector<double> lx = eig_factors(g, 0), ly = eig_factors(g, 1);
+            const double Cx = (g.dxyz[2] / g.dxyz[0]) * (g.dxyz[2] / g.dxyz[0]);
+            const double Cy = (g.dxyz[2] / g.dxyz[1]) * (g.dxyz[2] / g.dxyz[1]);
+            for (int sy = 0; sy < g.ny; sy++)
+                for (int sx = 0; sx < g.nx; sx++) {
+                    const double eig = column_eig(lx, ly, sx, sy, Cx, Cy);
+                    bool pin = std::fabs(eig + 2.) < 1.e-10;
+                    if (pin && zero_mode == 1 && (g.zlo_d || g.zhi_d)) pin = false;
+                    if (pin || !(std::fabs(eig + 2.) < zthomas_tau)) continue;
+                    zth_.push_back({sx, sy, m, 0, (long)zth_tab_.size()});
+                    zth_tab_.resize(zth_tab_.size() + 2 * (size_t)g.nz);
+                    zthomas_factors(eig, g.zlo_d ? eig - 1. : eig + 1., g.zhi_d ? eig - 1. : eig + 1., g.nz,
+                                    zth_tab_.data() + zth_tab_.size() - 2 * (size_t)g.nz);
+                }
         }
         auto &items = cls[(g.nz + 15) / 16];
         const int nbx = (g.nx + cpb - 1) / cpb;
@@ -395,6 +414,8 @@ int BatchCore::plan()
     for (HostClass &h : lz_) h.off = take(c, h.zi.size() * sizeof(ZItem));
     for (HostClass &h : lzp_) h.off = take(c, h.zi.size() * sizeof(ZItem));
     o_zc2_ = take(c, zc2_.size() * sizeof(double));
+    o_zth_ = take(c, zth_.size() * sizeof(ZThomasItem));
+    o_zth_tab_ = take(c, zth_tab_.size() * sizeof(double));
     o_smt_ = take(c, smt_.size() * sizeof(SmtItem));
     if (precision == 1) {
         o_sums_ = take(c, (size_t)nsums_ * sizeof(double));
@@ -431,6 +452,8 @@ int BatchCore::bind(char *base, TableCache *tc)
             h.d_items_alt = h.li_alt.empty() ? nullptr : base + h.off_alt;
         }
     d_zc2_ = (double *)(base + o_zc2_);
+    d_zth_ = (ZThomasItem *)(base + o_zth_);
+    d_zth_tab_ = (double *)(base + o_zth_tab_);
     d_smt_ = (SmtItem *)(base + o_smt_);
     int rc;
     for (auto *v : {&lx_, &ly_})
@@ -536,7 +559,9 @@ int BatchCore::upload(hipStream_t s)
     for (auto *v : {&lz_, &lzp_})
         for (HostClass &h : *v)
             if ((rc = put((ZItem *)h.d_items, h.zi, s))) return rc;
-    if ((rc = put(d_zc2_, zc2_, s)) || (rc = put(d_smt_, smt_, s))) return rc;
+    if ((rc = put(d_zc2_, zc2_, s)) || (rc = put(d_smt_, smt_, s)) || (rc = put(d_zth_, zth_, s)) ||
+        (rc = put(d_zth_tab_, zth_tab_, s)))
+        return rc;
     if (precision == 1) {
         // members without a pinned column add zeros in the inverse-x store
         PPB_CHECK(hipMemsetAsync(d_plane_, 0, (size_t)nplane_ * sizeof(double), s));
@@ -669,8 +694,18 @@ int BatchCore::run_z_stage(hipStream_t s, bool f32)
             z.pinned_zero = 1;
             PPB_CHECK(launch_zsolve_pcr_f32(16 * c.key, z, s));
         } else {
+            z.tau = zthomas_tau;  // the nearly singular columns are left to the Thomas sweeps
             PPB_CHECK(launch_zsolve_pcr_f64(16 * c.key, z, s));
         }
+    }
+    if (!f32 && !zth_.empty()) {
+        ZArgs z{};
+        z.data = d_tiles;
+        z.tiled = 1;
+        z.tile_w = zsolve_cpb<double>();
+        z.tile_rows = 0;  // ceil16(nz) per member
+        z.solves = d_zs_;
+        PPB_CHECK(launch_zthomas_f64(z, d_zth_, d_zth_tab_, (int)zth_.size(), s));
     }
     // FP64 columns of 513..1024 rows: the long-column kernel on the member's z tiles
     for (int m : zlong_) {
@@ -1093,9 +1128,10 @@ int pp_batch_set_option(pp_batch b, int key, long long value)
 int pp_batch_set_option_double(pp_batch b, int key, double value)
 {
     if (!b) return PP_INVALID_HANDLE;
-    if (key != PP_OPT_TAU || !(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
+    if ((key != PP_OPT_TAU && key != PP_OPT_ZTHOMAS_TAU) || !(value >= 0.0 && value < 4.0)) return PP_INPUT_ARGS;
     if (b->finalized) return PP_CALL_ORDER;
-    b->core.tau = value;
+    if (key == PP_OPT_TAU) b->core.tau = value;
+    else b->core.zthomas_tau = value;
     return PP_SUCCESS;
 }
 

@@ -56,6 +56,7 @@ class BatchCore {
     int refine = 2;
     int zero_mode = 1;
     double tau = 0.05;
+    double zthomas_tau = 1e-3;  // FP64: columns below it by the reference's Thomas sweeps (0: off)
     int force_var = -1;  // FP64 line classes: -1 tuned / default shapes, 0 default, 1 alternative
 
     std::vector<BatchMember> mem;
@@ -131,6 +132,10 @@ class BatchCore {
     std::map<int, long> zc2_off_;
     // FP64 members with 513..1024-row columns: the long-column kernel, one launch each
     std::vector<int> zlong_;
+    // FP64: nearly singular columns of the PCR classes' members, solved by the reference's
+    // Thomas sweeps after the z classes, and their factor tables
+    std::vector<ZThomasItem> zth_;
+    std::vector<double> zth_tab_;
     std::vector<int4> fix_;
     std::vector<HostClass> lfix_;  // 16-column tiles holding near-singular columns, per J
     std::vector<Mode00Item> m00_;
@@ -143,7 +148,7 @@ class BatchCore {
     int nz_max_ = 0;
     long nsums_ = 0, nplane_ = 0, nupart_ = 0, ntiles_ = 0;
     // region layout (byte offsets)
-    size_t o_u0_ = 0, o_cacc_ = 0, o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
+    size_t o_zth_ = 0, o_zth_tab_ = 0, o_u0_ = 0, o_cacc_ = 0, o_zc2_ = 0, o_smt_ = 0, o_rhs_ = 0, o_sol_ = 0, o_tiles_ = 0, o_w_ = 0, o_lam_ = 0, o_dtab_ = 0, o_ls_ = 0, o_zs_ = 0, o_sums_ = 0, o_plane_ = 0,
            o_upart_ = 0, o_fix_ = 0, o_m00_ = 0, o_res_ = 0, o_ritems_ = 0, o_rpitems_ = 0;
     std::map<int, AxisDev> tabs_;
     // device pointers of the tables
@@ -152,6 +157,8 @@ class BatchCore {
     ZSolve *d_zs_ = nullptr;
     int4 *d_fix_ = nullptr, *d_ritems_ = nullptr, *d_rpitems_ = nullptr;
     double *d_zc2_ = nullptr;
+    ZThomasItem *d_zth_ = nullptr;
+    double *d_zth_tab_ = nullptr;
     SmtItem *d_smt_ = nullptr;
     int smt_nz_max_ = 0;
 

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "pp_launch.h"
 #include "pp_setup.h"
 #include "pp_zsolve.h"
 
@@ -47,9 +48,9 @@ struct pp_solver {
     int mg_cycles = 200;                        // multigrid z solver: V-cycle limit
     double mg_tol = 1e-16;                      //   and relative max-norm residual tolerance
     // FP64 form: columns with |eig+2| below zthomas_tau (not pinned) are solved by the
-    // reference's serial Thomas after the PCR kernel (0: off); their list, built with the tables
-    double zthomas_tau = 0.0;
-    int4 *d_zthomas = nullptr;
+    // reference's Thomas sweeps after the PCR kernel (0: off); their list, built with the tables
+    double zthomas_tau = 1e-3;
+    pp::ZThomasItem *d_zthomas = nullptr;
     double *d_zthomas_tab = nullptr;  // their Thomas factors (zthomas_factors), 2 nz per column
     int nzthomas = 0;
     bool zthomas_ready = false;

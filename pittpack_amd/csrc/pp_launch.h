@@ -33,10 +33,15 @@ bool zsolve_supported_f32(int nz);
 bool zfix_tiles_supported(int nz);
 hipError_t launch_zfix_tiles_f32(int nz_max, const ZArgs &a, hipStream_t s);
 
-// the reference's serial Thomas (thomasLowMem) on the listed columns cols[i] = {sx, sy, 0, 0}
-// of FP64 data in place or on z tiles (a's layout), one block per column; nz <= 1024;
-// tab + 2 nz i = {1 / bet_j, tmp_j} of column i (zthomas_factors)
-hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, const double *tab, int ncols, hipStream_t s);
+// the reference's Thomas sweeps (thomasLowMem) on listed FP64 columns in place or on z tiles
+// (a's layout), one wave per column, nz <= 1024.  solve >= 0: batched, the member's field and
+// sizes from a.solves; toff: offset of the column's factors {1 / bet_j, tmp_j} in tab
+// (zthomas_factors, 2 nz values)
+struct ZThomasItem {
+    int sx, sy, solve, pad;
+    long toff;
+};
+hipError_t launch_zthomas_f64(const ZArgs &a, const ZThomasItem *items, const double *tab, int nitems, hipStream_t s);
 
 // thomasLowMem's factors of one column (src/triDiag.cpp:1693-1822): out[j] = 1 / bet_j,
 // out[n + j] = tmp_j (tmp_0 unused), with b0 / bn the end rows' diagonals

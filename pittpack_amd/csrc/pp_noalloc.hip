@@ -196,6 +196,7 @@ extern "C" int pp_solve_device_oop_noalloc(int nx, int ny, int nz, const char *b
     h.zero_mode = zero_mode;
     h.zsolver = 0;
     h.ztiles = 0;
+    h.zthomas_tau = 0.0;  // no Thomas columns: their list and factors would need allocations
     h.stream = s;
     for (int a = 0; a < 2; a++) {
         void **t[5] = {&h.tab64[a].tw, &h.tab64[a].A, &h.tab64[a].B, &h.tab64[a].C, &h.tab64[a].D};

@@ -423,6 +423,8 @@ bool zthomas_on(const pp_solver *h)
 int ensure_zthomas(pp_solver *h)
 {
     if (h->zthomas_ready) return PP_SUCCESS;
+    // the host copies of the eigenvalue factors come with the tables
+    if ((int)h->lamx_h.size() != h->g.nx || (int)h->lamy_h.size() != h->g.ny) return PP_CALL_ORDER;
     if (h->d_zthomas || h->d_zthomas_tab) {
         PP_HIP_CHECK(hipStreamSynchronize(h->stream));
         if (h->d_zthomas) PP_HIP_CHECK(hipFree(h->d_zthomas));
@@ -434,24 +436,23 @@ int ensure_zthomas(pp_solver *h)
     int rc;
     const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
     const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
-    std::vector<int4> cols;
+    std::vector<ZThomasItem> cols;
+    std::vector<double> tab;
+    const int n = h->g.nz;
     for (int sy = 0; sy < h->g.ny; sy++)
         for (int sx = 0; sx < h->g.nx; sx++) {
             const double eig = column_eig(h->lamx_h, h->lamy_h, sx, sy, Cx, Cy);
             bool pin = std::fabs(eig + 2.) < 1.e-10;
             if (pin && h->zero_mode == 1 && (h->g.zlo_d || h->g.zhi_d)) pin = false;
-            if (!pin && std::fabs(eig + 2.) < h->zthomas_tau) cols.push_back(make_int4(sx, sy, 0, 0));
+            if (pin || !(std::fabs(eig + 2.) < h->zthomas_tau)) continue;
+            cols.push_back({sx, sy, -1, 0, (long)tab.size()});
+            tab.resize(tab.size() + 2 * (size_t)n);
+            zthomas_factors(eig, h->g.zlo_d ? eig - 1. : eig + 1., h->g.zhi_d ? eig - 1. : eig + 1., n,
+                            tab.data() + tab.size() - 2 * (size_t)n);
         }
     if (!cols.empty()) {
-        const int n = h->g.nz;
-        std::vector<double> tab(cols.size() * 2 * (size_t)n);
-        for (size_t i = 0; i < cols.size(); i++) {
-            const double eig = column_eig(h->lamx_h, h->lamy_h, cols[i].x, cols[i].y, Cx, Cy);
-            zthomas_factors(eig, h->g.zlo_d ? eig - 1. : eig + 1., h->g.zhi_d ? eig - 1. : eig + 1., n,
-                            tab.data() + i * 2 * n);
-        }
-        if (hipMalloc((void **)&h->d_zthomas, cols.size() * sizeof(int4)) != hipSuccess) return PP_ALLOCATION_FAIL;
-        PP_HIP_CHECK(hipMemcpy(h->d_zthomas, cols.data(), cols.size() * sizeof(int4), hipMemcpyHostToDevice));
+        if (hipMalloc((void **)&h->d_zthomas, cols.size() * sizeof(ZThomasItem)) != hipSuccess) return PP_ALLOCATION_FAIL;
+        PP_HIP_CHECK(hipMemcpy(h->d_zthomas, cols.data(), cols.size() * sizeof(ZThomasItem), hipMemcpyHostToDevice));
         if ((rc = upload((void **)&h->d_zthomas_tab, tab))) return rc;
     }
     h->nzthomas = (int)cols.size();

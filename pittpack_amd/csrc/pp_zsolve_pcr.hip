@@ -135,56 +135,117 @@ namespace pp {
 // branch) on one listed FP64 column per block.  Its factors depend only on the column's
 // eigenvalue and end rows, so the host computes them in the reference's order (bet_0 = b0,
 // tmp_j = 1 / bet_{j-1}, bet_j = eig - tmp_j, the last row bn - tmp) and hands over
-// tab = {1 / bet_j, tmp_j} per column; the block stages the column and its factors in LDS and
-// one thread runs the two sweeps, each row one subtraction and one multiplication on the
-// serial chain.  The PCR kernel left these columns untouched (|eig + 2| below its tau).
-__global__ void __launch_bounds__(256) k_zthomas64(ZArgs a, const int4 *cols, const double *tab)
+// tab = {1 / bet_j, tmp_j} per column.  The two sweeps are then linear recurrences in the
+// column's values -- forward x_j = (x_j - x_{j-1}) / bet_j, backward x_j -= tmp_{j+1} x_{j+1}
+// -- which one wave runs as a scan: each lane composes the affine maps of its contiguous chunk
+// of rows (C = ceil(nz / 64) <= 16), a Kogge-Stone scan across the 64 lanes gives each lane the
+// value entering its chunk, and the lane sweeps its rows (about 40 dependent steps per sweep
+// instead of nz).  Against the serial sweep this reassociation moves a column by a few 1e-14
+// of its largest value; the PCR steps it replaces lose ~3e-12 on such columns.  The PCR
+// kernel left these columns untouched (|eig + 2| below its tau).
+constexpr int ZTH_CMAX = 16;  // rows per lane: nz <= 64 * 16
+
+__global__ void __launch_bounds__(64) k_zthomas64(ZArgs a0, const ZThomasItem *items, const double *tab)
 {
-    extern __shared__ double zth[];
+    constexpr int W = 64;
+    const ZThomasItem it = items[blockIdx.x];
+    ZArgs a = a0;
+    if (it.solve >= 0) {
+        // batched: the member's field (its z tiles), sizes
+        const ZSolve sv = a0.solves[it.solve];
+        a.data = reinterpret_cast<double *>(a0.data) + sv.offset;
+        a.SX = sv.SX;
+        a.SY = sv.SY;
+        a.nz = sv.nz;
+    }
     const int n = a.nz;
-    double *rh = zth, *ib = zth + n, *tp = zth + 2 * n;
-    const int4 sc = cols[blockIdx.x];
-    const double *t = tab + (long)blockIdx.x * 2 * n;
-    // element i of the column: in place (stride SX*SY) or in its z tile (stride W)
+    // element j of the column: in place (stride SX*SY) or in its z tile (stride W)
     long zs = (long)a.SX * a.SY;
-    long off = (long)sc.y * a.SX + sc.x;
+    long off = (long)it.sy * a.SX + it.sx;
     if (a.tiled) {
-        const int W = a.tile_w;
-        const int rows = a.tile_rows ? a.tile_rows : (a.nz + 15) / 16 * 16;
-        off = (((long)(sc.x / W) * a.SY + sc.y) * rows) * W + sc.x % W;
-        zs = W;
+        const int Wt = a.tile_w;
+        const int rows = a.tile_rows ? a.tile_rows : (n + 15) / 16 * 16;
+        off = (((long)(it.sx / Wt) * a.SY + it.sy) * rows) * Wt + it.sx % Wt;
+        zs = Wt;
     }
     double *data = reinterpret_cast<double *>(a.data);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        rh[i] = data[off + (long)i * zs];
-        ib[i] = t[i];
-        tp[i] = t[n + i];
+    const double *ib = tab + it.toff, *tp = ib + n;
+    const int L = threadIdx.x, C = (n + W - 1) / W, j0 = L * C;
+    double x[ZTH_CMAX], g[ZTH_CMAX], q[ZTH_CMAX];
+#pragma unroll
+    for (int k = 0; k < ZTH_CMAX; k++) {
+        const int j = j0 + k;
+        const bool in = k < C && j < n;
+        x[k] = in ? data[off + (long)j * zs] : 0.0;
+        g[k] = in ? ib[j] : 0.0;
+        q[k] = (in && j + 1 < n) ? tp[j + 1] : 0.0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = rh[0] * ib[0];
-        rh[0] = x;
-#pragma unroll 8
-        for (int j = 1; j < n; j++) {
-            x = (rh[j] - x) * ib[j];
-            rh[j] = x;
-        }
-#pragma unroll 8
-        for (int j = n - 2; j >= 0; j--) {
-            x = rh[j] - tp[j + 1] * x;
-            rh[j] = x;
+    // forward: x_j = -g_j x_{j-1} + x_j g_j, composed over the chunk, scanned in lane order
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int k = 0; k < ZTH_CMAX; k++) {
+        if (k < C && j0 + k < n) {
+            const double m = -g[k];
+            B = m * B + x[k] * g[k];
+            A = m * A;
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) data[off + (long)i * zs] = rh[i];
+    for (int s = 1; s < W; s <<= 1) {
+        const double Au = __shfl_up(A, s, W), Bu = __shfl_up(B, s, W);
+        if (L >= s) {
+            B = A * Bu + B;
+            A = A * Au;
+        }
+    }
+    double v = __shfl_up(B, 1, W);
+    if (L == 0) v = 0.0;
+#pragma unroll
+    for (int k = 0; k < ZTH_CMAX; k++) {
+        if (k < C && j0 + k < n) {
+            v = (x[k] - v) * g[k];
+            x[k] = v;
+        }
+    }
+    // backward: x_j = -tmp_{j+1} x_{j+1} + x_j, composed over the chunk from its last row,
+    // scanned in reverse lane order
+    A = 1.0;
+    B = 0.0;
+#pragma unroll
+    for (int k = ZTH_CMAX - 1; k >= 0; k--) {
+        if (k < C && j0 + k < n) {
+            const double m = -q[k];
+            B = m * B + x[k];
+            A = m * A;
+        }
+    }
+    for (int s = 1; s < W; s <<= 1) {
+        const double Ad = __shfl_down(A, s, W), Bd = __shfl_down(B, s, W);
+        if (L + s < W) {
+            B = A * Bd + B;
+            A = A * Ad;
+        }
+    }
+    v = __shfl_down(B, 1, W);
+    if (L == W - 1) v = 0.0;
+#pragma unroll
+    for (int k = ZTH_CMAX - 1; k >= 0; k--) {
+        if (k < C && j0 + k < n) {
+            v = x[k] - q[k] * v;
+            x[k] = v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < ZTH_CMAX; k++) {
+        const int j = j0 + k;
+        if (k < C && j < n) data[off + (long)j * zs] = x[k];
+    }
 }
 
-hipError_t launch_zthomas_f64(const ZArgs &a, const int4 *cols, const double *tab, int ncols, hipStream_t s)
+hipError_t launch_zthomas_f64(const ZArgs &a, const ZThomasItem *items, const double *tab, int nitems, hipStream_t s)
 {
-    if (ncols == 0) return hipSuccess;
-    if (a.nz < 3 || a.nz > 1024 || a.items) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zthomas64, dim3((unsigned)ncols), dim3(256), 3 * (size_t)a.nz * sizeof(double), s, a, cols,
-                       tab);
+    if (nitems == 0) return hipSuccess;
+    if ((!a.solves && (a.nz < 3 || a.nz > 64 * ZTH_CMAX)) || a.items) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zthomas64, dim3((unsigned)nitems), dim3(64), 0, s, a, items, tab);
     return hipGetLastError();
 }
 

@@ -158,6 +158,36 @@ def test_batch_periodic_z_and_long_columns(precision):
             assert err <= TOL, f"member {i} {bc} {(nx, ny, nz)}: {err:.3e}"
 
 
+@pytest.mark.parametrize("zthomas", [0.0, 0.5])
+def test_batch_zthomas(zthomas):
+    """the FP64 batch's nearly singular columns by the reference's Thomas sweeps (threshold 0.5:
+    many columns per member) or none by them (0): each member bitwise the single handle with
+    the same threshold, and within the FP64 bar of the oracle"""
+    print(f"batch zthomas={zthomas}", flush=True)
+    with pa.Batch(zthomas=zthomas) as B:
+        ids = [B.add(nx, ny, nz, bc, box) for nx, ny, nz, bc, box in MEMBERS]
+        B.finalize()
+        rhs = []
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            f = prng.uniform_pm1((nz, ny, nx), seed=700 + i)
+            rhs.append(f)
+            B.set_rhs(i, f)
+        B.solve()
+        for i, (nx, ny, nz, bc, box) in zip(ids, MEMBERS):
+            got = B.get_solution(i)
+            with pa.PoissonHIP(nx, ny, nz) as M:
+                M.set_option_double(pa.OPT_ZTHOMAS_TAU, zthomas)
+                M.assignBoundary(bc)
+                M.setBox(box)
+                M.setCoords(2)
+                M.assignRhs(rhs[i])
+                M.pittPack()
+                single = M.getSolution()
+            assert np.array_equal(got, single), f"member {i} {bc}: batched != single ({rel_maxdiff(got, single):.2e})"
+            ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+            assert rel_maxdiff(got, ref) <= TOL, f"member {i} {bc}"
+
+
 def test_batch_repeated_and_device_fill():
     """device generator == Python generator; the right-hand sides are kept, so a second solve
     reproduces the first"""

@@ -230,7 +230,7 @@ def test_call_order_and_errors():
         assert e.value.code == pa.UNSUPPORTED_SIZE
 
 
-def _solve_opts(f, bc, box, zthomas=0.0, zsolver=None, zlayout=None):
+def _solve_opts(f, bc, box, zthomas=None, zsolver=None, zlayout=None):
     nz, ny, nx = f.shape
     with pa.PoissonHIP(nx, ny, nz) as M:
         M.assignBoundary(bc)
@@ -240,7 +240,7 @@ def _solve_opts(f, bc, box, zthomas=0.0, zsolver=None, zlayout=None):
             M.set_option(pa.OPT_ZSOLVER, zsolver)
         if zlayout is not None:
             M.set_option(pa.OPT_ZLAYOUT, zlayout)
-        if zthomas:
+        if zthomas is not None:
             M.set_option_double(pa.OPT_ZTHOMAS_TAU, zthomas)
         M.assignRhs(f)
         M.pittPack()
@@ -250,14 +250,14 @@ def _solve_opts(f, bc, box, zthomas=0.0, zsolver=None, zlayout=None):
 def test_zthomas_thin_cells():
     """cells six times thinner in z than in x and y with Neumann z faces: the low (kx, ky)
     columns have |eig + 2| ~ 1e-5, where the PCR steps lose about ten times more digits than
-    the reference's Thomas (2e-12 against the oracle); PP_OPT_ZTHOMAS_TAU hands those columns to
-    the reference's serial Thomas and restores the FP64 bar"""
+    the reference's Thomas (2e-12 against the oracle with PP_OPT_ZTHOMAS_TAU = 0); the default
+    threshold hands those columns to the reference's Thomas sweeps and restores the FP64 bar"""
     print("zthomas thin cells NNNNNN (96, 64, 128)", flush=True)
     f = prng.uniform_pm1((96, 64, 128), seed=41)
     box = [0.0, 2.0, 0.0, 1.0, 0.0, 0.25]
     ref = oracle.solve(f, "NNNNNN", box, zero_mode=1)
-    e_pcr = rel_maxdiff(_solve_opts(f, "NNNNNN", box), ref)
-    e_thomas = rel_maxdiff(_solve_opts(f, "NNNNNN", box, zthomas=1e-3), ref)
+    e_pcr = rel_maxdiff(_solve_opts(f, "NNNNNN", box, zthomas=0.0), ref)
+    e_thomas = rel_maxdiff(_solve_opts(f, "NNNNNN", box), ref)
     print(f"  PCR {e_pcr:.2e}  with Thomas columns {e_thomas:.2e}", flush=True)
     assert e_thomas <= TOL and e_thomas <= e_pcr, (e_pcr, e_thomas)
 

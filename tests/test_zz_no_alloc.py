@@ -3,8 +3,9 @@ allocate device memory) against pp_solve_device_oop, the entry point it is a var
 torch device arrays through the C ABI.  Two calls in a row, the second on a smaller grid (the
 tables in the shared scratch array are rebuilt), then the first shape again, periodic z,
 arbitrary-length lines, a repeated call that reuses the resident tables, and the size and
-argument errors.  Bar: bitwise equal to the handle on the same (in-place) z layout, and within
-1e-12 of the handle's default configuration."""
+argument errors.  Bar: bitwise equal to the handle on the same (in-place) z layout with the
+PCR solve on every column (PP_OPT_ZTHOMAS_TAU = 0: this entry point does not take the Thomas
+columns), and within 1e-12 of the handle's default configuration."""
 import ctypes
 
 import numpy as np
@@ -37,7 +38,7 @@ def noalloc(shape, bc, box, f, zero_mode=1):
     return d_u.cpu().numpy()
 
 
-def handle(shape, bc, box, f, zlayout, zero_mode=1):
+def handle(shape, bc, box, f, zlayout, zero_mode=1, zthomas=None):
     import torch
 
     nz, ny, nx = shape
@@ -46,6 +47,8 @@ def handle(shape, bc, box, f, zlayout, zero_mode=1):
     with pa.PoissonHIP(nx, ny, nz) as M:
         M.set_option(pa.OPT_ZLAYOUT, zlayout)
         M.set_option(pa.OPT_ZERO_MODE, zero_mode)
+        if zthomas is not None:
+            M.set_option_double(pa.OPT_ZTHOMAS_TAU, zthomas)
         M.assignBoundary(bc)
         M.setBox(box)
         M.setCoords(2)
@@ -73,7 +76,7 @@ def test_noalloc_matches_handle_in_sequence():
         f = prng.uniform_pm1(shape, seed=40 + i)
         u = noalloc(shape, bc, box, f, zm)
         assert np.isfinite(u).all()
-        ref_inplace = handle(shape, bc, box, f, 0, zm)
+        ref_inplace = handle(shape, bc, box, f, 0, zm, zthomas=0.0)
         assert np.array_equal(u, ref_inplace), f"call {i}: differs from the in-place handle"
         ref = handle(shape, bc, box, f, 1, zm)
         assert np.abs(u - ref).max() <= 1e-12 * np.abs(ref).max(), f"call {i}"

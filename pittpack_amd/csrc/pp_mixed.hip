@@ -672,7 +672,14 @@ hipError_t launch_mode00_batch(const Mode00Item *items, int nitems, int nz_max, 
     return hipGetLastError();
 }
 
-bool residual_pairs(int nx) { return nx % 2 == 0 && nx >= 2; }
+// The pair kernel's 128-column tiles leave lanes idle on narrow grids (nx = 32: a quarter of
+// them busy); it is taken where it keeps at least as many lanes busy as the 64-column kernel
+bool residual_pairs(int nx)
+{
+    if (nx % 2 != 0 || nx < 2) return false;
+    const long pair_cols = (long)(nx + RP_CX - 1) / RP_CX * RP_CX, cols = (long)(nx + RES_TX - 1) / RES_TX * RES_TX;
+    return pair_cols <= cols;
+}
 
 int residual_tile_x_of(int nx) { return residual_pairs(nx) ? RP_CX : RES_TX; }
 

@@ -94,9 +94,11 @@ def test_mixed_periodic_z_thin_cells(bc):
     assert rel_maxdiff(got, ref) <= 1e-11
 
 
-# residual tiles whose first pair of columns is the grid's last (nx = 2 mod 128) and whose last
-# row is a tile's first (ny = 1 mod 4): both neighbours of one direction come from memory
-@pytest.mark.parametrize("shape", [(12, 9, 130), (7, 5, 258), (20, 13, 66)], ids=lambda s: "x".join(map(str, s)))
+# residual tiles at the grid's edge: the 64-column kernel where its last column or row is a
+# tile's first (nx = 1 mod 64 is odd; ny = 1 mod 4) and the two-cells-per-lane kernel (taken
+# where its 128-column tiles keep as many lanes busy: nx = 66, 128, 256) with ny = 1 mod 4
+@pytest.mark.parametrize("shape", [(12, 9, 130), (7, 5, 258), (20, 13, 66), (6, 13, 128), (8, 12, 256), (9, 17, 65)],
+                         ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("bc", ["PPPPNN", "DDDDDD", "NNNNDD", "DDNDDD", "NNPPPP"])
 def test_mixed_residual_tile_edges(bc, shape):
     print(f"mixed residual tile edges {bc} {shape}", flush=True)

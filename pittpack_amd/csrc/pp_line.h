@@ -324,6 +324,53 @@ constexpr int line_min_waves()
     return (M / T == 8 && M >= 256 && !(AX == 0 && !FWD && !std::is_same<TIn, TOut>::value && M >= 512)) ? 4 : 1;
 }
 
+// x lines whose T threads cover less than a 128-byte line per access (T * sizeof(R) < 128:
+// short lines) are staged: the block's lines are one contiguous run of memory, so the block
+// stores (loads) it with whole-wave contiguous accesses out of (into) the lines' LDS slices,
+// instead of T-element fragments per line and instruction.  Only where the stored precision
+// is the LDS precision (the slices hold values of type R); loads of inverse transforms only
+// (measured: the staged load slows the forward one, whose fragments feed the fused scaling
+// and line sums directly)
+template <typename TIn, typename R, int T, int AX, int FWD>
+constexpr bool line_stage_in()
+{
+    return AX == 0 && !FWD && T * (int)sizeof(R) < 128 && sizeof(TIn) == sizeof(R);
+}
+template <typename TOut, typename R, int T, int AX>
+constexpr bool line_stage_out()
+{
+    return AX == 0 && T * (int)sizeof(R) < 128 && sizeof(TOut) == sizeof(R);
+}
+
+// staged load of a block's x lines (line0 .. line0 + LINES - 1, those below nlines): element e
+// of the run goes to position e % N of line e / N's slice; then each thread reads its inputs
+// n = t + T c from its line's slice (the layout line_load gives)
+template <typename TIn, typename R, int M, int T, int LINES>
+__device__ __forceinline__ void stage_lines_in(const LineArgs &a, long line0, cpx<R> *lds, int t, int l,
+                                               line_xt<TIn> (&xin)[2 * (M / T)])
+{
+    constexpr int N = 2 * M, LS = line_slots<M>(), NT = T * LINES, C = 2 * (M / T);
+    const TIn *src = reinterpret_cast<const TIn *>(a.in) + line0 * N;
+    const long rem = a.nlines - line0;
+    const int total = (int)(rem < LINES ? rem : LINES) * N;
+    // a full block moves N / T = C elements per thread: every load is issued before the
+    // first LDS write
+    const int tid = (int)threadIdx.x;
+    R v[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) v[c] = (tid + NT * c < total) ? (R)src[tid + NT * c] : (R)0;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int e = tid + NT * c, l2 = e / N;
+        if (e < total) lr(lds + l2 * LS, e - l2 * N) = v[c];
+    }
+    __syncthreads();
+    cpx<R> *L = lds + l * LS;
+#pragma unroll
+    for (int c = 0; c < 2 * (M / T); c++) xin[c] = (line_xt<TIn>)lr(L, t + T * c);
+    __syncthreads();  // the slices are rewritten by the pre-processing next
+}
+
 // Issue the coalesced global loads of one line (column) of a block
 template <typename TIn, typename R, int M, int T, int AX, int FWD, int KIND>
 __device__ __forceinline__ void line_load(const LineArgs &a, const LineAddr &ad, int t, line_xt<TIn> (&xin)[2 * (M / T)])
@@ -377,7 +424,7 @@ __device__ __forceinline__ void line_load(const LineArgs &a, const LineAddr &ad,
 // slice
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad, int t, cpx<R> *L,
-                                          line_xt<TIn> (&xin)[2 * (M / T)])
+                                          line_xt<TIn> (&xin)[2 * (M / T)], cpx<R> *lds, long line0)
 {
     constexpr int N = 2 * M;
     constexpr int E = M / T;
@@ -399,6 +446,47 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
     };
     constexpr bool DIRECT_IN = line_direct_in<AX, FWD, KIND>();
     using XT = line_xt<TIn>;
+    // staged store (short x lines, line_stage_out): after a barrier, the block writes its lines'
+    // outputs -- real-view position store_pos(n) of each line's slice -- as one contiguous run
+    constexpr bool SOUT = line_stage_out<TOut, R, T, AX>();
+    // x inverse, staged: each line's plane term in a table after the slices (written here,
+    // read after the transform's barriers)
+    double *planes = reinterpret_cast<double *>(lds + LINES * line_slots<M>());
+    if constexpr (SOUT && !FWD) {
+        if (a.plane_add && t == 0) planes[threadIdx.x / T] = a.plane_add[lz];
+    }
+    auto store_staged = [&]() {
+        constexpr int LS = line_slots<M>(), NT = T * LINES, C = 2 * E;
+        const long rem = a.nlines - line0;
+        const int total = (int)(rem < LINES ? rem : LINES) * N;
+        const int tid = (int)threadIdx.x;
+        TOut *dst = out + line0 * N;
+        double val[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int e = tid + NT * c, l2 = e / N, n = e - l2 * N;
+            bool neg;
+            const int pos = store_pos<N, KIND, FWD>(n, neg);
+            val[c] = (e < total) ? (double)lr(lds + l2 * LS, pos) : 0.0;
+            if (neg) val[c] = -val[c];
+            val[c] = a.out_scale * val[c];
+        }
+        if constexpr (!FWD) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int e = tid + NT * c;
+                if (e < total) {
+                    if (a.plane_add) val[c] += planes[e / N];
+                    if (a.accumulate) val[c] += (double)dst[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int e = tid + NT * c;
+            if (e < total) dst[e] = (TOut)val[c];
+        }
+    };
 
     // x inverse: the solution being accumulated into and the plane term are read here, with
     // the input, so their latency overlaps the transform instead of sitting before each store
@@ -627,6 +715,18 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
                 }
             }
         }
+        if constexpr (SOUT) {
+            // short lines: the outputs go to their positions in the slice, then out as one run
+            __syncthreads();  // every thread's reads of the spectrum are done
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                lr(L, p1[m]) = o1[m];
+                lr(L, p2[m]) = o2[m];
+            }
+            __syncthreads();
+            store_staged();
+            return;
+        }
         // outputs k and N - k (N - 1 - k and k - 1 for DD) are contiguous across lanes:
         // stored from registers, no LDS round trip
         if (valid) {
@@ -687,6 +787,10 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
     }
     __syncthreads();
 
+    if constexpr (SOUT) {
+        store_staged();
+        return;
+    }
     // ---- coalesced store with fused output scaling ----
     if (valid) {
 #pragma unroll
@@ -732,8 +836,10 @@ __global__ void __launch_bounds__(T *LINES, (line_min_waves<TIn, TOut, R, M, T, 
     cpx<R> *L = lds + l * LS;
     const LineAddr ad = line_addr<LINES, AX>(a, N, bx, by, l);
     line_xt<TIn> xin[2 * E];
-    line_load<TIn, R, M, T, AX, FWD, KIND>(a, ad, t, xin);
-    line_body<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, ad, t, L, xin);
+    const long line0 = (long)bx * LINES;  // x lines: the block's first line
+    if constexpr (line_stage_in<TIn, R, T, AX, FWD>()) stage_lines_in<TIn, R, M, T, LINES>(a, line0, lds, t, l, xin);
+    else line_load<TIn, R, M, T, AX, FWD, KIND>(a, ad, t, xin);
+    line_body<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>(a, ad, t, L, xin, lds, line0);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -829,7 +935,8 @@ constexpr bool tiled32_half()
 template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
 hipError_t launch_line_blocks(const LineArgs &a, hipStream_t s)
 {
-    const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>);
+    const size_t lds = (size_t)LINES * line_slots<M>() * sizeof(cpx<R>) +
+                       ((line_stage_out<TOut, R, T, AX>() && !FWD) ? LINES * sizeof(double) : 0);
     auto kern = k_line<TIn, TOut, R, M, T, LINES, AX, FWD, KIND>;
     if (lds > 64 * 1024) {
         static bool attr_set = false;

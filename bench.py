@@ -44,18 +44,25 @@ MIXED_INIT = [("x_forward_f64_to_f32", 12), ("y_forward_f32", 8), ("z_solve_f32"
 MIXED_STEP = [("x_forward_f32", 8), ("y_forward_f32", 8), ("z_solve_f32", 8), ("z_fix_f64", 0), ("y_inverse_f32", 8)]
 
 
-def kernel_table(precision: int, refine: int):
+def kernel_table(precision: int, refine: int, fused: bool = True):
     """launch order of one solve with the algorithmic bytes per grid point of each kernel.  The
     mixed form holds the solution as FP32 parts while it is refined (u0, then the sum of the
-    corrections before the last); only the last store writes u in FP64."""
+    corrections before the last); only the last store writes u in FP64.  fused: each
+    correction's residual is computed inside its x-forward transform (one kernel reading f, u0
+    (+ the correction sum) and writing the transformed lines), else the residual kernel writes
+    r and the transform reads it."""
     if precision == 0:
         return FP64_KERNELS
     out = list(MIXED_INIT)
     out.append(("x_inverse_f32", 8) if refine > 0 else ("x_inverse_final_f64", 12))  # c -> u0 (or u)
     for k in range(1, refine + 1):
-        # f, u0 (+ the correction sum), r
-        out.append((f"residual_u0.{k}", 16) if k == 1 else (f"residual_u0_csum.{k}", 20))
-        out += [(f"{name}.{k}", b) for name, b in MIXED_STEP]
+        if fused:
+            out.append((f"residual_x_forward_u0.{k}", 16) if k == 1 else (f"residual_x_forward_u0_csum.{k}", 20))
+            out += [(f"{name}.{k}", b) for name, b in MIXED_STEP[1:]]
+        else:
+            # f, u0 (+ the correction sum), r
+            out.append((f"residual_u0.{k}", 16) if k == 1 else (f"residual_u0_csum.{k}", 20))
+            out += [(f"{name}.{k}", b) for name, b in MIXED_STEP]
         if k < refine:
             out.append((f"x_inverse_f32.{k}", 8 if k == 1 else 12))  # c (+ sum) -> sum
         else:
@@ -441,7 +448,11 @@ def main():
         return (t1 - t0) * 1e3 / args.steps, M.kernel_times(), M
 
     def roofline(precision: int, kt):
-        table = kernel_table(precision, args.refine)
+        # the mixed form fuses each correction's residual into its x forward where the grid
+        # allows (one launch mark fewer per step)
+        table = kernel_table(precision, args.refine, fused=True)
+        if len(kt) != len(table):
+            table = kernel_table(precision, args.refine, fused=False)
         names = [k for k, _ in table]
         kms = dict(zip(names, kt))
         bytes_pp = dict(table)

@@ -113,7 +113,12 @@ enum {
      * steps, which lose up to ten times more digits on nearly singular columns (long Neumann or
      * periodic axes, cells thinner in z).  Default 1e-3 (the columns where the two differ beyond
      * 1e-13); 0 = off. */
-    PP_OPT_ZTHOMAS_TAU = 15
+    PP_OPT_ZTHOMAS_TAU = 15,
+    /* mixed form, single handle: each refinement step computes the FP64 residual inside its
+     * FP32 x-forward transform (one kernel; the residual is never stored), where the x lines
+     * have a compiled plan and ny is a multiple of the kernel's lines per block (8 for 512-point
+     * lines); 0 = the residual kernel, then the transform.  Default 1. */
+    PP_OPT_FUSE_RESIDUAL = 16
 };
 
 /* PencilDcmp(argc, argv, nx, ny, nz) / make_Poisson (src/pencilDcmp.cpp:232-452,6432-6440).

@@ -48,6 +48,7 @@ OPT_LINE_VARIANT = 12
 OPT_MAX_OVERTAKE = 13
 OPT_IO_FORMAT = 14
 OPT_ZTHOMAS_TAU = 15
+OPT_FUSE_RESIDUAL = 16
 
 # every symbol include/pittpack_hip.h declares
 EXPORTED = (
@@ -551,9 +552,12 @@ class Server:
 
 def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | None = None,
           precision: int = 0, refine: int | None = None, tau: float | None = None,
-          line_variant: int | None = None, autotune: int | None = None, zlayout: int | None = None) -> np.ndarray:
+          line_variant: int | None = None, autotune: int | None = None, zlayout: int | None = None,
+          fuse: int | None = None) -> np.ndarray:
     """One-shot solve of an (nz, ny, nx) FP64 RHS (precision 0: FP64; 1: mixed + refinement).
-    zlayout: 1 (default) hands the z solve the y transform's z tiles where it can, 0 in place."""
+    zlayout: 1 (default) hands the z solve the y transform's z tiles where it can, 0 in place.
+    fuse: mixed form, 1 (default) computes each correction's residual inside its x forward, 0
+    runs the residual kernel and the transform apart."""
     f = np.ascontiguousarray(rhs, dtype=np.float64)
     nz, ny, nx = f.shape
     with PoissonHIP(nx, ny, nz) as M:
@@ -567,6 +571,8 @@ def solve(rhs, bc: str, box=None, zero_mode: int | None = None, zsolver: int | N
             M.set_option(OPT_ZSOLVER, zsolver)
         if zlayout is not None:
             M.set_option(OPT_ZLAYOUT, zlayout)
+        if fuse is not None:
+            M.set_option(OPT_FUSE_RESIDUAL, fuse)
         if precision:
             M.set_option(OPT_PRECISION, precision)
         if refine is not None:

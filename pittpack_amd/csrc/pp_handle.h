@@ -77,6 +77,7 @@ struct pp_solver {
     float *d_wt = nullptr;         // FP32 z tiles (32 columns x 16 J rows) between the y transforms
     int wtiles = 1;                // mixed form: hand the z solve FP32 z tiles where the sizes allow
     int wvar = 0;                  // mixed form, FP32 tiled y stages: 1 = half the threads per column
+    int fuse_res = 1;              // PP_OPT_FUSE_RESIDUAL: correction's residual inside its x forward
     bool mtuned = false;           // mixed form tuned (PP_OPT_AUTOTUNE)
     double *d_aux = nullptr;       // FP64 scratch field (RHS copy or solution) for host-facing calls
     double *d_line_sums = nullptr; // ny*nz

@@ -1,0 +1,320 @@
+// pp_resx.h -- the mixed form's correction step in one kernel: the FP64 residual of the
+// dz^2-scaled 7-point operator, r = (rhs*dz^2 - Dirichlet term) - L u, computed inside the FP32
+// x-forward transform of the lines it feeds.  r never goes through memory: against the
+// residual kernel followed by the transform (pp_mixed.hip k_residual_pair, then k_line) this
+// moves f, u and the transformed lines only, 8 bytes per grid point fewer.
+//
+// Replaces, for the refinement's corrections, the reference's right-hand-side assembly ahead
+// of the x transform (assignRhs's dz^2 scaling, src/pencilDcmp.cpp:5591-5647, and
+// modifyRhsDirichlet, 4623-4835) with the residual of the same system.
+//
+// A block owns LINES consecutive x lines = rows y0 .. y0+LINES-1 of one plane z (the host
+// launches it only when ny is a multiple of LINES).  It stages u(z) of those rows and of the
+// two rows around them (ghost rows at the y faces) in LDS in FP64, reads u(z-1), u(z+1) and f
+// for its own cells, computes r two cells per lane with the residual kernel's arithmetic,
+// rounds it to FP32 into the lines' slices and hands them to the transform (line_body).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pp_line.h"
+#include "pp_mixed.h"
+
+namespace pp {
+
+typedef double rx_d2 __attribute__((ext_vector_type(2)));
+typedef float rx_f2 __attribute__((ext_vector_type(2)));
+
+// bytes of LDS of the fused kernel: the FP64 row tile, or the lines' slices (they alias)
+template <int M, int LINES>
+constexpr size_t resx_lds()
+{
+    // the slices (overwriting the tile once it is read) never reach past the tile; the
+    // Dirichlet profiles ex (N) and ey (LINES rows) follow it
+    const size_t tile = (size_t)(LINES + 2) * 2 * M * sizeof(double);
+    const size_t slices = (size_t)LINES * line_slots<M>() * sizeof(cpx<float>);
+    return (tile > slices ? tile : slices) + (size_t)(2 * M + LINES) * sizeof(double);
+}
+
+// Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4,
+// at most 4): 3 for 512-point lines (44 KB per 256-thread block), 2 for 1024-point ones; at
+// most 2 where a thread holds other than 8 elements (their transforms need more registers)
+template <int M, int T, int LINES>
+constexpr int resx_waves()
+{
+    const int blocks = (int)((160 * 1024) / resx_lds<M, LINES>());
+    int w = blocks * (T * LINES / 64) / 4;
+    if ((M / T) != 8 && w > 2) w = 2;
+    return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+
+template <int M, int T, int LINES, int KIND, int SRC>
+__global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(LineArgs a, ResArgs r)
+{
+    using R = float;
+    constexpr int N = 2 * M, E = M / T, LS = line_slots<M>(), NT = T * LINES;
+    constexpr int HP = N / 2;            // cell pairs per row
+    constexpr int PP = LINES * HP / NT;  // own pairs per thread
+    static_assert(PP * NT == LINES * HP && PP == E, "a block's rows are E pairs per thread");
+    static_assert(NT % 64 == 0, "whole waves");
+    constexpr int HC = (N + NT - 1) / NT;  // pairs of the two outer rows per thread
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
+    double *ut = reinterpret_cast<double *>(smem_raw);  // (LINES + 2) rows of N: rows y0-1 .. y0+LINES
+    constexpr size_t TILE_D = (resx_lds<M, LINES>() / sizeof(double)) - (N + LINES);
+    double *exs = ut + TILE_D, *eys = exs + N;          // Dirichlet profiles ex[0..N), ey[y0..y0+LINES)
+    __shared__ double red[NT / 64];
+    const int tid = (int)threadIdx.x, t = tid % T, l = tid / T;
+    const int ny = r.ny, nz = r.nz, ng = ny / LINES;
+    // Block numbering: with ng a multiple of 8, XCD k (blocks b = k mod 8) owns row groups
+    // k*ng/8 .. (k+1)*ng/8 - 1 of every plane and takes them plane by plane, so the z- and
+    // y-neighbour rows a block reads were read by blocks that ran on the same XCD just before
+    // (its own L2); otherwise groups in plane order
+    int g, z;
+    if (ng % 8 == 0) {
+        const int gr = ng / 8, k = (int)blockIdx.x % 8, s = (int)blockIdx.x / 8;
+        z = s / gr;
+        g = k * gr + s % gr;
+    } else {
+        z = (int)blockIdx.x / ng;
+        g = (int)blockIdx.x % ng;
+    }
+    const int y0 = g * LINES, bx = z * ng + g;
+    const long sy = N, sz = (long)N * ny;
+    const long p0 = (long)z * sz + (long)y0 * sy;  // first cell of the block's rows
+    using gd2 = const __attribute__((address_space(1))) rx_d2;
+    using gf2 = const __attribute__((address_space(1))) rx_f2;
+    // u at an even cell offset and the next cell: FP32 parts summed in FP64 (as the residual
+    // kernel: (double)u0 + (double)u1)
+    auto ldu2 = [&](long o) -> rx_d2 {
+        const rx_f2 p = *(gf2 *)(r.u0 + o);
+        rx_d2 v = {(double)p.x, (double)p.y};
+        if constexpr (SRC == RES_U0_U1) {
+            const rx_f2 q = *(gf2 *)(r.u1 + o);
+            v.x += (double)q.x;
+            v.y += (double)q.y;
+        }
+        return v;
+    };
+
+    // ---- loads: own rows of u(z), the outer rows, u(z -/+ 1) and f of the own cells ----
+    // outer rows: inside the grid the neighbouring rows; at a y face the ghost row (P: the
+    // wrapped row, N: the face row itself, D: its negative)
+    int rowm = y0 - 1, rowp = y0 + LINES;
+    double sgm = 1.0, sgp = 1.0;
+    if (y0 == 0) {
+        rowm = r.bc[2] == 'P' ? ny - 1 : 0;
+        sgm = r.bc[2] == 'D' ? -1.0 : 1.0;
+    }
+    if (y0 + LINES == ny) {
+        rowp = r.bc[3] == 'P' ? 0 : ny - 1;
+        sgp = r.bc[3] == 'D' ? -1.0 : 1.0;
+    }
+    // z neighbours: inside the grid the next planes; at a z face P wraps, N and D read this
+    // plane's cell (signed): every pair loads unconditionally, so no branch holds a wait
+    const bool zlo = z == 0, zhi = z == nz - 1;
+    const long om = zlo ? (r.bc[4] == 'P' ? (long)(nz - 1) * sz : 0) : -sz;
+    const long op = zhi ? (r.bc[5] == 'P' ? -(long)(nz - 1) * sz : 0) : sz;
+    const double sgzm = (zlo && r.bc[4] == 'D') ? -1.0 : 1.0, sgzp = (zhi && r.bc[5] == 'D') ? -1.0 : 1.0;
+    {
+        // own rows (kept in the tile only) and outer rows: loaded, then written to the tile
+        rx_d2 uc[PP], hv[HC];
+#pragma unroll
+        for (int c = 0; c < PP; c++) uc[c] = ldu2(p0 + 2 * (tid + NT * c));
+#pragma unroll
+        for (int i = 0; i < HC; i++) {
+            const int h = min(tid + NT * i, N - 1), j = h < HP ? 0 : 1;
+            hv[i] = ldu2((long)z * sz + (long)(j ? rowp : rowm) * sy + 2 * (h - j * HP));
+        }
+#pragma unroll
+        for (int c = 0; c < PP; c++) *(rx_d2 *)(ut + N + 2 * (tid + NT * c)) = uc[c];
+#pragma unroll
+        for (int i = 0; i < HC; i++) {
+            const int h = tid + NT * i, j = h < HP ? 0 : 1;
+            if (h < N) *(rx_d2 *)(ut + (j ? (LINES + 1) * N : 0) + 2 * (h - j * HP)) = hv[i] * (j ? sgp : sgm);
+        }
+        for (int i = tid; i < N + LINES; i += NT) exs[i] = i < N ? r.dtab[i] : r.dtab[N + y0 + (i - N)];
+    }
+    // z neighbours and the right-hand side of the own pairs, as raw loads issued PD pairs ahead
+    // of their use: each pair's arithmetic sits between scheduling barriers, so only the pairs
+    // in flight hold registers (all of them at once would need over 250)
+    constexpr int PD = 2;
+    struct Raw {
+        rx_f2 m0, m1, p0, p1;
+        rx_d2 f;
+    };
+    Raw raw[PP];
+    auto issue = [&](int c) {
+        const long o = p0 + 2 * (tid + NT * c);
+        raw[c].m0 = *(gf2 *)(r.u0 + o + om);
+        raw[c].p0 = *(gf2 *)(r.u0 + o + op);
+        if constexpr (SRC == RES_U0_U1) {
+            raw[c].m1 = *(gf2 *)(r.u1 + o + om);
+            raw[c].p1 = *(gf2 *)(r.u1 + o + op);
+        }
+        raw[c].f = *(gd2 *)(r.f + o);
+    };
+    // (double)u0 + (double)u1, as ldu2
+    auto sum2 = [](rx_f2 a0, rx_f2 a1) -> rx_d2 {
+        rx_d2 v = {(double)a0.x, (double)a0.y};
+        if constexpr (SRC == RES_U0_U1) {
+            v.x += (double)a1.x;
+            v.y += (double)a1.y;
+        }
+        return v;
+    };
+#pragma unroll
+    for (int c = 0; c < PD && c < PP; c++) issue(c);
+    __syncthreads();
+
+    // ---- residual per cell pair (the residual kernel's arithmetic) ----
+    const double *ef = r.dtab + N + ny + nz;
+    const int dm = r.dmask;
+    const double ezz = r.dtab[N + ny + z];
+    const double ef0 = ef[0], ef1 = ef[1], ef2 = ef[2], ef3 = ef[3], ef4 = ef[4], ef5 = ef[5];
+    const double defect = r.pin_defect ? r.pin_defect[0] : 0.0;
+    const char bx0 = r.bc[0], bx1 = r.bc[1];
+    const double gxm = bx0 == 'D' ? -1.0 : 1.0, gxp = bx1 == 'D' ? -1.0 : 1.0;
+    // pinned (0,0) column: the block's partial sum of u over plane 0 (periodic z: planes 0 and
+    // nz - 1, the latter held as u(z-1)); ng entries, one per row group
+    const bool sums = r.upart && zlo, zwrap = r.bc[4] == 'P';
+    double sp = 0.0;
+    rx_f2 rv[PP];
+#pragma unroll
+    for (int c = 0; c < PP; c++) {
+        if (c + PD < PP) issue(c + PD);
+        __builtin_amdgcn_sched_barrier(0);
+        const int q = 2 * (tid + NT * c), l2 = q / N, x = q - l2 * N, y = y0 + l2;
+        const double *row = ut + (l2 + 1) * N;
+        const rx_d2 u = *(const rx_d2 *)(row + x);
+        const rx_d2 umc = sum2(raw[c].m0, raw[c].m1) * sgzm, upc = sum2(raw[c].p0, raw[c].p1) * sgzp;
+        const rx_d2 fvc = raw[c].f;
+        if (sums) sp += (u.x + u.y) + (zwrap ? umc.x + umc.y : 0.0);
+        const rx_d2 uym = *(const rx_d2 *)(row - N + x), uyp = *(const rx_d2 *)(row + N + x);
+        // x neighbours: inside the row from the tile; at x = 0 and x = N - 1 the ghost closure
+        // (P wraps within the row, N mirrors: the cell itself, D: its negative)
+        const int il = x > 0 ? x - 1 : (bx0 == 'P' ? N - 1 : 0);
+        const int ir = x + 2 < N ? x + 2 : (bx1 == 'P' ? 0 : N - 1);
+        const double uxl = (x > 0 ? 1.0 : gxm) * row[il];
+        const double uxr = (x + 2 < N ? 1.0 : gxp) * row[ir];
+        const double Lu0 = ((u.y - u.x) - (u.x - uxl)) * r.Cx + ((uyp.x - u.x) - (u.x - uym.x)) * r.Cy +
+                           ((upc.x - u.x) - (u.x - umc.x));
+        const double Lu1 = ((uxr - u.y) - (u.y - u.x)) * r.Cx + ((uyp.y - u.y) - (u.y - uym.y)) * r.Cy +
+                           ((upc.y - u.y) - (u.y - umc.y));
+        // Dirichlet terms, as the residual kernel forms them, with selects (no branch per pair):
+        // x and y faces a per-cell constant times ez[z], z faces a constant on the end planes
+        const double ex0 = exs[x], ex1 = exs[x + 1], eyc = eys[l2];
+        const bool fx0 = (dm & 1) && x == 0, fx1 = (dm & 2) && x + 1 == N - 1;
+        const bool fy0 = (dm & 4) && y == 0, fy1 = (dm & 8) && y == ny - 1;
+        rx_d2 cxy = {0.0, 0.0};
+        cxy.x = fx0 ? cxy.x + 2. * ef0 * eyc : cxy.x;
+        cxy.y = fx1 ? cxy.y + 2. * ef1 * eyc : cxy.y;
+        cxy.x = fy0 ? cxy.x + 2. * ex0 * ef2 : cxy.x;
+        cxy.y = fy0 ? cxy.y + 2. * ex1 * ef2 : cxy.y;
+        cxy.x = fy1 ? cxy.x + 2. * ex0 * ef3 : cxy.x;
+        cxy.y = fy1 ? cxy.y + 2. * ex1 * ef3 : cxy.y;
+        const rx_d2 czlo = (dm & 16) ? rx_d2{2. * ex0 * eyc * ef4, 2. * ex1 * eyc * ef4} : rx_d2{0.0, 0.0};
+        const rx_d2 czhi = (dm & 32) ? rx_d2{2 * ex0 * eyc * ef5, 2 * ex1 * eyc * ef5} : rx_d2{0.0, 0.0};
+        rx_d2 v = fvc;
+        v = v * r.c3;
+        v = v * r.c3;
+        v = v - cxy * ezz;
+        if (zlo) v = v - czlo;
+        if (zhi) v = v - czhi;
+        double res0 = v.x - Lu0, res1 = v.y - Lu1;
+        if (zlo) {
+            res0 += defect;
+            res1 += defect;
+        }
+        // the result is pinned here (an empty asm that takes it in registers): otherwise the
+        // arithmetic sinks to the slice writes after the barrier and every pair's loads stay live
+        float r0 = (float)res0, r1 = (float)res1;
+        asm volatile("" : "+v"(r0), "+v"(r1));
+        rv[c] = rx_f2{r0, r1};
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (sums) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off);
+        if ((tid & 63) == 0) red[tid >> 6] = sp;
+    }
+    __syncthreads();  // the tile is read (the slices overwrite it), the wave sums are in red
+    if (sums && tid == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; i++) s += red[i];
+        r.upart[g] = s;
+    }
+
+    // ---- r into the lines' slices, then each thread's inputs n = t + T c of its line ----
+#pragma unroll
+    for (int c = 0; c < PP; c++) {
+        const int q = 2 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
+        lr(lds + l2 * LS, x) = rv[c].x;
+        lr(lds + l2 * LS, x + 1) = rv[c].y;
+    }
+    __syncthreads();
+    cpx<R> *L = lds + l * LS;
+    line_xt<float> xin[2 * E];
+#pragma unroll
+    for (int c = 0; c < 2 * E; c++) xin[c] = lr(L, t + T * c);
+    __syncthreads();  // the slices are rewritten by the pre-processing next
+    const LineAddr ad = line_addr<LINES, 0>(a, N, bx, 0, l);
+    line_body<float, float, float, M, T, LINES, 0, 1, KIND>(a, ad, t, L, xin, lds, (long)bx * LINES);
+}
+
+template <int M, int KIND, int SRC>
+hipError_t launch_resx_cfg(const LineArgs &a, const ResArgs &r, hipStream_t s)
+{
+    constexpr int T = line_threads(M, 0);
+    if constexpr (T == 0) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr int LINES = line_lines<float, M, 0>();
+        if (r.nx != 2 * M || r.ny % LINES != 0 || a.items || a.nlines != (long)r.ny * r.nz) return hipErrorInvalidValue;
+        constexpr size_t lds = resx_lds<M, LINES>();
+        auto kern = k_resx<M, T, LINES, KIND, SRC>;
+        if constexpr (lds > 64 * 1024) {
+            static bool attr_set = false;
+            if (!attr_set) {
+                hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                if (e != hipSuccess) return e;
+                attr_set = true;
+            }
+        }
+        const unsigned blocks = (unsigned)((long)(r.ny / LINES) * r.nz);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * LINES), lds, s, a, r);
+        return hipGetLastError();
+    }
+}
+
+// the fused correction step of one x transform kind (instantiated in pp_line_<kind>.hip):
+// a is the FP32 x-forward's arguments (out: the work field, line sums), r the residual's
+// (src: RES_U0 or RES_U0_U1)
+template <int KIND>
+hipError_t launch_resx_kind(int N, int src, const LineArgs &a, const ResArgs &r, hipStream_t s)
+{
+    if (src != RES_U0 && src != RES_U0_U1) return hipErrorInvalidValue;
+    switch (N / 2) {
+#define PPM(m) \
+    case m: return src == RES_U0 ? launch_resx_cfg<m, KIND, RES_U0>(a, r, s) : launch_resx_cfg<m, KIND, RES_U0_U1>(a, r, s);
+        PP_FFT_LENGTHS(PPM)
+#undef PPM
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// x lines per block of the fused kernel for line length N (0: no compiled plan); the fused
+// step applies where ny is a multiple of it, and writes ny / resx_lines(N) partial sums
+inline int resx_lines(int N)
+{
+    if (N % 2) return 0;
+    switch (N / 2) {
+#define PPM(m) \
+    case m: return line_threads(m, 0) ? line_lines<float, m, 0>() : 0;
+        PP_FFT_LENGTHS(PPM)
+#undef PPM
+    default: return 0;
+    }
+}
+
+}  // namespace pp

@@ -18,6 +18,7 @@
 #include "pp_line_bs.h"
 #include "pp_launch.h"
 #include "pp_mixed.h"
+#include "pp_resx.h"
 #include "pp_setup.h"
 #include "pp_zsolve.h"
 #include "pp_handle.h"
@@ -46,6 +47,12 @@ extern template hipError_t launch_line_kind<KIND_NN>(int, int, int, int, const L
 extern template hipError_t launch_line_kind<KIND_PP>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_ND>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
+
+extern template hipError_t launch_resx_kind<KIND_DD>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_NN>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_PP>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_ND>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_DN>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
 
 extern template hipError_t launch_line_bs_kind<KIND_DD>(int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_bs_kind<KIND_NN>(int, int, int, const LineArgs &, hipStream_t);
@@ -76,6 +83,20 @@ hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArg
     case KIND_PP: return launch_line_kind<KIND_PP>(form, N, AX, FWD, a, s);
     case KIND_ND: return launch_line_kind<KIND_ND>(form, N, AX, FWD, a, s);
     case KIND_DN: return launch_line_kind<KIND_DN>(form, N, AX, FWD, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// the mixed form's fused correction step (pp_resx.h): residual of the current solution (src:
+// RES_U0 or RES_U0_U1) into the FP32 x-forward transform of kind `kind`
+hipError_t launch_resx(int kind, int N, int src, const LineArgs &a, const ResArgs &r, hipStream_t s)
+{
+    switch (kind) {
+    case KIND_DD: return launch_resx_kind<KIND_DD>(N, src, a, r, s);
+    case KIND_NN: return launch_resx_kind<KIND_NN>(N, src, a, r, s);
+    case KIND_PP: return launch_resx_kind<KIND_PP>(N, src, a, r, s);
+    case KIND_ND: return launch_resx_kind<KIND_ND>(N, src, a, r, s);
+    case KIND_DN: return launch_resx_kind<KIND_DN>(N, src, a, r, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -892,6 +913,14 @@ int mixed_inner(pp_solver *h)
 // in FP64 -- and only the last store writes u in FP64 (u0 + csum + c_R); csum's FP32 rounding
 // (eps32 |c| ~ 1e-14 |u| after the first correction) is what the next step corrects.  f and u
 // must not alias.
+// the correction step runs fused (pp_resx.h) where the x lines have a compiled plan and ny
+// is a multiple of the fused kernel's lines per block (PP_OPT_FUSE_RESIDUAL, default on)
+static bool mixed_fused(const pp_solver *h)
+{
+    const int lines = resx_lines(h->g.nx);
+    return h->fuse_res && line_compiled(h->g.nx) && lines > 0 && h->g.ny % lines == 0;
+}
+
 int run_mixed(pp_solver *h, const double *f, double *u)
 {
     hipStream_t s = h->stream;
@@ -930,6 +959,7 @@ int run_mixed(pp_solver *h, const double *f, double *u)
     if ((rc = mark(h))) return rc;
     const double Cx = (h->g.dxyz[2] / h->g.dxyz[0]) * (h->g.dxyz[2] / h->g.dxyz[0]);
     const double Cy = (h->g.dxyz[2] / h->g.dxyz[1]) * (h->g.dxyz[2] / h->g.dxyz[1]);
+    const bool fused = mixed_fused(h);
     for (int it = 0; it < R; it++) {
         const bool last = it == R - 1;
         ResArgs r{};
@@ -949,13 +979,17 @@ int run_mixed(pp_solver *h, const double *f, double *u)
         r.dmask = h->g.dmask;
         r.pin_defect = h->pinned00 ? h->d_plane_add + h->g.nz : nullptr;
         r.upart = h->pinned00 ? h->d_upart : nullptr;
-        PP_HIP_CHECK(launch_residual_f32(r, it == 0 ? RES_U0 : RES_U0_U1, s));
-        if ((rc = mark(h))) return rc;
-        {
-            LineArgs a = line_args(h, 0, nullptr);
-            a.in = h->d_w;
-            a.out = h->d_w;
-            a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
+        const int src = it == 0 ? RES_U0 : RES_U0_U1;
+        LineArgs a = line_args(h, 0, nullptr);
+        a.in = h->d_w;
+        a.out = h->d_w;
+        a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
+        if (fused) {
+            // residual computed inside the x forward: one kernel, r never stored
+            PP_HIP_CHECK(launch_resx(h->g.kind[0], h->g.nx, src, a, r, s));
+        } else {
+            PP_HIP_CHECK(launch_residual_f32(r, src, s));
+            if ((rc = mark(h))) return rc;
             PP_HIP_CHECK(launch_line(FORM_F32, h->g.kind[0], h->g.nx, 0, 1, a, s));
         }
         if ((rc = mark(h))) return rc;
@@ -966,7 +1000,8 @@ int run_mixed(pp_solver *h, const double *f, double *u)
             const double b0 = h->g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
             const double bn = h->g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
             PP_HIP_CHECK(launch_mode00(h->d_line_sums, h->g.ny, h->g.nz, b0, bn, 1. / h->g.scale, h->d_plane_add, h->d_upart,
-                                       residual_parts(h->g.nx, h->g.ny), 1, h->g.zper, s));
+                                       fused ? h->g.ny / resx_lines(h->g.nx) : residual_parts(h->g.nx, h->g.ny), 1,
+                                       h->g.zper, s));
         }
         {
             LineArgs a = line_args(h, 0, nullptr);
@@ -1362,6 +1397,10 @@ int pp_set_option(pp_handle h, int key, long long value)
     case PP_OPT_IO_FORMAT:
         if (value != PP_IO_HDF5 && value != PP_IO_RAW) return PP_INPUT_ARGS;
         h->io_format = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_FUSE_RESIDUAL:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        h->fuse_res = (int)value;
         return PP_SUCCESS;
     case PP_OPT_REFINE:
         if (value < 0 || value > 16) return PP_INPUT_ARGS;

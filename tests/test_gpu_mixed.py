@@ -37,6 +37,93 @@ def test_mixed_parity(bc, shape):
         assert err <= TOL, f"{bc} {shape} zero_mode={zero_mode}: rel max diff {err:.3e}"
 
 
+# Short x lines (T lanes per line covering less than a 128-byte line) store through their LDS
+# slices as one contiguous run per block, and inverse transforms load that way: lengths of
+# every thread count that takes that path (FP64 up to T = 8 threads per line, FP32 up to 16) in
+# every x kind, both precisions; 35 lines per grid, so each length's last block is ragged.
+# Cells are cubes (the refinement's 7-point solution is then the spectral one to the bar).
+STAGED_NX = [8, 12, 20, 28, 32, 48, 64, 80, 112, 128, 160, 224, 256, 320, 384, 448, 640]
+X_KINDS = ["PPPPNN", "NNNNDD", "DDDDDD", "NDDNDN", "DNNNDD"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nx", STAGED_NX)
+@pytest.mark.parametrize("bc", X_KINDS)
+def test_short_x_lines_staged(bc, nx, precision):
+    shape = (7, 5, nx)
+    print(f"short x lines {bc} {shape} precision={precision}", flush=True)
+    f = prng.uniform_pm1(shape, seed=nx * 7 + precision + sum(map(ord, bc)))
+    box = [0.0, 1.0, 0.0, 5.0 / nx, 0.0, 7.0 / nx]
+    ref = oracle.solve(f, bc, box, zero_mode=1)  # the library's default (0,0)-mode rule
+    got = pa.solve(f, bc, box, precision=precision)
+    err = rel_maxdiff(got, ref)
+    # refined mixed solves converge to the 7-point solution, ~1e-12 from the spectral one along
+    # long periodic x axes (test_mixed_converges_to_stencil_solution)
+    bar = 3e-12 if precision and bc[0] == "P" and nx >= 256 else TOL
+    assert err <= bar, f"{bc} {shape} precision={precision}: rel max diff {err:.3e}"
+    if precision:
+        # the FP32 transforms alone (no refinement): a misplaced element would be O(1)
+        got0 = pa.solve(f, bc, box, precision=1, refine=0)
+        err0 = rel_maxdiff(got0, ref)
+        assert err0 <= 1e-5, f"{bc} {shape} FP32 solve: rel max diff {err0:.3e}"
+
+
+# The correction step fused into one kernel (the residual computed inside the FP32 x forward,
+# PP_OPT_FUSE_RESIDUAL, default on where ny is a multiple of the kernel's lines per block):
+# every fusing line length (32 .. 1024 points, lines per block 64 .. 4), ny = 8 x lines per block
+# (the XCD-grouped block numbering) and fewer, every x kind, Dirichlet faces, the pinned
+# column (its plane-0 sums come from the fused blocks) and periodic z, one to three steps (the
+# first reads u0, later ones u0 + the correction sum).  Bar: the unfused path within 1e-13, and
+# the oracle at 1e-12 after two or more steps, 1e-11 after one (or the unfused path's own error
+# where that is larger).
+FUSED_SHAPES = [(6, 64, 32), (5, 32, 128), (4, 128, 256), (4, 16, 256), (4, 64, 512), (5, 8, 512), (4, 4, 1024),
+                (5, 32, 192), (4, 16, 640), (6, 16, 384), (4, 64, 96)]
+FUSED_BCS = ["DDNDDD", "PPPPNN", "NNNNNN", "DDDDDD", "NDDNDN", "DNNNDD", "PPPPPP", "NNNNPP", "DDPPDD"]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("bc", FUSED_BCS)
+def test_fused_residual_step(bc, shape):
+    nz, ny, nx = shape
+    print(f"fused residual step {bc} {shape}", flush=True)
+    f = prng.uniform_pm1(shape, seed=nx + 3 * ny + 7 * nz + sum(map(ord, bc)))
+    box = [0.0, 1.0, 0.0, ny / nx, 0.0, nz / nx]
+    ref = oracle.solve(f, bc, box, zero_mode=1)  # the library's default (0,0)-mode rule
+    for refine in (1, 2, 3):
+        fused = pa.solve(f, bc, box, precision=1, refine=refine)
+        plain = pa.solve(f, bc, box, precision=1, refine=refine, fuse=0)
+        d = rel_maxdiff(fused, plain)
+        assert d <= 1e-13, f"{bc} {shape} refine={refine}: fused vs unfused {d:.3e}"
+        err = rel_maxdiff(fused, ref)
+        # the FP64 bar after two or more steps, 1e-11 after one -- or, where the unfused step
+        # itself is further from the oracle, no worse than it: one FP32 correction on 512-wide
+        # pinned grids (~2e-11), and after two the 7-point solution on a 1024-long periodic axis
+        # (~1.7e-12 from the spectral one, as test_mixed_converges_to_stencil_solution explains)
+        bar = max(TOL if refine >= 2 else 1e-11, 1.01 * rel_maxdiff(plain, ref))
+        assert err <= bar, f"{bc} {shape} refine={refine}: fused vs oracle {err:.3e}"
+
+
+def test_fused_residual_option():
+    """PP_OPT_FUSE_RESIDUAL takes 0 or 1 only; the fused step's launch count (one kernel per
+    correction instead of two) shows in the handle's launch marks"""
+    f = prng.uniform_pm1((4, 16, 256), seed=5)
+    with pa.PoissonHIP(256, 16, 4) as M:
+        with pytest.raises(pa.PittPackError):
+            M.set_option(pa.OPT_FUSE_RESIDUAL, 2)
+        M.assignBoundary("DDNDDD")
+        M.setBox([0, 1, 0, 1, 0, 1])
+        M.setCoords(2)
+        M.set_option(pa.OPT_PRECISION, 1)
+        M.set_option(pa.OPT_TIMING, 1)
+        counts = []
+        for fuse in (1, 0):
+            M.set_option(pa.OPT_FUSE_RESIDUAL, fuse)
+            M.assignRhs(f)  # the solve consumes its right-hand side
+            M.pittPack()
+            counts.append(len(M.kernel_times()))
+        assert counts[1] - counts[0] == 2, counts  # two refinement steps, one kernel fewer each
+
+
 # periodic z: the z stage computes in FP64 on the FP32 work field (one-pass circulant solve for
 # lengths with a compiled complex plan, real z-line transforms around a division otherwise);
 # the singular column (x and y in {P, N}) follows the reference's averaged row 0

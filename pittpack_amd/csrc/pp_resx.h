@@ -11,7 +11,7 @@
 // A block owns LINES consecutive x lines = rows y0 .. y0+LINES-1 of one plane z (the host
 // launches it only when ny is a multiple of LINES).  It stages u(z) of those rows and of the
 // two rows around them (ghost rows at the y faces) in LDS in FP64, reads u(z-1), u(z+1) and f
-// for its own cells, computes r two cells per lane with the residual kernel's arithmetic,
+// for its own cells, computes r four cells per lane with the residual kernel's arithmetic,
 // rounds it to FP32 into the lines' slices and hands them to the transform (line_body).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -22,7 +22,8 @@
 namespace pp {
 
 typedef double rx_d2 __attribute__((ext_vector_type(2)));
-typedef float rx_f2 __attribute__((ext_vector_type(2)));
+typedef double rx_d4 __attribute__((ext_vector_type(4)));
+typedef float rx_f4 __attribute__((ext_vector_type(4)));
 
 // bytes of LDS of the fused kernel: the FP64 row tile, or the lines' slices (they alias)
 template <int M, int LINES>
@@ -35,16 +36,17 @@ constexpr size_t resx_lds()
     return (tile > slices ? tile : slices) + (size_t)(2 * M + LINES) * sizeof(double);
 }
 
-// Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4,
-// at most 4): 3 for 512-point lines (44 KB per 256-thread block), 2 for 1024-point ones; at
-// most 2 where a thread holds other than 8 elements (their transforms need more registers)
+// Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4),
+// at most 3 (<= 168 VGPRs; the quarter-wave kinds spill at 128): 3 for 512-point lines (44
+// KB per 256-thread block), 2 for 1024-point ones; at most 2 where a thread holds other than 8
+// elements (their transforms need more registers)
 template <int M, int T, int LINES>
 constexpr int resx_waves()
 {
     const int blocks = (int)((160 * 1024) / resx_lds<M, LINES>());
     int w = blocks * (T * LINES / 64) / 4;
     if ((M / T) != 8 && w > 2) w = 2;
-    return w < 1 ? 1 : (w > 4 ? 4 : w);
+    return w < 1 ? 1 : (w > 3 ? 3 : w);
 }
 
 template <int M, int T, int LINES, int KIND, int SRC>
@@ -52,11 +54,12 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
 {
     using R = float;
     constexpr int N = 2 * M, E = M / T, LS = line_slots<M>(), NT = T * LINES;
-    constexpr int HP = N / 2;            // cell pairs per row
-    constexpr int PP = LINES * HP / NT;  // own pairs per thread
-    static_assert(PP * NT == LINES * HP && PP == E, "a block's rows are E pairs per thread");
+    // four adjacent x cells per lane and access: u parts as one 16-byte load each, f as two
+    constexpr int QR = N / 4;            // cell quads per row
+    constexpr int QQ = LINES * QR / NT;  // own quads per thread
+    static_assert(N % 4 == 0 && QQ * NT == LINES * QR && 2 * QQ == E, "a block's rows are E / 2 quads per thread");
     static_assert(NT % 64 == 0, "whole waves");
-    constexpr int HC = (N + NT - 1) / NT;  // pairs of the two outer rows per thread
+    constexpr int HC = (2 * QR + NT - 1) / NT;  // quads of the two outer rows per thread
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
     double *ut = reinterpret_cast<double *>(smem_raw);  // (LINES + 2) rows of N: rows y0-1 .. y0+LINES
@@ -82,18 +85,18 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
     const long sy = N, sz = (long)N * ny;
     const long p0 = (long)z * sz + (long)y0 * sy;  // first cell of the block's rows
     using gd2 = const __attribute__((address_space(1))) rx_d2;
-    using gf2 = const __attribute__((address_space(1))) rx_f2;
-    // u at an even cell offset and the next cell: FP32 parts summed in FP64 (as the residual
-    // kernel: (double)u0 + (double)u1)
-    auto ldu2 = [&](long o) -> rx_d2 {
-        const rx_f2 p = *(gf2 *)(r.u0 + o);
-        rx_d2 v = {(double)p.x, (double)p.y};
-        if constexpr (SRC == RES_U0_U1) {
-            const rx_f2 q = *(gf2 *)(r.u1 + o);
-            v.x += (double)q.x;
-            v.y += (double)q.y;
-        }
+    using gf4 = const __attribute__((address_space(1))) rx_f4;
+    // (double)u0 + (double)u1 per cell, as the residual kernel sums the parts
+    auto sum4 = [](rx_f4 a0, rx_f4 a1) -> rx_d4 {
+        rx_d4 v = __builtin_convertvector(a0, rx_d4);
+        if constexpr (SRC == RES_U0_U1) v += __builtin_convertvector(a1, rx_d4);
         return v;
+    };
+    auto ldu4 = [&](long o) -> rx_d4 {
+        const rx_f4 p = *(gf4 *)(r.u0 + o);
+        rx_f4 q = p;
+        if constexpr (SRC == RES_U0_U1) q = *(gf4 *)(r.u1 + o);
+        return sum4(p, q);
     };
 
     // ---- loads: own rows of u(z), the outer rows, u(z -/+ 1) and f of the own cells ----
@@ -110,63 +113,67 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
         sgp = r.bc[3] == 'D' ? -1.0 : 1.0;
     }
     // z neighbours: inside the grid the next planes; at a z face P wraps, N and D read this
-    // plane's cell (signed): every pair loads unconditionally, so no branch holds a wait
+    // plane's cell (signed): every quad loads unconditionally, so no branch holds a wait
     const bool zlo = z == 0, zhi = z == nz - 1;
     const long om = zlo ? (r.bc[4] == 'P' ? (long)(nz - 1) * sz : 0) : -sz;
     const long op = zhi ? (r.bc[5] == 'P' ? -(long)(nz - 1) * sz : 0) : sz;
     const double sgzm = (zlo && r.bc[4] == 'D') ? -1.0 : 1.0, sgzp = (zhi && r.bc[5] == 'D') ? -1.0 : 1.0;
     {
         // own rows (kept in the tile only) and outer rows: loaded, then written to the tile
-        rx_d2 uc[PP], hv[HC];
+        rx_d4 uc[QQ], hv[HC];
 #pragma unroll
-        for (int c = 0; c < PP; c++) uc[c] = ldu2(p0 + 2 * (tid + NT * c));
+        for (int c = 0; c < QQ; c++) uc[c] = ldu4(p0 + 4 * (tid + NT * c));
 #pragma unroll
         for (int i = 0; i < HC; i++) {
-            const int h = min(tid + NT * i, N - 1), j = h < HP ? 0 : 1;
-            hv[i] = ldu2((long)z * sz + (long)(j ? rowp : rowm) * sy + 2 * (h - j * HP));
+            const int h = min(tid + NT * i, 2 * QR - 1), j = h < QR ? 0 : 1;
+            hv[i] = ldu4((long)z * sz + (long)(j ? rowp : rowm) * sy + 4 * (h - j * QR));
         }
 #pragma unroll
-        for (int c = 0; c < PP; c++) *(rx_d2 *)(ut + N + 2 * (tid + NT * c)) = uc[c];
+        for (int c = 0; c < QQ; c++) {
+            double *d = ut + N + 4 * (tid + NT * c);
+            *(rx_d2 *)d = uc[c].lo;
+            *(rx_d2 *)(d + 2) = uc[c].hi;
+        }
 #pragma unroll
         for (int i = 0; i < HC; i++) {
-            const int h = tid + NT * i, j = h < HP ? 0 : 1;
-            if (h < N) *(rx_d2 *)(ut + (j ? (LINES + 1) * N : 0) + 2 * (h - j * HP)) = hv[i] * (j ? sgp : sgm);
+            const int h = tid + NT * i, j = h < QR ? 0 : 1;
+            if (h < 2 * QR) {
+                double *d = ut + (j ? (LINES + 1) * N : 0) + 4 * (h - j * QR);
+                const rx_d4 v = hv[i] * (j ? sgp : sgm);
+                *(rx_d2 *)d = v.lo;
+                *(rx_d2 *)(d + 2) = v.hi;
+            }
         }
         for (int i = tid; i < N + LINES; i += NT) exs[i] = i < N ? r.dtab[i] : r.dtab[N + y0 + (i - N)];
     }
-    // z neighbours and the right-hand side of the own pairs, as raw loads issued PD pairs ahead
-    // of their use: each pair's arithmetic sits between scheduling barriers, so only the pairs
-    // in flight hold registers (all of them at once would need over 250)
+    // z neighbours and the right-hand side of the own quads, as raw loads issued PD quads ahead
+    // of their use: each quad's arithmetic sits between scheduling barriers, so only the quads
+    // in flight hold registers
     constexpr int PD = 2;
     struct Raw {
-        rx_f2 m0, m1, p0, p1;
-        rx_d2 f;
+        rx_f4 m0, m1, p0, p1;
+        rx_d2 f0, f1;
     };
-    Raw raw[PP];
+    Raw raw[QQ];
     auto issue = [&](int c) {
-        const long o = p0 + 2 * (tid + NT * c);
-        raw[c].m0 = *(gf2 *)(r.u0 + o + om);
-        raw[c].p0 = *(gf2 *)(r.u0 + o + op);
+        const long o = p0 + 4 * (tid + NT * c);
+        raw[c].m0 = *(gf4 *)(r.u0 + o + om);
+        raw[c].p0 = *(gf4 *)(r.u0 + o + op);
         if constexpr (SRC == RES_U0_U1) {
-            raw[c].m1 = *(gf2 *)(r.u1 + o + om);
-            raw[c].p1 = *(gf2 *)(r.u1 + o + op);
+            raw[c].m1 = *(gf4 *)(r.u1 + o + om);
+            raw[c].p1 = *(gf4 *)(r.u1 + o + op);
+        } else {
+            raw[c].m1 = raw[c].m0;
+            raw[c].p1 = raw[c].p0;
         }
-        raw[c].f = *(gd2 *)(r.f + o);
-    };
-    // (double)u0 + (double)u1, as ldu2
-    auto sum2 = [](rx_f2 a0, rx_f2 a1) -> rx_d2 {
-        rx_d2 v = {(double)a0.x, (double)a0.y};
-        if constexpr (SRC == RES_U0_U1) {
-            v.x += (double)a1.x;
-            v.y += (double)a1.y;
-        }
-        return v;
+        raw[c].f0 = *(gd2 *)(r.f + o);
+        raw[c].f1 = *(gd2 *)(r.f + o + 2);
     };
 #pragma unroll
-    for (int c = 0; c < PD && c < PP; c++) issue(c);
+    for (int c = 0; c < PD && c < QQ; c++) issue(c);
     __syncthreads();
 
-    // ---- residual per cell pair (the residual kernel's arithmetic) ----
+    // ---- residual per cell quad (the residual kernel's arithmetic) ----
     const double *ef = r.dtab + N + ny + nz;
     const int dm = r.dmask;
     const double ezz = r.dtab[N + ny + z];
@@ -178,58 +185,63 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
     // nz - 1, the latter held as u(z-1)); ng entries, one per row group
     const bool sums = r.upart && zlo, zwrap = r.bc[4] == 'P';
     double sp = 0.0;
-    rx_f2 rv[PP];
+    rx_f4 rv[QQ];
 #pragma unroll
-    for (int c = 0; c < PP; c++) {
-        if (c + PD < PP) issue(c + PD);
+    for (int c = 0; c < QQ; c++) {
+        if (c + PD < QQ) issue(c + PD);
         __builtin_amdgcn_sched_barrier(0);
-        const int q = 2 * (tid + NT * c), l2 = q / N, x = q - l2 * N, y = y0 + l2;
+        const int q = 4 * (tid + NT * c), l2 = q / N, x = q - l2 * N, y = y0 + l2;
         const double *row = ut + (l2 + 1) * N;
-        const rx_d2 u = *(const rx_d2 *)(row + x);
-        const rx_d2 umc = sum2(raw[c].m0, raw[c].m1) * sgzm, upc = sum2(raw[c].p0, raw[c].p1) * sgzp;
-        const rx_d2 fvc = raw[c].f;
-        if (sums) sp += (u.x + u.y) + (zwrap ? umc.x + umc.y : 0.0);
-        const rx_d2 uym = *(const rx_d2 *)(row - N + x), uyp = *(const rx_d2 *)(row + N + x);
+        const rx_d2 ua = *(const rx_d2 *)(row + x), ub = *(const rx_d2 *)(row + x + 2);
+        const rx_d2 ma = *(const rx_d2 *)(row - N + x), mb = *(const rx_d2 *)(row - N + x + 2);
+        const rx_d2 pa = *(const rx_d2 *)(row + N + x), pb = *(const rx_d2 *)(row + N + x + 2);
+        const double u[4] = {ua.x, ua.y, ub.x, ub.y};
+        const double uym[4] = {ma.x, ma.y, mb.x, mb.y}, uyp[4] = {pa.x, pa.y, pb.x, pb.y};
+        const rx_d4 umv = sum4(raw[c].m0, raw[c].m1) * sgzm, upv = sum4(raw[c].p0, raw[c].p1) * sgzp;
+        const double um[4] = {umv.x, umv.y, umv.z, umv.w}, up[4] = {upv.x, upv.y, upv.z, upv.w};
+        const double fv[4] = {raw[c].f0.x, raw[c].f0.y, raw[c].f1.x, raw[c].f1.y};
+        if (sums) sp += ((u[0] + u[1]) + (u[2] + u[3])) + (zwrap ? (um[0] + um[1]) + (um[2] + um[3]) : 0.0);
         // x neighbours: inside the row from the tile; at x = 0 and x = N - 1 the ghost closure
         // (P wraps within the row, N mirrors: the cell itself, D: its negative)
         const int il = x > 0 ? x - 1 : (bx0 == 'P' ? N - 1 : 0);
-        const int ir = x + 2 < N ? x + 2 : (bx1 == 'P' ? 0 : N - 1);
+        const int ir = x + 4 < N ? x + 4 : (bx1 == 'P' ? 0 : N - 1);
         const double uxl = (x > 0 ? 1.0 : gxm) * row[il];
-        const double uxr = (x + 2 < N ? 1.0 : gxp) * row[ir];
-        const double Lu0 = ((u.y - u.x) - (u.x - uxl)) * r.Cx + ((uyp.x - u.x) - (u.x - uym.x)) * r.Cy +
-                           ((upc.x - u.x) - (u.x - umc.x));
-        const double Lu1 = ((uxr - u.y) - (u.y - u.x)) * r.Cx + ((uyp.y - u.y) - (u.y - uym.y)) * r.Cy +
-                           ((upc.y - u.y) - (u.y - umc.y));
-        // Dirichlet terms, as the residual kernel forms them, with selects (no branch per pair):
-        // x and y faces a per-cell constant times ez[z], z faces a constant on the end planes
-        const double ex0 = exs[x], ex1 = exs[x + 1], eyc = eys[l2];
-        const bool fx0 = (dm & 1) && x == 0, fx1 = (dm & 2) && x + 1 == N - 1;
+        const double uxr = (x + 4 < N ? 1.0 : gxp) * row[ir];
+        // Dirichlet profiles of the quad's cells and row
+        const rx_d2 xa = *(const rx_d2 *)(exs + x), xb = *(const rx_d2 *)(exs + x + 2);
+        const double exv[4] = {xa.x, xa.y, xb.x, xb.y};
+        const double eyc = eys[l2];
         const bool fy0 = (dm & 4) && y == 0, fy1 = (dm & 8) && y == ny - 1;
-        rx_d2 cxy = {0.0, 0.0};
-        cxy.x = fx0 ? cxy.x + 2. * ef0 * eyc : cxy.x;
-        cxy.y = fx1 ? cxy.y + 2. * ef1 * eyc : cxy.y;
-        cxy.x = fy0 ? cxy.x + 2. * ex0 * ef2 : cxy.x;
-        cxy.y = fy0 ? cxy.y + 2. * ex1 * ef2 : cxy.y;
-        cxy.x = fy1 ? cxy.x + 2. * ex0 * ef3 : cxy.x;
-        cxy.y = fy1 ? cxy.y + 2. * ex1 * ef3 : cxy.y;
-        const rx_d2 czlo = (dm & 16) ? rx_d2{2. * ex0 * eyc * ef4, 2. * ex1 * eyc * ef4} : rx_d2{0.0, 0.0};
-        const rx_d2 czhi = (dm & 32) ? rx_d2{2 * ex0 * eyc * ef5, 2 * ex1 * eyc * ef5} : rx_d2{0.0, 0.0};
-        rx_d2 v = fvc;
-        v = v * r.c3;
-        v = v * r.c3;
-        v = v - cxy * ezz;
-        if (zlo) v = v - czlo;
-        if (zhi) v = v - czhi;
-        double res0 = v.x - Lu0, res1 = v.y - Lu1;
-        if (zlo) {
-            res0 += defect;
-            res1 += defect;
+        float rq[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double xl = i == 0 ? uxl : u[i - 1], xr = i == 3 ? uxr : u[i + 1];
+            const double Lu = ((xr - u[i]) - (u[i] - xl)) * r.Cx + ((uyp[i] - u[i]) - (u[i] - uym[i])) * r.Cy +
+                              ((up[i] - u[i]) - (u[i] - um[i]));
+            // Dirichlet terms, as the residual kernel forms them, with selects (no branch per
+            // cell): x and y faces a per-cell constant times ez[z], z faces a constant on the
+            // end planes
+            double cxy = 0.0;
+            if (i == 0) cxy = ((dm & 1) && x == 0) ? cxy + 2. * ef0 * eyc : cxy;
+            if (i == 3) cxy = ((dm & 2) && x + 3 == N - 1) ? cxy + 2. * ef1 * eyc : cxy;
+            cxy = fy0 ? cxy + 2. * exv[i] * ef2 : cxy;
+            cxy = fy1 ? cxy + 2. * exv[i] * ef3 : cxy;
+            const double czlo = (dm & 16) ? 2. * exv[i] * eyc * ef4 : 0.0;
+            const double czhi = (dm & 32) ? 2 * exv[i] * eyc * ef5 : 0.0;
+            double v = fv[i];
+            v = v * r.c3;
+            v = v * r.c3;
+            v = v - cxy * ezz;
+            if (zlo) v = v - czlo;
+            if (zhi) v = v - czhi;
+            double res = v - Lu;
+            if (zlo) res += defect;
+            rq[i] = (float)res;
         }
         // the result is pinned here (an empty asm that takes it in registers): otherwise the
-        // arithmetic sinks to the slice writes after the barrier and every pair's loads stay live
-        float r0 = (float)res0, r1 = (float)res1;
-        asm volatile("" : "+v"(r0), "+v"(r1));
-        rv[c] = rx_f2{r0, r1};
+        // arithmetic sinks to the slice writes after the barrier and every quad's loads stay live
+        asm volatile("" : "+v"(rq[0]), "+v"(rq[1]), "+v"(rq[2]), "+v"(rq[3]));
+        rv[c] = rx_f4{rq[0], rq[1], rq[2], rq[3]};
         __builtin_amdgcn_sched_barrier(0);
     }
     if (sums) {
@@ -247,10 +259,12 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
 
     // ---- r into the lines' slices, then each thread's inputs n = t + T c of its line ----
 #pragma unroll
-    for (int c = 0; c < PP; c++) {
-        const int q = 2 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
+    for (int c = 0; c < QQ; c++) {
+        const int q = 4 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
         lr(lds + l2 * LS, x) = rv[c].x;
         lr(lds + l2 * LS, x + 1) = rv[c].y;
+        lr(lds + l2 * LS, x + 2) = rv[c].z;
+        lr(lds + l2 * LS, x + 3) = rv[c].w;
     }
     __syncthreads();
     cpx<R> *L = lds + l * LS;

@@ -29,24 +29,25 @@ typedef float rx_f4 __attribute__((ext_vector_type(4)));
 template <int M, int LINES>
 constexpr size_t resx_lds()
 {
-    // the slices (overwriting the tile once it is read) never reach past the tile; the
-    // Dirichlet profiles ex (N) and ey (LINES rows) follow it
+    // the slices overwrite the tile once it is read (40 KB for 512-point lines: four blocks
+    // per CU); the plane-0 wave sums go to the tile's tail, which the slices never reach
     const size_t tile = (size_t)(LINES + 2) * 2 * M * sizeof(double);
     const size_t slices = (size_t)LINES * line_slots<M>() * sizeof(cpx<float>);
-    return (tile > slices ? tile : slices) + (size_t)(2 * M + LINES) * sizeof(double);
+    return tile > slices ? tile : slices;
 }
 
-// Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4),
-// at most 3 (<= 168 VGPRs; the quarter-wave kinds spill at 128): 3 for 512-point lines (44
-// KB per 256-thread block), 2 for 1024-point ones; at most 2 where a thread holds other than 8
-// elements (their transforms need more registers)
+// Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4):
+// 4 for 512- and 1024-point lines (40 and 48 KB per block, <= 128 VGPRs); at most 3 for shorter
+// lines (their second-step kernels spill at 128) and at most 2 where a thread holds other than
+// 8 elements (their transforms need more registers)
 template <int M, int T, int LINES>
 constexpr int resx_waves()
 {
     const int blocks = (int)((160 * 1024) / resx_lds<M, LINES>());
     int w = blocks * (T * LINES / 64) / 4;
     if ((M / T) != 8 && w > 2) w = 2;
-    return w < 1 ? 1 : (w > 3 ? 3 : w);
+    if (M < 256 && w > 3) w = 3;
+    return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
 
 template <int M, int T, int LINES, int KIND, int SRC>
@@ -63,9 +64,9 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<R> *lds = reinterpret_cast<cpx<R> *>(smem_raw);
     double *ut = reinterpret_cast<double *>(smem_raw);  // (LINES + 2) rows of N: rows y0-1 .. y0+LINES
-    constexpr size_t TILE_D = (resx_lds<M, LINES>() / sizeof(double)) - (N + LINES);
-    double *exs = ut + TILE_D, *eys = exs + N;          // Dirichlet profiles ex[0..N), ey[y0..y0+LINES)
-    __shared__ double red[NT / 64];
+    static_assert((size_t)LINES * line_slots<M>() + NT / 64 <= (size_t)(LINES + 2) * N,  // in doubles
+                  "the wave sums sit in the tile's tail, past the slices");
+    double *red = ut + (LINES + 2) * N - NT / 64;  // plane-0 wave sums, once the tile is read
     const int tid = (int)threadIdx.x, t = tid % T, l = tid / T;
     const int ny = r.ny, nz = r.nz, ng = ny / LINES;
     // Block numbering: with ng a multiple of 8, XCD k (blocks b = k mod 8) owns row groups
@@ -144,16 +145,17 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
                 *(rx_d2 *)(d + 2) = v.hi;
             }
         }
-        for (int i = tid; i < N + LINES; i += NT) exs[i] = i < N ? r.dtab[i] : r.dtab[N + y0 + (i - N)];
     }
     // z neighbours and the right-hand side of the own quads, as raw loads issued PD quads ahead
     // of their use: each quad's arithmetic sits between scheduling barriers, so only the quads
     // in flight hold registers
-    constexpr int PD = 2;
+    constexpr int PD = 1;
     struct Raw {
         rx_f4 m0, m1, p0, p1;
-        rx_d2 f0, f1;
+        rx_d2 f0, f1, x0, x1;  // f and, with Dirichlet faces, the ex profile of the quad
+        double ey;             // and ey of its row
     };
+    const int dm = r.dmask;
     Raw raw[QQ];
     auto issue = [&](int c) {
         const long o = p0 + 4 * (tid + NT * c);
@@ -168,6 +170,17 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
         }
         raw[c].f0 = *(gd2 *)(r.f + o);
         raw[c].f1 = *(gd2 *)(r.f + o + 2);
+        // Dirichlet profiles (cached table loads; every quad loads them when a face is D, so
+        // the loop has no branch)
+        if (dm) {
+            const int q = 4 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
+            raw[c].x0 = *(gd2 *)(r.dtab + x);
+            raw[c].x1 = *(gd2 *)(r.dtab + x + 2);
+            raw[c].ey = *(const __attribute__((address_space(1))) double *)(r.dtab + N + y0 + l2);
+        } else {
+            raw[c].x0 = raw[c].x1 = rx_d2{0.0, 0.0};
+            raw[c].ey = 0.0;
+        }
     };
 #pragma unroll
     for (int c = 0; c < PD && c < QQ; c++) issue(c);
@@ -175,7 +188,6 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
 
     // ---- residual per cell quad (the residual kernel's arithmetic) ----
     const double *ef = r.dtab + N + ny + nz;
-    const int dm = r.dmask;
     const double ezz = r.dtab[N + ny + z];
     const double ef0 = ef[0], ef1 = ef[1], ef2 = ef[2], ef3 = ef[3], ef4 = ef[4], ef5 = ef[5];
     const double defect = r.pin_defect ? r.pin_defect[0] : 0.0;
@@ -208,9 +220,8 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
         const double uxl = (x > 0 ? 1.0 : gxm) * row[il];
         const double uxr = (x + 4 < N ? 1.0 : gxp) * row[ir];
         // Dirichlet profiles of the quad's cells and row
-        const rx_d2 xa = *(const rx_d2 *)(exs + x), xb = *(const rx_d2 *)(exs + x + 2);
-        const double exv[4] = {xa.x, xa.y, xb.x, xb.y};
-        const double eyc = eys[l2];
+        const double exv[4] = {raw[c].x0.x, raw[c].x0.y, raw[c].x1.x, raw[c].x1.y};
+        const double eyc = raw[c].ey;
         const bool fy0 = (dm & 4) && y == 0, fy1 = (dm & 8) && y == ny - 1;
         float rq[4];
 #pragma unroll
@@ -244,12 +255,13 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
         rv[c] = rx_f4{rq[0], rq[1], rq[2], rq[3]};
         __builtin_amdgcn_sched_barrier(0);
     }
+    __syncthreads();  // the tile is read: the slices and the wave sums overwrite it
     if (sums) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off);
         if ((tid & 63) == 0) red[tid >> 6] = sp;
+        __syncthreads();
     }
-    __syncthreads();  // the tile is read (the slices overwrite it), the wave sums are in red
     if (sums && tid == 0) {
         double s = 0.0;
 #pragma unroll

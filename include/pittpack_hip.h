@@ -114,10 +114,11 @@ enum {
      * periodic axes, cells thinner in z).  Default 1e-3 (the columns where the two differ beyond
      * 1e-13); 0 = off. */
     PP_OPT_ZTHOMAS_TAU = 15,
-    /* mixed form, single handle: each refinement step computes the FP64 residual inside its
-     * FP32 x-forward transform (one kernel; the residual is never stored), where the x lines
-     * have a compiled plan and ny is a multiple of the kernel's lines per block (8 for 512-point
-     * lines); 0 = the residual kernel, then the transform.  Default 1. */
+    /* mixed form: each refinement step computes the FP64 residual inside its FP32 x-forward
+     * transform (one kernel; the residual is never stored), where the x lines have a compiled
+     * plan and ny is a multiple of the kernel's lines per block (8 for 512-point lines); 0 = the
+     * residual kernel, then the transform.  Default 1.  Batches: per member by the same rule,
+     * set before pp_batch_finalize. */
     PP_OPT_FUSE_RESIDUAL = 16
 };
 

@@ -344,7 +344,7 @@ class Batch:
 
     def __init__(self, precision: int = 0, refine: int | None = None, tau: float | None = None,
                  zero_mode: int | None = None, graph: int | None = None, autotune: int | None = None,
-                 line_variant: int | None = None, zthomas: float | None = None):
+                 line_variant: int | None = None, zthomas: float | None = None, fuse: int | None = None):
         self._L = load_library()
         self._h = ctypes.c_void_p()
         _check(self._L.pp_batch_create(ctypes.byref(self._h)), "pp_batch_create")
@@ -366,6 +366,8 @@ class Batch:
                    "batch_set_option_double")
         if zero_mode is not None:
             self.set_option(OPT_ZERO_MODE, zero_mode)
+        if fuse is not None:
+            self.set_option(OPT_FUSE_RESIDUAL, fuse)
 
     def add(self, nx: int, ny: int, nz: int, bc: str, box=None) -> int:
         X = np.ascontiguousarray(box if box is not None else [0, 1, 0, 1, 0, 1], dtype=np.float64)

@@ -20,6 +20,7 @@
 #include "pp_batch_core.h"
 #include "pp_launch.h"
 #include "pp_line_bs.h"
+#include "pp_resx.h"
 
 #define PPB_CHECK(x)                                  \
     do {                                              \
@@ -302,11 +303,15 @@ int BatchCore::plan_mixed()
             it.b0 = g.zlo_d ? -2.0 - 1. : -2.0 + 1.;
             it.bn = g.zhi_d ? -2.0 - 1. : -2.0 + 1.;
             it.inv_scale = 1. / g.scale;
-            it.nparts = residual_parts(g.nx, g.ny);
+            // partial sums of plane 0: the residual kernel's tiles, or one per row group of
+            // the fused kernel
+            it.nparts = mfused_[i] ? g.ny / resx_lines(g.nx) : residual_parts(g.nx, g.ny);
             it.zper = g.zper ? 1 : 0;
             m00_.push_back(it);
             m00_mem_.push_back((int)i);
         }
+        // fused members: their residual runs inside the corrections' x forward (lxc_)
+        if (mfused_[i]) continue;
         // the single-solve handle's residual kernel for this member's nx (results bitwise alike)
         const bool pair = residual_pairs(g.nx);
         if (pair) resp_two_ = resp_two_ || g.ny % residual_tile_y() == 1;
@@ -383,7 +388,27 @@ int BatchCore::plan()
     }
     int rc;
     if (precision == 1) {
-        if ((rc = plan_lines<float>(0, lx_)) || (rc = plan_lines<float>(1, ly_)) || (rc = plan_mixed())) return rc;
+        if ((rc = plan_lines<float>(0, lx_)) || (rc = plan_lines<float>(1, ly_))) return rc;
+        // the single-solve handle's rule for fusing a member's corrections
+        mfused_.assign(mem.size(), 0);
+        for (size_t i = 0; i < mem.size(); i++) {
+            const SolveSetup &g = mem[i].g;
+            const int lines = resx_lines(g.nx);
+            mfused_[i] = fuse_res && line_compiled(g.nx) && lines > 0 && g.ny % lines == 0;
+        }
+        lxc_.clear();
+        for (const HostClass &c : lx_)
+            for (int f = 1; f >= 0; f--) {
+                HostClass h;
+                h.key = c.key;
+                h.kind = c.kind;
+                h.fused = f != 0;
+                for (const LineItem &it : c.li)
+                    if ((mfused_[it.solve] != 0) == h.fused) h.li.push_back(it);
+                h.nitems = (int)h.li.size();
+                if (h.nitems) lxc_.push_back(std::move(h));
+            }
+        if ((rc = plan_mixed())) return rc;
     } else {
         if ((rc = plan_lines<double>(0, lx_)) || (rc = plan_lines<double>(1, ly_))) return rc;
     }
@@ -406,7 +431,7 @@ int BatchCore::plan()
     o_dtab_ = take(c, dtab_.size() * sizeof(double));
     o_ls_ = take(c, ls_.size() * sizeof(LineSolve));
     o_zs_ = take(c, zs_.size() * sizeof(ZSolve));
-    for (auto *v : {&lx_, &ly_})
+    for (auto *v : {&lx_, &ly_, &lxc_})
         for (HostClass &h : *v) {
             h.off = take(c, h.li.size() * sizeof(LineItem));
             if (!h.li_alt.empty()) h.off_alt = take(c, h.li_alt.size() * sizeof(LineItem));
@@ -446,7 +471,7 @@ int BatchCore::bind(char *base, TableCache *tc)
     d_dtab_ = (double *)(base + o_dtab_);
     d_ls_ = (LineSolve *)(base + o_ls_);
     d_zs_ = (ZSolve *)(base + o_zs_);
-    for (auto *v : {&lx_, &ly_, &lz_, &lzp_})
+    for (auto *v : {&lx_, &ly_, &lz_, &lzp_, &lxc_})
         for (HostClass &h : *v) {
             h.d_items = base + h.off;
             h.d_items_alt = h.li_alt.empty() ? nullptr : base + h.off_alt;
@@ -551,7 +576,7 @@ int BatchCore::upload(hipStream_t s)
     if ((rc = put(d_lam_, lam_, s)) || (rc = put(d_dtab_, dtab_, s)) || (rc = put(d_ls_, ls_, s)) ||
         (rc = put(d_zs_, zs_, s)))
         return rc;
-    for (auto *v : {&lx_, &ly_})
+    for (auto *v : {&lx_, &ly_, &lxc_})
         for (HostClass &h : *v)
             if ((rc = put((LineItem *)h.d_items, h.li, s)) ||
                 (!h.li_alt.empty() && (rc = put((LineItem *)h.d_items_alt, h.li_alt, s))))
@@ -848,7 +873,29 @@ int BatchCore::run_mixed(hipStream_t s, Marks *mk)
         PPB_CHECK(launch_residual_f32_batch(d_res_, d_ritems_, (int)ritems_.size(), res_two_, src, s));
         PPB_CHECK(launch_residual_pair_f32_batch(d_res_, d_rpitems_, (int)rpitems_.size(), resp_two_, src, s));
         if ((rc = mark(s, mk))) return rc;
-        if ((rc = run_line_stage(s, lx_, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0))) return rc;
+        // the corrections' x forward: fused members compute their residual inside it
+        for (const HostClass &c : lxc_) {
+            if (c.fused) {
+                const AxisDev &t = tabs_.at(c.key);
+                LineArgs a{};
+                a.in = d_w;
+                a.out = d_w;
+                a.tw = t.tw;
+                a.A = t.A;
+                a.B = t.B;
+                a.C = t.C;
+                a.D = t.D;
+                a.in_scale = 1.0;
+                a.out_scale = 1.0;
+                a.items = (const LineItem *)c.d_items;
+                a.solves = d_ls_;
+                a.nlines = c.nitems;
+                a.line_sums = sums;
+                PPB_CHECK(launch_resx(c.kind, c.key, src, a, ResArgs{}, d_res_, s));
+            } else if ((rc = launch_class(s, c, FORM_F32, 0, 1, d_w, d_w, 0, sums, nullptr, 0, 0))) {
+                return rc;
+            }
+        }
         if ((rc = mark(s, mk))) return rc;
         if ((rc = mixed_inner(s, mk))) return rc;
         if (nm00) PPB_CHECK(launch_mode00_batch(d_m00_, nm00, nz_max_, 1, s));
@@ -1095,6 +1142,11 @@ int pp_batch_set_option(pp_batch b, int key, long long value)
         if (value != 0 && value != 1) return PP_INPUT_ARGS;
         if (b->finalized) return PP_CALL_ORDER;
         b->autotune = (int)value;
+        return PP_SUCCESS;
+    case PP_OPT_FUSE_RESIDUAL:
+        if (value != 0 && value != 1) return PP_INPUT_ARGS;
+        if (b->finalized) return PP_CALL_ORDER;
+        b->core.fuse_res = (int)value;
         return PP_SUCCESS;
     case PP_OPT_LINE_VARIANT:
         if (value < -1 || value > 1) return PP_INPUT_ARGS;

@@ -58,6 +58,7 @@ class BatchCore {
     double tau = 0.05;
     double zthomas_tau = 1e-3;  // FP64: columns below it by the reference's Thomas sweeps (0: off)
     int force_var = -1;  // FP64 line classes: -1 tuned / default shapes, 0 default, 1 alternative
+    int fuse_res = 1;    // mixed form: corrections fused into the x forward where a member's rows allow
 
     std::vector<BatchMember> mem;
 
@@ -101,6 +102,7 @@ class BatchCore {
         std::vector<LineItem> li_alt;
         size_t off_alt = 0;
         void *d_items_alt = nullptr;
+        bool fused = false;  // correction x classes: members whose residual runs in k_resx
     };
     int launch_class(hipStream_t s, const HostClass &c, int form, int axis, int fwd, const void *in, void *out,
                      int bflags, double *sums, const double *plane, int accumulate, int ztiled,
@@ -122,6 +124,11 @@ class BatchCore {
     std::vector<LineSolve> ls_;
     std::vector<ZSolve> zs_;
     std::vector<HostClass> lx_, ly_, lz_;
+    // mixed form: the corrections' x forward, each x class split into the members whose
+    // residual is computed inside it (k_resx: ny a multiple of its lines per block, as the
+    // single-solve handle decides) and the rest (residual kernel, then the FP32 transform)
+    std::vector<HostClass> lxc_;
+    std::vector<char> mfused_;
     // periodic z: one-pass classes (key nz, a compiled complex plan; items {member, column
     // block, y row}), members of other lengths (real z-line transforms, one launch each), the
     // singular columns of FP64 members, and 2 cos(2 pi k / nz) per distinct periodic length

@@ -5,5 +5,6 @@
 
 namespace pp {
 template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
-template hipError_t launch_resx_kind<KIND_DN>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+template hipError_t launch_resx_kind<KIND_DN>(int, int, const LineArgs &, const ResArgs &, const ResArgs *,
+                                                  hipStream_t);
 }  // namespace pp

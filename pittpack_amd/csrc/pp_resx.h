@@ -38,20 +38,21 @@ constexpr size_t resx_lds()
 
 // Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4):
 // 4 for 512- and 1024-point lines (40 and 48 KB per block, <= 128 VGPRs); at most 3 for shorter
-// lines (their second-step kernels spill at 128) and at most 2 where a thread holds other than
-// 8 elements (their transforms need more registers)
-template <int M, int T, int LINES>
+// lines and for the batched form (their second-step kernels spill at 128) and at most 2 where a
+// thread holds other than 8 elements (their transforms need more registers)
+template <int M, int T, int LINES, bool BATCH>
 constexpr int resx_waves()
 {
     const int blocks = (int)((160 * 1024) / resx_lds<M, LINES>());
     int w = blocks * (T * LINES / 64) / 4;
     if ((M / T) != 8 && w > 2) w = 2;
-    if (M < 256 && w > 3) w = 3;
+    if ((M < 256 || BATCH) && w > 3) w = 3;
     return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
 
-template <int M, int T, int LINES, int KIND, int SRC>
-__global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(LineArgs a, ResArgs r)
+template <int M, int T, int LINES, int KIND, int SRC, bool BATCH>
+__global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES, BATCH>())) k_resx(LineArgs a0, ResArgs r0,
+                                                                                       const ResArgs *rsolves)
 {
     using R = float;
     constexpr int N = 2 * M, E = M / T, LS = line_slots<M>(), NT = T * LINES;
@@ -68,20 +69,34 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
                   "the wave sums sit in the tile's tail, past the slices");
     double *red = ut + (LINES + 2) * N - NT / 64;  // plane-0 wave sums, once the tile is read
     const int tid = (int)threadIdx.x, t = tid % T, l = tid / T;
-    const int ny = r.ny, nz = r.nz, ng = ny / LINES;
-    // Block numbering: with ng a multiple of 8, XCD k (blocks b = k mod 8) owns row groups
-    // k*ng/8 .. (k+1)*ng/8 - 1 of every plane and takes them plane by plane, so the z- and
-    // y-neighbour rows a block reads were read by blocks that ran on the same XCD just before
-    // (its own L2); otherwise groups in plane order
+    LineArgs a = a0;
+    ResArgs r = r0;
     int g, z;
-    if (ng % 8 == 0) {
-        const int gr = ng / 8, k = (int)blockIdx.x % 8, s = (int)blockIdx.x / 8;
-        z = s / gr;
-        g = k * gr + s % gr;
+    if constexpr (BATCH) {
+        // batched: block b works on items[b] = {member, x-line block, 0}; the member's line
+        // fields resolve into a (line_block) and its residual arguments come from rsolves
+        int ib, jb;
+        line_block<float, float, float, LINES, 0>(a0, a, ib, jb);
+        r = rsolves[a0.items[blockIdx.x].solve];
+        const int ngm = r.ny / LINES;
+        z = ib / ngm;
+        g = ib % ngm;
     } else {
-        z = (int)blockIdx.x / ng;
-        g = (int)blockIdx.x % ng;
+        // Block numbering: with ng a multiple of 8, XCD k (blocks b = k mod 8) owns row groups
+        // k*ng/8 .. (k+1)*ng/8 - 1 of every plane and takes them plane by plane, so the z- and
+        // y-neighbour rows a block reads were read by blocks that ran on the same XCD just
+        // before (its own L2); otherwise groups in plane order
+        const int ngs = r.ny / LINES;
+        if (ngs % 8 == 0) {
+            const int gr = ngs / 8, k = (int)blockIdx.x % 8, s = (int)blockIdx.x / 8;
+            z = s / gr;
+            g = k * gr + s % gr;
+        } else {
+            z = (int)blockIdx.x / ngs;
+            g = (int)blockIdx.x % ngs;
+        }
     }
+    const int ny = r.ny, nz = r.nz, ng = ny / LINES;
     const int y0 = g * LINES, bx = z * ng + g;
     const long sy = N, sz = (long)N * ny;
     const long p0 = (long)z * sz + (long)y0 * sy;  // first cell of the block's rows
@@ -289,40 +304,50 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES>())) k_resx(
 }
 
 template <int M, int KIND, int SRC>
-hipError_t launch_resx_cfg(const LineArgs &a, const ResArgs &r, hipStream_t s)
+hipError_t launch_resx_cfg(const LineArgs &a, const ResArgs &r, const ResArgs *rsolves, hipStream_t s)
 {
     constexpr int T = line_threads(M, 0);
     if constexpr (T == 0) {
         return hipErrorInvalidValue;
     } else {
         constexpr int LINES = line_lines<float, M, 0>();
-        if (r.nx != 2 * M || r.ny % LINES != 0 || a.items || a.nlines != (long)r.ny * r.nz) return hipErrorInvalidValue;
+        // batched (a.items, a.nlines items): every member's nx = 2M and ny % LINES == 0 is the
+        // planner's to ensure; one solve: checked here
+        if (!a.items && (r.nx != 2 * M || r.ny % LINES != 0 || a.nlines != (long)r.ny * r.nz)) return hipErrorInvalidValue;
+        if (a.items && !rsolves) return hipErrorInvalidValue;
         constexpr size_t lds = resx_lds<M, LINES>();
-        auto kern = k_resx<M, T, LINES, KIND, SRC>;
+        // one solve and the batched form (members resolved per block) are separate kernels, so
+        // the single solve's kernel carries no per-member indirection
+        const int bat = a.items ? 1 : 0;
+        auto kern = bat ? k_resx<M, T, LINES, KIND, SRC, true> : k_resx<M, T, LINES, KIND, SRC, false>;
         if constexpr (lds > 64 * 1024) {
-            static bool attr_set = false;
-            if (!attr_set) {
+            static bool attr_set[2] = {false, false};
+            if (!attr_set[bat]) {
                 hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 if (e != hipSuccess) return e;
-                attr_set = true;
+                attr_set[bat] = true;
             }
         }
-        const unsigned blocks = (unsigned)((long)(r.ny / LINES) * r.nz);
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * LINES), lds, s, a, r);
+        const unsigned blocks = bat ? (unsigned)a.nlines : (unsigned)((long)(r.ny / LINES) * r.nz);
+        if (blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * LINES), lds, s, a, r, rsolves);
         return hipGetLastError();
     }
 }
 
 // the fused correction step of one x transform kind (instantiated in pp_line_<kind>.hip):
 // a is the FP32 x-forward's arguments (out: the work field, line sums), r the residual's
-// (src: RES_U0 or RES_U0_U1)
+// (src: RES_U0 or RES_U0_U1); batched (a.items): rsolves holds each member's residual
+// arguments (device memory) and r is unused
 template <int KIND>
-hipError_t launch_resx_kind(int N, int src, const LineArgs &a, const ResArgs &r, hipStream_t s)
+hipError_t launch_resx_kind(int N, int src, const LineArgs &a, const ResArgs &r, const ResArgs *rsolves,
+                            hipStream_t s)
 {
     if (src != RES_U0 && src != RES_U0_U1) return hipErrorInvalidValue;
     switch (N / 2) {
 #define PPM(m) \
-    case m: return src == RES_U0 ? launch_resx_cfg<m, KIND, RES_U0>(a, r, s) : launch_resx_cfg<m, KIND, RES_U0_U1>(a, r, s);
+    case m: return src == RES_U0 ? launch_resx_cfg<m, KIND, RES_U0>(a, r, rsolves, s) \
+                                 : launch_resx_cfg<m, KIND, RES_U0_U1>(a, r, rsolves, s);
         PP_FFT_LENGTHS(PPM)
 #undef PPM
     default: return hipErrorInvalidValue;
@@ -342,5 +367,9 @@ inline int resx_lines(int N)
     default: return 0;
     }
 }
+
+// dispatch over the x transform kind (pp_solver.hip)
+hipError_t launch_resx(int kind, int N, int src, const LineArgs &a, const ResArgs &r, const ResArgs *rsolves,
+                       hipStream_t s);
 
 }  // namespace pp

@@ -48,11 +48,11 @@ extern template hipError_t launch_line_kind<KIND_PP>(int, int, int, int, const L
 extern template hipError_t launch_line_kind<KIND_ND>(int, int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_kind<KIND_DN>(int, int, int, int, const LineArgs &, hipStream_t);
 
-extern template hipError_t launch_resx_kind<KIND_DD>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
-extern template hipError_t launch_resx_kind<KIND_NN>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
-extern template hipError_t launch_resx_kind<KIND_PP>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
-extern template hipError_t launch_resx_kind<KIND_ND>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
-extern template hipError_t launch_resx_kind<KIND_DN>(int, int, const LineArgs &, const ResArgs &, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_DD>(int, int, const LineArgs &, const ResArgs &, const ResArgs *, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_NN>(int, int, const LineArgs &, const ResArgs &, const ResArgs *, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_PP>(int, int, const LineArgs &, const ResArgs &, const ResArgs *, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_ND>(int, int, const LineArgs &, const ResArgs &, const ResArgs *, hipStream_t);
+extern template hipError_t launch_resx_kind<KIND_DN>(int, int, const LineArgs &, const ResArgs &, const ResArgs *, hipStream_t);
 
 extern template hipError_t launch_line_bs_kind<KIND_DD>(int, int, int, const LineArgs &, hipStream_t);
 extern template hipError_t launch_line_bs_kind<KIND_NN>(int, int, int, const LineArgs &, hipStream_t);
@@ -89,14 +89,15 @@ hipError_t launch_line(int form, int kind, int N, int AX, int FWD, const LineArg
 
 // the mixed form's fused correction step (pp_resx.h): residual of the current solution (src:
 // RES_U0 or RES_U0_U1) into the FP32 x-forward transform of kind `kind`
-hipError_t launch_resx(int kind, int N, int src, const LineArgs &a, const ResArgs &r, hipStream_t s)
+hipError_t launch_resx(int kind, int N, int src, const LineArgs &a, const ResArgs &r, const ResArgs *rsolves,
+                       hipStream_t s)
 {
     switch (kind) {
-    case KIND_DD: return launch_resx_kind<KIND_DD>(N, src, a, r, s);
-    case KIND_NN: return launch_resx_kind<KIND_NN>(N, src, a, r, s);
-    case KIND_PP: return launch_resx_kind<KIND_PP>(N, src, a, r, s);
-    case KIND_ND: return launch_resx_kind<KIND_ND>(N, src, a, r, s);
-    case KIND_DN: return launch_resx_kind<KIND_DN>(N, src, a, r, s);
+    case KIND_DD: return launch_resx_kind<KIND_DD>(N, src, a, r, rsolves, s);
+    case KIND_NN: return launch_resx_kind<KIND_NN>(N, src, a, r, rsolves, s);
+    case KIND_PP: return launch_resx_kind<KIND_PP>(N, src, a, r, rsolves, s);
+    case KIND_ND: return launch_resx_kind<KIND_ND>(N, src, a, r, rsolves, s);
+    case KIND_DN: return launch_resx_kind<KIND_DN>(N, src, a, r, rsolves, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -986,7 +987,7 @@ int run_mixed(pp_solver *h, const double *f, double *u)
         a.line_sums = h->pinned00 ? h->d_line_sums : nullptr;
         if (fused) {
             // residual computed inside the x forward: one kernel, r never stored
-            PP_HIP_CHECK(launch_resx(h->g.kind[0], h->g.nx, src, a, r, s));
+            PP_HIP_CHECK(launch_resx(h->g.kind[0], h->g.nx, src, a, r, nullptr, s));
         } else {
             PP_HIP_CHECK(launch_residual_f32(r, src, s));
             if ((rc = mark(h))) return rc;

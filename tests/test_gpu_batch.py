@@ -340,3 +340,56 @@ def test_batch_autotune_vs_oracle():
         _batch_vs_oracle(B, "autotuned")
         n = B.tuning()
         assert 0 <= n <= 2 * (B.launch_count()[0] + B.launch_count()[1])
+
+
+# mixed-form batch members whose corrections are fused into the x forward (ny a multiple of the
+# fused kernel's lines per block: 64 for 32-point lines, 32 for 128, 16 for 256, 8 for 512)
+# next to members that are not (ny = 48, 20, 12): same classes split into fused and plain parts
+FUSED_MEMBERS = [
+    (32, 64, 6, "DDNDDD", [0.0, 1.0, 0.0, 2.0, 0.0, 0.1875]),
+    (32, 48, 5, "DDNDDD", [0.0, 1.0, 0.0, 1.5, 0.0, 0.15625]),
+    (128, 32, 5, "PPPPNN", [0.0, 1.0, 0.0, 0.25, 0.0, 0.0390625]),
+    (128, 20, 4, "PPPPNN", [0.0, 1.0, 0.0, 0.15625, 0.0, 0.03125]),
+    (256, 16, 4, "NNNNNN", [0.0, 1.0, 0.0, 0.0625, 0.0, 0.015625]),
+    (256, 12, 4, "NDDNDN", [0.0, 1.0, 0.0, 0.046875, 0.0, 0.015625]),
+    (512, 8, 4, "DDDDDD", [0.0, 1.0, 0.0, 0.015625, 0.0, 0.0078125]),
+    (192, 32, 5, "NNNNPP", [0.0, 1.0, 0.0, 1.0 / 6, 0.0, 5.0 / 192]),
+]
+
+
+def test_batch_fused_corrections():
+    """the batched mixed form with its corrections fused per member (the single-solve handle's
+    rule): each member within 1e-14 of the single handle and at the FP64 bar against the oracle
+    (zero_mode 1, the library's default), and within 1e-13 of the same batch with the fusion
+    off (PP_OPT_FUSE_RESIDUAL = 0 before finalize); a graph replay equals the eager solve"""
+    print("batch fused corrections", flush=True)
+    rhs = [prng.uniform_pm1((nz, ny, nx), seed=900 + i) for i, (nx, ny, nz, _, _) in enumerate(FUSED_MEMBERS)]
+    out = {}
+    for fuse in (1, 0):
+        with pa.Batch(precision=1, fuse=fuse) as B:
+            for nx, ny, nz, bc, box in FUSED_MEMBERS:
+                B.add(nx, ny, nz, bc, box)
+            B.finalize()
+            for i, f in enumerate(rhs):
+                B.set_rhs(i, f)
+            B.solve()
+            first = [B.get_solution(i) for i in range(len(rhs))]
+            B.solve()  # captured into the graph
+            B.solve()  # replayed
+            for i in range(len(rhs)):
+                assert np.array_equal(B.get_solution(i), first[i]), f"fuse={fuse} member {i}: replay != eager"
+            out[fuse] = first
+    for i, (nx, ny, nz, bc, box) in enumerate(FUSED_MEMBERS):
+        got = out[1][i]
+        single = pa.solve(rhs[i], bc, box, precision=1)
+        assert rel_maxdiff(got, single) <= 1e-14, f"member {i}: batched vs single {rel_maxdiff(got, single):.2e}"
+        plain = out[0][i]
+        assert rel_maxdiff(got, plain) <= 1e-13, f"member {i}: fused vs unfused {rel_maxdiff(got, plain):.2e}"
+        ref = oracle.solve(rhs[i], bc, box, zero_mode=1)
+        bar = max(TOL, 1.01 * rel_maxdiff(plain, ref))
+        assert rel_maxdiff(got, ref) <= bar, f"member {i} {bc}: {rel_maxdiff(got, ref):.2e}"
+    with pa.Batch(precision=1) as B:
+        B.add(32, 32, 4, "PPPPNN")
+        B.finalize()
+        with pytest.raises(pa.PittPackError):
+            B.set_option(pa.OPT_FUSE_RESIDUAL, 0)  # after finalize

@@ -37,9 +37,10 @@ constexpr size_t resx_lds()
 }
 
 // Register bound: the waves per SIMD the LDS tile allows (blocks per CU x waves per block / 4):
-// 4 for 512- and 1024-point lines (40 and 48 KB per block, <= 128 VGPRs); at most 3 for shorter
-// lines and for the batched form (their second-step kernels spill at 128) and at most 2 where a
-// thread holds other than 8 elements (their transforms need more registers)
+// 4 for 512-point lines (40 KB per 256-thread block, <= 128 VGPRs), 3 for 1024-point ones (48
+// KB); at most 3 for shorter lines and for the batched form (their second-step kernels spill at
+// 128) and at most 2 where a thread holds other than 8 elements (their transforms need more
+// registers)
 template <int M, int T, int LINES, bool BATCH>
 constexpr int resx_waves()
 {

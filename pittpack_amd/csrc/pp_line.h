@@ -422,7 +422,7 @@ __device__ __forceinline__ void line_load(const LineArgs &a, const LineAddr &ad,
 
 // Transform one line (column) whose inputs are loaded, and store it: L is the line's LDS
 // slice
-template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND>
+template <typename TIn, typename TOut, typename R, int M, int T, int LINES, int AX, int FWD, int KIND, bool PRE = false>
 __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad, int t, cpx<R> *L,
                                           line_xt<TIn> (&xin)[2 * (M / T)], cpx<R> *lds, long line0)
 {
@@ -527,51 +527,55 @@ __device__ __forceinline__ void line_body(const LineArgs &a, const LineAddr &ad,
         }
     }
 
-    // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
-    // no table load sits in the element loop except ex[n] on face lines
-    const double *ex = a.dtab;
-    if constexpr (AX == 0 && FWD) {
-        const DirLine dl = dirichlet_line(a, ad.ly, lz);
-        const double cyz = dl.cyz, cx0 = dl.cx0, cxn = dl.cxn;
-#pragma unroll
-        for (int c = 0; c < 2 * E; c++) {
-            xin[c] = (XT)(xin[c] * a.in_scale);  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
-            xin[c] = (XT)(xin[c] * a.in_scale);
+    // PRE: the caller has placed the inputs at their load positions in the line's slice (and
+    // passed a barrier), so this load-side processing is skipped
+    if constexpr (!PRE) {
+        // ---- fused input scaling / Dirichlet boundary term, scatter into LDS ----
+        // no table load sits in the element loop except ex[n] on face lines
+        const double *ex = a.dtab;
+        if constexpr (AX == 0 && FWD) {
+            const DirLine dl = dirichlet_line(a, ad.ly, lz);
+            const double cyz = dl.cyz, cx0 = dl.cx0, cxn = dl.cxn;
+    #pragma unroll
+            for (int c = 0; c < 2 * E; c++) {
+                xin[c] = (XT)(xin[c] * a.in_scale);  // rhs * c3 * c3 (src/pencilDcmp.cpp:5639)
+                xin[c] = (XT)(xin[c] * a.in_scale);
+            }
+            // one branch per thread, not per element: y/z-face lines, then the x-face elements
+            // n = 0 (thread 0, c = 0) and n = N - 1 (thread T - 1, c = 2E - 1)
+            if (cyz != 0.0) {
+    #pragma unroll
+                for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] - gld(ex, t + T * c) * cyz);
+            }
+            if (t == 0) xin[0] = (XT)(xin[0] - cx0);
+            if (t == T - 1) xin[2 * E - 1] = (XT)(xin[2 * E - 1] - cxn);
+        } else {
+    #pragma unroll
+            for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] * a.in_scale);
         }
-        // one branch per thread, not per element: y/z-face lines, then the x-face elements
-        // n = 0 (thread 0, c = 0) and n = N - 1 (thread T - 1, c = 2E - 1)
-        if (cyz != 0.0) {
-#pragma unroll
-            for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] - gld(ex, t + T * c) * cyz);
+        double lsum = 0.0;
+        if constexpr (!DIRECT_IN) {
+    #pragma unroll
+            for (int c = 0; c < 2 * E; c++) {
+                const int n = t + T * c;
+                const double v = xin[c];
+                lsum += v;
+                bool neg;
+                const int pos = load_pos<N, KIND, FWD>(n, neg);
+                lr(L, pos) = neg ? (R)(-v) : (R)v;
+            }
         }
-        if (t == 0) xin[0] = (XT)(xin[0] - cx0);
-        if (t == T - 1) xin[2 * E - 1] = (XT)(xin[2 * E - 1] - cxn);
-    } else {
-#pragma unroll
-        for (int c = 0; c < 2 * E; c++) xin[c] = (XT)(xin[c] * a.in_scale);
+        if constexpr (AX == 0 && FWD) {
+            if (a.line_sums) {
+                // the T threads of a line are adjacent lanes of one wave: butterfly reduction
+    #pragma unroll
+                for (int off = T / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, T);
+                if (t == 0 && valid) a.line_sums[(long)ad.lz * a.ny + ad.ly] = lsum;
+            }
+        }
+        (void)lsum;
+        if constexpr (!DIRECT_IN) __syncthreads();
     }
-    double lsum = 0.0;
-    if constexpr (!DIRECT_IN) {
-#pragma unroll
-        for (int c = 0; c < 2 * E; c++) {
-            const int n = t + T * c;
-            const double v = xin[c];
-            lsum += v;
-            bool neg;
-            const int pos = load_pos<N, KIND, FWD>(n, neg);
-            lr(L, pos) = neg ? (R)(-v) : (R)v;
-        }
-    }
-    if constexpr (AX == 0 && FWD) {
-        if (a.line_sums) {
-            // the T threads of a line are adjacent lanes of one wave: butterfly reduction
-#pragma unroll
-            for (int off = T / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, T);
-            if (t == 0 && valid) a.line_sums[(long)ad.lz * a.ny + ad.ly] = lsum;
-        }
-    }
-    (void)lsum;
-    if constexpr (!DIRECT_IN) __syncthreads();
 
     // ---- pre-processing: FFT input in registers, layout k = t + T*m ----
     cpx<R> v[E];

@@ -285,7 +285,32 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES, BATCH>())) 
         r.upart[g] = s;
     }
 
-    // ---- r into the lines' slices, then each thread's inputs n = t + T c of its line ----
+    const LineAddr ad = line_addr<LINES, 0>(a, N, bx, 0, l);
+    cpx<R> *L = lds + l * LS;
+    line_xt<float> xin[2 * E];
+    // no line sums wanted: r goes straight to the transform's input positions in the slices
+    // (the scatter line_body would do), and the transform starts from there (first step 0.52
+    // -> 0.48 ms at 512^3); single-solve kernels of 8 elements per thread only, where the
+    // second transform path still fits the register bound
+    constexpr bool DIRECT = (M / T == 8) && !BATCH;
+    if (DIRECT && !a.line_sums) {
+#pragma unroll
+        for (int c = 0; c < QQ; c++) {
+            const int q = 4 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
+            const float rq[4] = {rv[c].x, rv[c].y, rv[c].z, rv[c].w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                bool neg;
+                const int pos = load_pos<N, KIND, 1>(x + i, neg);
+                lr(lds + l2 * LS, pos) = neg ? -rq[i] : rq[i];
+            }
+        }
+        __syncthreads();
+        if constexpr (DIRECT) line_body<float, float, float, M, T, LINES, 0, 1, KIND, true>(a, ad, t, L, xin, lds, (long)bx * LINES);
+        return;
+    }
+    // ---- r into the lines' slices, then each thread's inputs n = t + T c of its line (whose
+    // line sums the transform forms) ----
 #pragma unroll
     for (int c = 0; c < QQ; c++) {
         const int q = 4 * (tid + NT * c), l2 = q / N, x = q - l2 * N;
@@ -295,12 +320,9 @@ __global__ void __launch_bounds__(T *LINES, (resx_waves<M, T, LINES, BATCH>())) 
         lr(lds + l2 * LS, x + 3) = rv[c].w;
     }
     __syncthreads();
-    cpx<R> *L = lds + l * LS;
-    line_xt<float> xin[2 * E];
 #pragma unroll
     for (int c = 0; c < 2 * E; c++) xin[c] = lr(L, t + T * c);
     __syncthreads();  // the slices are rewritten by the pre-processing next
-    const LineAddr ad = line_addr<LINES, 0>(a, N, bx, 0, l);
     line_body<float, float, float, M, T, LINES, 0, 1, KIND>(a, ad, t, L, xin, lds, (long)bx * LINES);
 }
 
